@@ -1,0 +1,217 @@
+/*
+ * statecraft_hip.h -- C-ABI of the MI355X (gfx950) statevector engine.
+ *
+ * This is the drop-in boundary for StateCraft's dense `statevector` engine.
+ * A host engine (the Java FFM adapter `HipStatevectorEngine`, the C++ mirror in
+ * statecraft_amd/engine, or the Python ctypes mirror) lowers a QuantumCircuit
+ * into a flat array of `sc_op` records plus one array of complex matrix
+ * entries, and calls these functions.  Every function is synchronous from the
+ * caller's point of view, takes plain pointers and sizes, and never retains a
+ * host pointer past the call.
+ *
+ * Reference interfaces replaced (paths relative to the StateCraft tree):
+ *   SimulatorEngine.simulate           core/src/main/java/com/omaarr90/statecraft/core/engine/SimulatorEngine.java:3-9
+ *   StatevectorEngine.simulate op loop engines/src/main/java/com/omaarr90/statecraft/engines/statevector/StatevectorEngine.java:63-167
+ *   StatevectorOps.apply*              engines/.../statevector/StatevectorOps.java:19-747
+ *
+ * Error convention.  Every function returns an int status.  The reference
+ * throws Java exceptions; the host maps the status to the same class:
+ *   SC_E_NULL        -> NullPointerException
+ *   SC_E_ARG         -> IllegalArgumentException
+ *   SC_E_UNSUPPORTED -> UnsupportedOperationException
+ *   SC_E_STATE       -> IllegalStateException
+ *   SC_E_OOM / SC_E_HIP -> UnsupportedOperationException / IllegalStateException
+ * sc_last_error() returns the message of the last failure on the calling thread.
+ *
+ * State layout in HBM: 2^n complex128 amplitudes, interleaved (re, im), index
+ * bit q = qubit q (little-endian), exactly the reference's double[] layout
+ * (docs/core/statevector-layout.md:13-31), addressed with 64-bit indices so a
+ * state may hold up to 34 qubits (256 GiB).
+ */
+#ifndef STATECRAFT_HIP_H
+#define STATECRAFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define SC_OK 0
+#define SC_E_NULL 1
+#define SC_E_ARG 2
+#define SC_E_UNSUPPORTED 3
+#define SC_E_STATE 4
+#define SC_E_OOM 5
+#define SC_E_HIP 6
+
+/* ---- op kinds: one per sealed QuantumCircuit.Operation record ---------------
+ * QuantumCircuit.java:362-574.  Field meaning per kind:
+ *   SC_OP_SINGLE  SingleGateOperation(gate, qubit)        q0=qubit, mat -> 4 complex (row-major 2x2)
+ *   SC_OP_CNOT    CnotOperation(control, target)          q0=control, q1=target
+ *   SC_OP_TWO     TwoQubitGateOperation(matrix, first, second)
+ *                                                         q0=first, q1=second, mat -> 16 complex;
+ *                                                         local index l = 2*bit(first) + bit(second)
+ *   SC_OP_DIAG2   TwoQubitDiagonalOperation(diag, first, second)
+ *                                                         q0=first, q1=second, mat -> 4 complex
+ *   SC_OP_SWAP    SwapOperation(first, second)            q0=first, q1=second
+ *   SC_OP_MCU     MultiControlOperation(gate, target, controls)
+ *                                                         q0=target, controls=bit mask, mat -> 4 complex
+ *   SC_OP_MEASURE MeasureOperation(qubits)                controls=bit mask of measured qubits
+ * Gradient metadata (no reference counterpart): `param` is the index of the
+ * circuit parameter the gate depends on (-1 for none) and the gate equals
+ * exp(-i * theta * coeff * G) for the Pauli generator `gen` on its qubits, so
+ * dU/dtheta = -i * coeff * G * U.
+ */
+#define SC_OP_SINGLE 1
+#define SC_OP_CNOT 2
+#define SC_OP_TWO 3
+#define SC_OP_DIAG2 4
+#define SC_OP_SWAP 5
+#define SC_OP_MCU 6
+#define SC_OP_MEASURE 7
+
+#define SC_GEN_NONE 0
+#define SC_GEN_X 1  /* X on q0 */
+#define SC_GEN_Y 2  /* Y on q0 */
+#define SC_GEN_Z 3  /* Z on q0 */
+#define SC_GEN_ZZ 4 /* Z(q0) Z(q1) */
+
+typedef struct sc_op {
+    int32_t kind;      /* SC_OP_* */
+    int32_t q0;        /* first qubit field of the reference record */
+    int32_t q1;        /* second qubit field, -1 when absent */
+    int32_t mat;       /* offset, in complex entries, into the `mats` array; -1 when unused */
+    uint64_t controls; /* SC_OP_MCU control mask / SC_OP_MEASURE qubit mask */
+    int32_t param;     /* gradient parameter index, -1 for none */
+    int32_t gen;       /* SC_GEN_* */
+    double coeff;      /* generator coefficient */
+} sc_op;
+
+/* ---- execution options (sc_ctx_set_option) -------------------------------- */
+#define SC_OPT_EXEC_MODE 1      /* SC_EXEC_*; default SC_EXEC_SWEEP */
+#define SC_OPT_MAX_FUSED 2      /* max qubits of a dense fused block (1..5), default 4 */
+#define SC_OPT_PROFILE 3        /* 1: time every sweep/gate launch with HIP events */
+#define SC_OPT_LOW_QUBITS 4     /* qubits always resident in a sweep chunk for coalescing (0..8), default 4 */
+#define SC_OPT_TEST_DRIFT 5     /* test hook: adds value*1e-12 to psi[0] at every adjoint checkpoint, default 0 */
+
+#define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
+#define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
+#define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
+
+typedef struct sc_ctx sc_ctx;
+typedef struct sc_state sc_state;
+
+/* ---- library / context ---------------------------------------------------- */
+const char *sc_last_error(void);
+int sc_version(void);
+/* StatevectorEngine(StatevectorExecutionConfig) StatevectorEngine.java:46-55: one context = one device + one stream. */
+int sc_ctx_create(int device, sc_ctx **out);
+int sc_ctx_destroy(sc_ctx *ctx);
+int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value);
+int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value);
+int sc_ctx_synchronize(sc_ctx *ctx);
+/* Device facts for reporting: HBM bytes free/total, CU count. */
+int sc_ctx_device_info(sc_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, int *cu_count);
+
+/* ---- state lifetime / init ------------------------------------------------ */
+/* new double[2^(n+1)] StatevectorEngine.java:68-69; batch>1 allocates `batch` independent states. */
+int sc_state_create(sc_ctx *ctx, int num_qubits, int batch, sc_state **out);
+int sc_state_destroy(sc_state *st);
+int sc_state_qubits(const sc_state *st, int *num_qubits, int *batch);
+/* resetZeroState / resetBasisState StatevectorEngine.java:345-357 (index = OR of 1<<qubit), every batch element. */
+int sc_state_set_basis(sc_state *st, uint64_t index);
+/* populateFromState StatevectorEngine.java:340-343; n_complex = 2^n * batch (batch-major). */
+int sc_state_upload(sc_state *st, const double *interleaved, uint64_t n_complex);
+/* toStateVector StatevectorEngine.java:367-369. */
+int sc_state_download(sc_state *st, double *interleaved, uint64_t n_complex);
+/* Device-to-device copy of all amplitudes (same shape). */
+int sc_state_copy(sc_state *dst, const sc_state *src);
+
+/* ---- gate application ----------------------------------------------------- */
+/* The op loop of StatevectorEngine.simulate :90-151 for every unitary op kind;
+ * validates each op like Operation.validateTargets (QuantumCircuit.java:362-574);
+ * a unitary op after SC_OP_MEASURE fails with SC_E_UNSUPPORTED (:92-95).
+ * `mats` holds interleaved complex entries referenced by sc_op.mat.  With
+ * batch > 1 the same circuit is applied to every batch element. */
+int sc_apply_ops(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex);
+/* Batched circuits with per-element matrices: element b uses mats + b*mat_stride (complex entries). */
+int sc_apply_ops_batched(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t mat_stride);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* computeMeasurementProbabilities StatevectorEngine.java:274-296: out[o] for o in [0, 2^m), outcome bit k <-
+ * qubit measured[k]; `measured` sorted ascending (MeasurementInstruction.java:64-81); batch element `elem`. */
+int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, double *out);
+/* performMeasurement :229-257 = normalize :298-312 + cumulative :314-323 + sampleOutcome :325-338 for each
+ * uniform u (SplittableRandom.nextDouble() values drawn by the host in shot order).  outcomes[s] in [0, 2^m).
+ * boundary_gap (may be NULL) receives min over shots of |u - cum[outcome]|, |u - cum[outcome-1]|. */
+int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const double *uniforms, int64_t shots,
+              int64_t *outcomes, double *boundary_gap);
+/* Sum of |a_i|^2 (ComplexArrays.norm2Sq, ComplexArrays.java:74-143) for batch element `elem`. */
+int sc_norm_sq(sc_state *st, int elem, double *out);
+/* Probability checksum: out[0] = sum_i p_i, out[1] = sum_i p_i * w(i) with w(i) = splitmix64(i)*2^-64. */
+int sc_probability_checksum(sc_state *st, int elem, double out[2]);
+
+/* ---- noise (Monte-Carlo Kraus unravelling) -------------------------------- */
+/* computeBranchWeight StatevectorEngine.java:511-530 for k single-qubit Kraus operators (4 complex each) on
+ * `target`, all in one sweep: weights[j] = || K_j psi ||^2. */
+int sc_kraus_weights(sc_state *st, int elem, int target, const double *kraus, int k, double *weights);
+/* applySingleQubitKraus + renormalizeState :426-439, :590-600: apply K (4 complex) on target, then rescale by
+ * 1/sqrt(normSq) when |normSq-1| > 1e-12.  norm_sq_out (may be NULL) receives normSq. */
+int sc_apply_kraus(sc_state *st, int elem, int target, const double *kraus_op, double *norm_sq_out);
+/* Batched trajectories: state `st` holds `batch` copies of one circuit's state; after every op i with
+ * noise_after[i] != 0, a depolarising channel of probability p acts on each of the op's qubits
+ * (q0, then q1) with the reference's unravelling (sampleKrausOperator :463-491 + sampleWeightedIndex
+ * :579-588 + renormalizeState :590-600).  uniforms: [batch][n_draws] SplittableRandom.nextDouble() values,
+ * one per scheduled (op, qubit) in circuit order; n_draws must equal that count. */
+int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
+                                     uint64_t n_mat_complex, const uint8_t *noise_after, double p,
+                                     const double *uniforms, uint64_t n_draws);
+
+/* ---- observables and gradients (no reference counterpart) ----------------- */
+/* Pauli-sum observable H = sum_t coeff[t] * P_t with P_t given by an x-mask and z-mask per term (Y = X and Z
+ * on the same qubit, with the i factor accounted).  out[b] = <psi_b|H|psi_b> for every batch element. */
+int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                         int n_terms, double *out);
+/* Adjoint gradient of E(theta) = <psi(theta)|H|psi(theta)>.  On entry `st` holds the initial states; the
+ * call runs the circuit forward (leaving psi_b = U(theta_b)|init_b> in `st`), then sweeps the bra
+ * lambda = H psi and the ket psi backwards through U^dagger with one inner-product reduction per
+ * parameterised op.  mats holds n_mat_complex entries, or batch * n_mat_complex entries (element b at
+ * b * n_mat_complex) when mats_per_element != 0.  grads[b*n_params + p] = dE/dtheta_p; energies (may be
+ * NULL) receives E_b.  checkpoint_every > 0 stores the forward state every that many ops and, during the
+ * reverse sweep, re-anchors psi from a checkpoint when max-abs drift exceeds 1e-10.  Parameterised ops:
+ * SC_OP_SINGLE / SC_OP_MCU with gen X, Y or Z on q0, SC_OP_DIAG2 with gen ZZ. */
+int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                        int mats_per_element, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                        int n_terms, int n_params, int checkpoint_every, double *grads, double *energies);
+/* Statistics of the last sc_adjoint_gradient on this context: re-anchors performed, max drift seen. */
+int sc_adjoint_stats(sc_ctx *ctx, uint64_t *reanchors, double *max_drift);
+
+/* ---- tracing -------------------------------------------------------------- */
+/* With SC_OPT_PROFILE=1 every kernel launch is bracketed by HIP events on the context's stream.  Query per
+ * kernel family: number of launches, total milliseconds, and the algorithmic HBM bytes credited. */
+#define SC_KFAM_GATE 0     /* per-op kernels */
+#define SC_KFAM_FUSED 1    /* dense fused-block kernel */
+#define SC_KFAM_SWEEP 2    /* LDS-blocked multi-gate sweep kernel */
+#define SC_KFAM_MEASURE 3  /* probability / sampling kernels */
+#define SC_KFAM_NOISE 4    /* Kraus weight / renormalise kernels */
+#define SC_KFAM_ADJOINT 5  /* adjoint backward kernels */
+#define SC_KFAM_COUNT 6
+int sc_profile_reset(sc_ctx *ctx);
+int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_ms, double *bytes);
+/* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
+int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks);
+/* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
+ * (n >= 12) and reports its passes, stages and fused 16x16 blocks.  When state_inout is non-NULL (2^n
+ * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
+ * movement, so plans can be validated without a GPU. */
+int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                        uint64_t n_mat_complex, int low_qubits, uint64_t *passes, uint64_t *stages,
+                        uint64_t *fused_blocks, double *state_inout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STATECRAFT_HIP_H */

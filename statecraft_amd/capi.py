@@ -1,0 +1,361 @@
+"""ctypes binding of libstatecraft_hip.so (include/statecraft_hip.h).
+
+The shared library is the product; this module only marshals numpy arrays
+into the C-ABI and maps status codes onto the reference's exception classes
+(StatevectorEngine throws IllegalArgumentException, UnsupportedOperationException
+and IllegalStateException; see engines/.../statevector/StatevectorEngine.java).
+There is no fallback: if the library is missing, loading fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libstatecraft_hip.so"
+
+# ---- constants (statecraft_hip.h) -------------------------------------------
+SC_OK, SC_E_NULL, SC_E_ARG, SC_E_UNSUPPORTED, SC_E_STATE, SC_E_OOM, SC_E_HIP = range(7)
+OP_SINGLE, OP_CNOT, OP_TWO, OP_DIAG2, OP_SWAP, OP_MCU, OP_MEASURE = range(1, 8)
+GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
+OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT = range(1, 6)
+EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
+KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT = range(6)
+
+OP_DTYPE = np.dtype(
+    [
+        ("kind", np.int32),
+        ("q0", np.int32),
+        ("q1", np.int32),
+        ("mat", np.int32),
+        ("controls", np.uint64),
+        ("param", np.int32),
+        ("gen", np.int32),
+        ("coeff", np.float64),
+    ],
+    align=True,
+)
+assert OP_DTYPE.itemsize == 40
+
+
+class NullPointerException(TypeError):
+    pass
+
+
+class IllegalArgumentException(ValueError):
+    pass
+
+
+class UnsupportedOperationException(NotImplementedError):
+    pass
+
+
+class IllegalStateException(RuntimeError):
+    pass
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_EXC = {
+    SC_E_NULL: NullPointerException,
+    SC_E_ARG: IllegalArgumentException,
+    SC_E_UNSUPPORTED: UnsupportedOperationException,
+    SC_E_STATE: IllegalStateException,
+    SC_E_OOM: UnsupportedOperationException,
+    SC_E_HIP: HipError,
+}
+
+EXPORTED = [
+    "sc_last_error", "sc_version", "sc_ctx_create", "sc_ctx_destroy", "sc_ctx_set_option", "sc_ctx_get_option",
+    "sc_ctx_synchronize", "sc_ctx_device_info", "sc_state_create", "sc_state_destroy", "sc_state_qubits",
+    "sc_state_set_basis", "sc_state_upload", "sc_state_download", "sc_state_copy", "sc_apply_ops",
+    "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
+    "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_expectation_pauli",
+    "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
+    "sc_plan_sweeps_host",
+]
+
+_lib = None
+
+P = ctypes.c_void_p
+DP = ctypes.POINTER(ctypes.c_double)
+I32P = ctypes.POINTER(ctypes.c_int32)
+I64P = ctypes.POINTER(ctypes.c_int64)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+i64 = ctypes.c_int64
+
+
+def _sig(lib):
+    s = {
+        "sc_last_error": (ctypes.c_char_p, []),
+        "sc_version": (i32, []),
+        "sc_ctx_create": (i32, [i32, ctypes.POINTER(P)]),
+        "sc_ctx_destroy": (i32, [P]),
+        "sc_ctx_set_option": (i32, [P, i32, i64]),
+        "sc_ctx_get_option": (i32, [P, i32, I64P]),
+        "sc_ctx_synchronize": (i32, [P]),
+        "sc_ctx_device_info": (i32, [P, U64P, U64P, ctypes.POINTER(i32)]),
+        "sc_state_create": (i32, [P, i32, i32, ctypes.POINTER(P)]),
+        "sc_state_destroy": (i32, [P]),
+        "sc_state_qubits": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "sc_state_set_basis": (i32, [P, u64]),
+        "sc_state_upload": (i32, [P, DP, u64]),
+        "sc_state_download": (i32, [P, DP, u64]),
+        "sc_state_copy": (i32, [P, P]),
+        "sc_apply_ops": (i32, [P, P, u64, DP, u64]),
+        "sc_apply_ops_batched": (i32, [P, P, u64, DP, u64]),
+        "sc_probabilities": (i32, [P, i32, I32P, i32, DP]),
+        "sc_sample": (i32, [P, i32, I32P, i32, DP, i64, I64P, DP]),
+        "sc_norm_sq": (i32, [P, i32, DP]),
+        "sc_probability_checksum": (i32, [P, i32, DP]),
+        "sc_kraus_weights": (i32, [P, i32, i32, DP, i32, DP]),
+        "sc_apply_kraus": (i32, [P, i32, i32, DP, DP]),
+        "sc_run_depolarizing_trajectories": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64]),
+        "sc_expectation_pauli": (i32, [P, U64P, U64P, DP, i32, DP]),
+        "sc_adjoint_gradient": (i32, [P, P, u64, DP, u64, i32, U64P, U64P, DP, i32, i32, i32, DP, DP]),
+        "sc_adjoint_stats": (i32, [P, U64P, DP]),
+        "sc_profile_reset": (i32, [P]),
+        "sc_profile_read": (i32, [P, i32, U64P, DP, DP]),
+        "sc_last_plan_stats": (i32, [P, U64P, U64P, U64P]),
+        "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, U64P, U64P, U64P, DP]),
+    }
+    for name, (res, args) in s.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """Load the HIP engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = Path(os.environ.get("STATECRAFT_HIP_LIB", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(
+                f"{path} is missing: build the HIP engine with `python -m statecraft_amd.build` "
+                "(there is no CPU fallback)"
+            )
+        l = ctypes.CDLL(str(path))
+        _sig(l)
+        _lib = l
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != SC_OK:
+        msg = lib().sc_last_error().decode("utf-8", "replace")
+        raise _EXC.get(rc, RuntimeError)(msg)
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(DP)
+
+
+def as_f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def complex_to_f64(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64)
+
+
+def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None):
+    """Host-only sweep plan statistics; with `state` (complex 2^n) also executes the plan on the host."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+    a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    out = None
+    ptr = None
+    if state is not None:
+        out = np.array(state, dtype=np.complex128, copy=True)
+        ptr = dptr(out.view(np.float64))
+    check(lib().sc_plan_sweeps_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(low_qubits),
+                                    ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ptr))
+    return {"passes": a.value, "stages": b.value, "fused_blocks": c.value}, out
+
+
+class Context:
+    """One device + one HIP stream (StatevectorEngine(StatevectorExecutionConfig) analogue)."""
+
+    def __init__(self, device: int = 0, mode: int | None = None):
+        h = P()
+        check(lib().sc_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        if mode is not None:
+            self.set_option(OPT_EXEC_MODE, mode)
+
+    def set_option(self, opt: int, value: int) -> None:
+        check(lib().sc_ctx_set_option(self.h, opt, int(value)))
+
+    def get_option(self, opt: int) -> int:
+        v = ctypes.c_int64()
+        check(lib().sc_ctx_get_option(self.h, opt, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self) -> None:
+        check(lib().sc_ctx_synchronize(self.h))
+
+    def device_info(self):
+        f, t, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        check(lib().sc_ctx_device_info(self.h, ctypes.byref(f), ctypes.byref(t), ctypes.byref(c)))
+        return f.value, t.value, c.value
+
+    def profile_reset(self) -> None:
+        check(lib().sc_profile_reset(self.h))
+
+    def profile_read(self, family: int):
+        n, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+        check(lib().sc_profile_read(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)))
+        return n.value, ms.value, by.value
+
+    def plan_stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().sc_last_plan_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"sweeps": a.value, "launches": b.value, "fused_blocks": c.value}
+
+    def adjoint_stats(self):
+        a, b = ctypes.c_uint64(), ctypes.c_double()
+        check(lib().sc_adjoint_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return {"reanchors": a.value, "max_drift": b.value}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().sc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class State:
+    """A resident batch of 2^n complex128 states in HBM."""
+
+    def __init__(self, ctx: Context, n: int, batch: int = 1):
+        h = P()
+        check(lib().sc_state_create(ctx.h, int(n), int(batch), ctypes.byref(h)))
+        self.h, self.ctx, self.n, self.batch = h, ctx, int(n), int(batch)
+
+    @property
+    def size(self) -> int:
+        return self.batch << self.n
+
+    def set_basis(self, index: int) -> None:
+        check(lib().sc_state_set_basis(self.h, int(index)))
+
+    def upload(self, amps) -> None:
+        a = np.ascontiguousarray(amps, dtype=np.complex128).reshape(-1)
+        check(lib().sc_state_upload(self.h, dptr(a.view(np.float64)), a.size))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=np.complex128)
+        check(lib().sc_state_download(self.h, dptr(out.view(np.float64)), out.size))
+        return out if self.batch == 1 else out.reshape(self.batch, -1)
+
+    def copy_from(self, other: "State") -> None:
+        check(lib().sc_state_copy(self.h, other.h))
+
+    def apply(self, ops: np.ndarray, mats) -> None:
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+        check(lib().sc_apply_ops(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2))
+
+    def apply_batched(self, ops: np.ndarray, mats_per_elem) -> None:
+        """mats_per_elem: complex array [batch, n_mat]."""
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = np.ascontiguousarray(mats_per_elem, dtype=np.complex128)
+        assert m.ndim == 2 and m.shape[0] == self.batch
+        check(lib().sc_apply_ops_batched(self.h, ops.ctypes.data_as(P), len(ops), dptr(m.view(np.float64)),
+                                         m.shape[1]))
+
+    def probabilities(self, measured=None, elem: int = 0) -> np.ndarray:
+        meas = np.arange(self.n, dtype=np.int32) if measured is None else np.ascontiguousarray(measured, np.int32)
+        out = np.empty(1 << len(meas), dtype=np.float64)
+        check(lib().sc_probabilities(self.h, elem, meas.ctypes.data_as(I32P), len(meas), dptr(out)))
+        return out
+
+    def sample(self, measured, uniforms, elem: int = 0):
+        meas = np.ascontiguousarray(measured, np.int32)
+        u = as_f64(uniforms)
+        out = np.empty(len(u), dtype=np.int64)
+        gap = ctypes.c_double()
+        check(lib().sc_sample(self.h, elem, meas.ctypes.data_as(I32P), len(meas), dptr(u), len(u),
+                              out.ctypes.data_as(I64P), ctypes.byref(gap)))
+        return out, gap.value
+
+    def norm_sq(self, elem: int = 0) -> float:
+        v = ctypes.c_double()
+        check(lib().sc_norm_sq(self.h, elem, ctypes.byref(v)))
+        return v.value
+
+    def checksum(self, elem: int = 0):
+        out = np.empty(2)
+        check(lib().sc_probability_checksum(self.h, elem, dptr(out)))
+        return float(out[0]), float(out[1])
+
+    def kraus_weights(self, target: int, kraus, elem: int = 0) -> np.ndarray:
+        k = complex_to_f64(np.asarray(kraus, np.complex128).reshape(-1, 4))
+        w = np.empty(k.size // 8)
+        check(lib().sc_kraus_weights(self.h, elem, target, dptr(k), len(w), dptr(w)))
+        return w
+
+    def apply_kraus(self, target: int, op, elem: int = 0) -> float:
+        k = complex_to_f64(np.asarray(op, np.complex128).reshape(4))
+        ns = ctypes.c_double()
+        check(lib().sc_apply_kraus(self.h, elem, target, dptr(k), ctypes.byref(ns)))
+        return ns.value
+
+    def expectation(self, xmask, zmask, coeff) -> np.ndarray:
+        xm = np.ascontiguousarray(xmask, np.uint64)
+        zm = np.ascontiguousarray(zmask, np.uint64)
+        c = as_f64(coeff)
+        out = np.empty(self.batch)
+        check(lib().sc_expectation_pauli(self.h, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P), dptr(c), len(c),
+                                         dptr(out)))
+        return out
+
+    def adjoint_gradient(self, ops, mats, n_params: int, xmask, zmask, coeff, checkpoint_every: int = 0,
+                         per_element: bool = False):
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = np.ascontiguousarray(mats, dtype=np.complex128)
+        n_mat = m.shape[-1] if m.ndim == 2 else m.size
+        xm = np.ascontiguousarray(xmask, np.uint64)
+        zm = np.ascontiguousarray(zmask, np.uint64)
+        c = as_f64(coeff)
+        grads = np.empty(self.batch * n_params)
+        energies = np.empty(self.batch)
+        mf = m.reshape(-1).view(np.float64) if m.size else np.zeros(2)
+        check(lib().sc_adjoint_gradient(self.h, ops.ctypes.data_as(P), len(ops), dptr(mf), n_mat,
+                                        1 if per_element else 0, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P),
+                                        dptr(c), len(c), n_params, checkpoint_every, dptr(grads), dptr(energies)))
+        return grads.reshape(self.batch, n_params), energies
+
+    def run_depolarizing_trajectories(self, ops, mats, noise_after, p: float, uniforms) -> None:
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+        na = np.ascontiguousarray(noise_after, np.uint8)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        assert u.ndim == 2 and u.shape[0] == self.batch
+        check(lib().sc_run_depolarizing_trajectories(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
+                                                     na.ctypes.data_as(U8P), float(p), dptr(u), u.shape[1]))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().sc_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,588 @@
+// sc_adjoint.hip -- Pauli-sum expectations, the adjoint-gradient reverse sweep, and batched noisy
+// trajectories.  None of these has a counterpart in the reference engine; their inputs and outputs use
+// the same op encoding and conventions as sc_apply_ops.
+//
+// Adjoint method (Jones & Gacon, "Efficient calculation of gradients in classical simulations of
+// variational quantum algorithms", arXiv:2009.02823): with psi_g the state after op g and
+// lambda_g = U_{g+1}^dag ... U_N^dag H psi_N, an op U_g = exp(-i theta c G) contributes
+// dE/dtheta += 2 c Im <lambda_g | G | psi_g>.  The reverse sweep reads psi and lambda once per op, forms
+// that inner product over the op's index pairs, applies U_g^dag to both and writes them back -- one
+// fused kernel per op (k_adj_pair / k_adj_two / k_adj_diag2 / k_adj_swap).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sc_internal.hpp"
+#include "sc_plan.hpp"
+
+namespace sc {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kParts = 64;  // partial sums per batch element
+
+__device__ __forceinline__ double2 cmul(double ar, double ai, double2 x) {
+    return make_double2(ar * x.x - ai * x.y, ar * x.y + ai * x.x);
+}
+__device__ __forceinline__ double2 cmad(double ar, double ai, double2 x, double2 acc) {
+    acc.x = fma(ar, x.x, acc.x);
+    acc.x = fma(-ai, x.y, acc.x);
+    acc.y = fma(ar, x.y, acc.y);
+    acc.y = fma(ai, x.x, acc.y);
+    return acc;
+}
+// conj(a) * b accumulated
+__device__ __forceinline__ void cdot_acc(double2 a, double2 b, double &re, double &im) {
+    re = fma(a.x, b.x, re);
+    re = fma(a.y, b.y, re);
+    im = fma(a.x, b.y, im);
+    im = fma(-a.y, b.x, im);
+}
+
+__device__ __forceinline__ void block_reduce2(double re, double im, double *out) {
+    __shared__ double sr[kBlock], si[kBlock];
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sr[0];
+        out[1] = si[0];
+    }
+}
+
+__device__ __forceinline__ uint64_t insert_zero(uint64_t x, int bit) {
+    uint64_t lo = x & ((1ULL << bit) - 1ULL);
+    return ((x ^ lo) << 1) | lo;
+}
+__device__ __forceinline__ uint64_t expand_mask(uint64_t x, uint64_t mask) {
+    while (mask) {
+        int p = __builtin_ctzll(mask);
+        x = insert_zero(x, p);
+        mask &= mask - 1ULL;
+    }
+    return x;
+}
+
+// sum_i conj(psi_{i^x}) (-1)^{|i&z|} psi_i per element: grid (kParts, batch), out[(elem*kParts+blk)*2]
+__global__ void __launch_bounds__(kBlock) k_pauli_partial(const double2 *__restrict__ amp, int n, uint64_t x,
+                                                          uint64_t z, double *__restrict__ out) {
+    const uint64_t elem = blockIdx.y;
+    const double2 *base = amp + (elem << n);
+    const uint64_t dim = 1ULL << n;
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < dim; i += (uint64_t)kParts * kBlock) {
+        double2 a = base[i ^ x], b = base[i];
+        if (__popcll(i & z) & 1) b = make_double2(-b.x, -b.y);
+        cdot_acc(a, b, re, im);
+    }
+    block_reduce2(re, im, out + (elem * kParts + blockIdx.x) * 2);
+}
+
+// sum_i conj(a_i) b_i per element
+__global__ void __launch_bounds__(kBlock) k_dot_partial(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                        int n, double *__restrict__ out) {
+    const uint64_t elem = blockIdx.y;
+    const uint64_t dim = 1ULL << n;
+    const double2 *pa = a + (elem << n), *pb = b + (elem << n);
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < dim; i += (uint64_t)kParts * kBlock)
+        cdot_acc(pa[i], pb[i], re, im);
+    block_reduce2(re, im, out + (elem * kParts + blockIdx.x) * 2);
+}
+
+// lam (+)= coeff * i^ny * (-1)^{|(i^x)&z|} psi_{i^x}
+__global__ void __launch_bounds__(kBlock) k_pauli_apply(const double2 *__restrict__ psi, double2 *__restrict__ lam,
+                                                        int n, uint64_t total, uint64_t x, uint64_t z, double cr,
+                                                        double ci, int first) {
+    const uint64_t dmask = (1ULL << n) - 1ULL;
+    for (uint64_t g = blockIdx.x * (uint64_t)kBlock + threadIdx.x; g < total; g += (uint64_t)gridDim.x * kBlock) {
+        uint64_t i = g & dmask;
+        uint64_t j = i ^ x;
+        double2 v = psi[(g - i) + j];
+        if (__popcll(j & z) & 1) v = make_double2(-v.x, -v.y);
+        double2 t = cmul(cr, ci, v);
+        if (first) {
+            lam[g] = t;
+        } else {
+            double2 l = lam[g];
+            lam[g] = make_double2(l.x + t.x, l.y + t.y);
+        }
+    }
+}
+
+// Max |a-b| over all components: grid-stride partial maxima.
+__global__ void __launch_bounds__(kBlock) k_maxdiff(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                    uint64_t count, double *__restrict__ out) {
+    __shared__ double sh[kBlock];
+    double m = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock) {
+        double2 x = a[i], y = b[i];
+        m = fmax(m, fmax(fabs(x.x - y.x), fabs(x.y - y.y)));
+        if (x.x != x.x || x.y != x.y) m = INFINITY;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ void k_add_to_first(double2 *amp, int n, int batch, double delta) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch) amp[(uint64_t)b << n].x += delta;
+}
+
+// ---- adjoint step kernels: grid (kParts, batch); inv = per-element inverse matrices ----------------
+// Pair op on `target` with controls (compacted): U^dag 2x2 (or exact X), inner product with gen G on target.
+__global__ void __launch_bounds__(kBlock) k_adj_pair(double2 *__restrict__ psi, double2 *__restrict__ lam, int n,
+                                                     int target, uint64_t fixed, uint64_t controls, int log_groups,
+                                                     const double *__restrict__ inv, uint64_t inv_stride, int is_x,
+                                                     int gen, double *__restrict__ part) {
+    const uint64_t elem = blockIdx.y;
+    double2 *P = psi + (elem << n), *L = lam + (elem << n);
+    const double *g = inv + elem * inv_stride;
+    const double g0r = g[0], g0i = g[1], g1r = g[2], g1i = g[3], g2r = g[4], g2i = g[5], g3r = g[6], g3i = g[7];
+    const uint64_t stride = 1ULL << target;
+    const uint64_t groups = 1ULL << log_groups;
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < groups; i += (uint64_t)kParts * kBlock) {
+        uint64_t i0 = expand_mask(i, fixed) | controls, i1 = i0 | stride;
+        double2 p0 = P[i0], p1 = P[i1], l0 = L[i0], l1 = L[i1];
+        if (gen == SC_GEN_X) {
+            cdot_acc(l0, p1, re, im);
+            cdot_acc(l1, p0, re, im);
+        } else if (gen == SC_GEN_Y) {  // Y p: (-i p1, i p0)
+            cdot_acc(l0, make_double2(p1.y, -p1.x), re, im);
+            cdot_acc(l1, make_double2(-p0.y, p0.x), re, im);
+        } else if (gen == SC_GEN_Z) {
+            cdot_acc(l0, p0, re, im);
+            cdot_acc(l1, make_double2(-p1.x, -p1.y), re, im);
+        }
+        if (is_x) {
+            P[i0] = p1, P[i1] = p0, L[i0] = l1, L[i1] = l0;
+        } else {
+            P[i0] = cmad(g1r, g1i, p1, cmul(g0r, g0i, p0));
+            P[i1] = cmad(g3r, g3i, p1, cmul(g2r, g2i, p0));
+            L[i0] = cmad(g1r, g1i, l1, cmul(g0r, g0i, l0));
+            L[i1] = cmad(g3r, g3i, l1, cmul(g2r, g2i, l0));
+        }
+    }
+    if (part) block_reduce2(re, im, part + (elem * kParts + blockIdx.x) * 2);
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_two(double2 *__restrict__ psi, double2 *__restrict__ lam, int n,
+                                                    int first, int second, const double *__restrict__ inv,
+                                                    uint64_t inv_stride) {
+    const uint64_t elem = blockIdx.y;
+    double2 *S[2] = {psi + (elem << n), lam + (elem << n)};
+    const double *m = inv + elem * inv_stride;
+    const uint64_t fm = 1ULL << first, sm = 1ULL << second;
+    const uint64_t groups = 1ULL << (n - 2);
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < groups; i += (uint64_t)kParts * kBlock) {
+        uint64_t b = expand_mask(i, fm | sm);
+        uint64_t off[4] = {b, b | sm, b | fm, b | fm | sm};
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            double2 a[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) a[c] = S[s][off[c]];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                double2 acc = cmul(m[8 * r], m[8 * r + 1], a[0]);
+#pragma unroll
+                for (int c = 1; c < 4; c++) acc = cmad(m[8 * r + 2 * c], m[8 * r + 2 * c + 1], a[c], acc);
+                S[s][off[r]] = acc;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_diag2(double2 *__restrict__ psi, double2 *__restrict__ lam, int n,
+                                                      int first, int second, const double *__restrict__ inv,
+                                                      uint64_t inv_stride, int gen, double *__restrict__ part) {
+    const uint64_t elem = blockIdx.y;
+    double2 *P = psi + (elem << n), *L = lam + (elem << n);
+    const double *d = inv + elem * inv_stride;
+    const uint64_t dim = 1ULL << n;
+    double re = 0.0, im = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < dim; i += (uint64_t)kParts * kBlock) {
+        int bf = (int)((i >> first) & 1ULL), bs = (int)((i >> second) & 1ULL);
+        int l = 2 * bf + bs;
+        double2 p = P[i], q = L[i];
+        if (gen == SC_GEN_ZZ) cdot_acc(q, (bf ^ bs) ? make_double2(-p.x, -p.y) : p, re, im);
+        P[i] = cmul(d[2 * l], d[2 * l + 1], p);
+        L[i] = cmul(d[2 * l], d[2 * l + 1], q);
+    }
+    if (part) block_reduce2(re, im, part + (elem * kParts + blockIdx.x) * 2);
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_swap(double2 *__restrict__ psi, double2 *__restrict__ lam, int n,
+                                                     int first, int second) {
+    const uint64_t elem = blockIdx.y;
+    double2 *S[2] = {psi + (elem << n), lam + (elem << n)};
+    const uint64_t fm = 1ULL << first, sm = 1ULL << second;
+    const uint64_t groups = 1ULL << (n - 2);
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < groups; i += (uint64_t)kParts * kBlock) {
+        uint64_t b = expand_mask(i, fm | sm);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            double2 x = S[s][b | fm];
+            S[s][b | fm] = S[s][b | sm];
+            S[s][b | sm] = x;
+        }
+    }
+}
+
+// Final fixed-order reduction of kParts complex partials per (slot, elem).
+__global__ void k_parts_final(const double *__restrict__ part, uint64_t count, double *__restrict__ out) {
+    uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < kParts; b++) {
+        re += part[(k * kParts + b) * 2];
+        im += part[(k * kParts + b) * 2 + 1];
+    }
+    out[2 * k] = re;
+    out[2 * k + 1] = im;
+}
+
+unsigned grid_for(Context *ctx, uint64_t total) {
+    uint64_t blocks = (total + kBlock - 1) / kBlock;
+    uint64_t cap = (uint64_t)ctx->cu_count * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    return (unsigned)blocks;
+}
+
+int set_device(Context *ctx) {
+    SC_HIP_CHECK(hipSetDevice(ctx->device));
+    return SC_OK;
+}
+
+// per-element complex results of kParts partials (count = slots*batch) -> host
+int finish_parts(Context *ctx, double *part, uint64_t count, double *host) {
+    double *out = part + count * kParts * 2;
+    hipLaunchKernelGGL(k_parts_final, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, part, count,
+                       out);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipMemcpyAsync(host, out, count * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+// <psi_b|H|psi_b> for all b: out[b]
+int expectation(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, const double *c,
+                int n_terms, double *out) {
+    const uint64_t B = (uint64_t)st->batch;
+    int rc = ensure_scratch(ctx, (size_t)B * (kParts + 1) * 2 * sizeof(double));
+    if (rc) return rc;
+    double *part = (double *)ctx->scratch;
+    std::vector<double> res(2 * B);
+    for (uint64_t b = 0; b < B; b++) out[b] = 0.0;
+    for (int t = 0; t < n_terms; t++) {
+        {
+            ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 16.0 * (double)(B << st->n));
+            hipLaunchKernelGGL(k_pauli_partial, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, amp, st->n,
+                               xm[t], zm[t], part);
+            SC_HIP_CHECK(hipGetLastError());
+        }
+        if ((rc = finish_parts(ctx, part, B, res.data()))) return rc;
+        int ny = __builtin_popcountll(xm[t] & zm[t]) & 3;
+        for (uint64_t b = 0; b < B; b++) {
+            double re = res[2 * b], im = res[2 * b + 1];
+            double v = ny == 0 ? re : ny == 1 ? -im : ny == 2 ? -re : im;
+            out[b] += c[t] * v;
+        }
+    }
+    return SC_OK;
+}
+
+int check_terms(const State *st, const uint64_t *xm, const uint64_t *zm, const double *c, int n_terms) {
+    if (n_terms <= 0) return fail(SC_E_ARG, "observable must have at least one term");
+    if (!xm || !zm || !c) return fail(SC_E_NULL, "observable arrays are null");
+    for (int t = 0; t < n_terms; t++)
+        if (st->n < 64 && (((xm[t] | zm[t]) >> st->n) != 0))
+            return fail(SC_E_ARG, "observable term " + std::to_string(t) + " acts outside the register");
+    return SC_OK;
+}
+
+// U^dag of an op's matrix (row-major complex), count entries
+void inverse_matrix(int kind, const double *m, double *out) {
+    if (kind == SC_OP_DIAG2) {
+        for (int j = 0; j < 4; j++) out[2 * j] = m[2 * j], out[2 * j + 1] = -m[2 * j + 1];
+        return;
+    }
+    int d = kind == SC_OP_TWO ? 4 : 2;
+    for (int r = 0; r < d; r++)
+        for (int c = 0; c < d; c++) {
+            out[2 * (r * d + c)] = m[2 * (c * d + r)];
+            out[2 * (r * d + c) + 1] = -m[2 * (c * d + r) + 1];
+        }
+}
+
+}  // namespace
+
+}  // namespace sc
+
+using namespace sc;
+
+extern "C" {
+
+int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, const double *coeff, int n_terms,
+                         double *out) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (!out) return fail(SC_E_NULL, "out is null");
+    int rc = check_terms(st, xmask, zmask, coeff, n_terms);
+    if (rc) return rc;
+    if ((rc = set_device(st->ctx))) return rc;
+    return expectation(st->ctx, st, st->amp, xmask, zmask, coeff, n_terms, out);
+}
+
+int sc_adjoint_stats(sc_ctx *ctx, uint64_t *reanchors, double *max_drift) {
+    if (!ctx) return fail(SC_E_NULL, "ctx is null");
+    if (reanchors) *reanchors = ctx->adj_reanchors;
+    if (max_drift) *max_drift = ctx->adj_max_drift;
+    return SC_OK;
+}
+
+int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                        int mats_per_element, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                        int n_terms, int n_params, int checkpoint_every, double *grads, double *energies) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (n_ops > 0 && !ops) return fail(SC_E_NULL, "ops is null");
+    if (n_mat_complex > 0 && !mats) return fail(SC_E_NULL, "mats is null");
+    if (!grads) return fail(SC_E_NULL, "grads is null");
+    if (n_params < 0) return fail(SC_E_ARG, "n_params must be non-negative");
+    if (checkpoint_every < 0) return fail(SC_E_ARG, "checkpoint_every must be non-negative");
+    sc_ctx *ctx = static_cast<sc_ctx *>(st->ctx);
+    int rc = check_terms(st, xmask, zmask, coeff, n_terms);
+    if (rc) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    const int n = st->n;
+    const uint64_t B = (uint64_t)st->batch;
+    std::vector<const double *> elems;
+    if (mats_per_element)
+        for (uint64_t b = 0; b < B; b++) elems.push_back(mats + 2 * n_mat_complex * b);
+    else
+        elems.push_back(mats);
+    std::vector<PlanOp> pops;
+    if ((rc = analyze_ops(n, ops, n_ops, elems, n_mat_complex, pops))) return rc;
+    if (pops.size() != n_ops) return fail(SC_E_ARG, "adjoint circuits must not contain measurements");
+    if (n < 2) return fail(SC_E_ARG, "adjoint gradients need at least 2 qubits");
+    // parameter metadata
+    std::vector<int> slot(n_ops, -1);
+    int n_slots = 0;
+    for (uint64_t i = 0; i < n_ops; i++) {
+        const sc_op &o = ops[i];
+        if (o.param < 0) continue;
+        if (o.param >= n_params) return fail(SC_E_ARG, "op " + std::to_string(i) + " parameter index out of range");
+        bool ok = ((o.kind == SC_OP_SINGLE || o.kind == SC_OP_MCU) &&
+                   (o.gen == SC_GEN_X || o.gen == SC_GEN_Y || o.gen == SC_GEN_Z)) ||
+                  (o.kind == SC_OP_DIAG2 && o.gen == SC_GEN_ZZ);
+        if (!ok) return fail(SC_E_UNSUPPORTED, "op " + std::to_string(i) + ": unsupported generator for its kind");
+        slot[i] = n_slots++;
+    }
+    // inverse matrices per element: inv[e * n_mat_complex + off]
+    const uint64_t E = elems.size();
+    std::vector<double> inv(2 * std::max<uint64_t>(n_mat_complex, 1) * E, 0.0);
+    for (uint64_t e = 0; e < E; e++)
+        for (uint64_t i = 0; i < n_ops; i++)
+            if (ops[i].mat >= 0 && ops[i].kind != SC_OP_CNOT && ops[i].kind != SC_OP_SWAP)
+                inverse_matrix(ops[i].kind, elems[e] + 2 * (int64_t)ops[i].mat,
+                               inv.data() + 2 * (e * n_mat_complex + ops[i].mat));
+    DevBuf dinv, dlam, dckpt, dpart;
+    if ((rc = dinv.alloc(inv.size() * sizeof(double)))) return rc;
+    SC_HIP_CHECK(hipMemcpyAsync(dinv.p, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t inv_stride = mats_per_element ? 2 * n_mat_complex : 0;
+    const size_t state_bytes = (size_t)(B << n) * sizeof(double2);
+    // ---- forward with checkpoints
+    const uint64_t C = (uint64_t)checkpoint_every;
+    uint64_t n_ckpt = (C > 0 && n_ops > 0) ? (n_ops - 1) / C : 0;
+    if (n_ckpt > 0 && (rc = dckpt.alloc(state_bytes * n_ckpt))) return rc;
+    {
+        uint64_t begin = 0;
+        for (uint64_t k = 0; k <= n_ckpt; k++) {
+            uint64_t end = (k < n_ckpt) ? (k + 1) * C : n_ops;
+            if (end > begin) {
+                rc = apply_ops_impl(st, ops + begin, end - begin, mats, n_mat_complex,
+                                    mats_per_element ? n_mat_complex : 0, mats_per_element != 0);
+                if (rc) return rc;
+            }
+            if (k < n_ckpt)
+                SC_HIP_CHECK(hipMemcpyAsync((char *)dckpt.p + state_bytes * k, st->amp, state_bytes,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+            begin = end;
+        }
+    }
+    // ---- lambda = H psi
+    if ((rc = dlam.alloc(state_bytes))) return rc;
+    double2 *lam = (double2 *)dlam.p;
+    const uint64_t total = B << n;
+    for (int t = 0; t < n_terms; t++) {
+        int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+        double pr = ny == 0 ? 1.0 : ny == 2 ? -1.0 : 0.0, pi = ny == 1 ? 1.0 : ny == 3 ? -1.0 : 0.0;
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 48.0 * (double)total);
+        hipLaunchKernelGGL(k_pauli_apply, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                           total, xmask[t], zmask[t], coeff[t] * pr, coeff[t] * pi, t == 0 ? 1 : 0);
+        SC_HIP_CHECK(hipGetLastError());
+    }
+    // ---- energies E_b = Re <psi|lambda>
+    const uint64_t part_count = (uint64_t)std::max(n_slots, 1) * B;
+    if ((rc = dpart.alloc((part_count * kParts + part_count) * 2 * sizeof(double) + B * (kParts + 1) * 16)))
+        return rc;
+    double *part = (double *)dpart.p;
+    double *epart = part + (part_count * kParts + part_count) * 2;
+    if (energies) {
+        hipLaunchKernelGGL(k_dot_partial, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                           epart);
+        SC_HIP_CHECK(hipGetLastError());
+        std::vector<double> e(2 * B);
+        if ((rc = finish_parts(ctx, epart, B, e.data()))) return rc;
+        for (uint64_t b = 0; b < B; b++) energies[b] = e[2 * b];
+    }
+    // ---- reverse sweep
+    ctx->adj_reanchors = 0;
+    ctx->adj_max_drift = 0.0;
+    std::vector<double> mx;
+    const unsigned dgrid = grid_for(ctx, total);
+    for (int64_t g = (int64_t)n_ops - 1; g >= 0; g--) {
+        // psi should now equal the state after op g: compare with the checkpoint taken there
+        if (C > 0 && (uint64_t)(g + 1) % C == 0 && (uint64_t)(g + 1) / C >= 1 && (uint64_t)(g + 1) / C <= n_ckpt) {
+            uint64_t k = (uint64_t)(g + 1) / C - 1;
+            double2 *ck = (double2 *)((char *)dckpt.p + state_bytes * k);
+            if (ctx->test_drift > 0) {
+                hipLaunchKernelGGL(k_add_to_first, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   st->amp, n, (int)B, (double)ctx->test_drift * 1e-12);
+                SC_HIP_CHECK(hipGetLastError());
+            }
+            if ((rc = ensure_scratch(ctx, (size_t)dgrid * sizeof(double)))) return rc;
+            hipLaunchKernelGGL(k_maxdiff, dim3(dgrid), dim3(kBlock), 0, ctx->stream, st->amp, ck, total,
+                               (double *)ctx->scratch);
+            SC_HIP_CHECK(hipGetLastError());
+            mx.resize(dgrid);
+            SC_HIP_CHECK(hipMemcpyAsync(mx.data(), ctx->scratch, dgrid * sizeof(double), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+            SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            double drift = *std::max_element(mx.begin(), mx.end());
+            ctx->adj_max_drift = std::max(ctx->adj_max_drift, drift);
+            if (!(drift <= 1e-10)) {
+                SC_HIP_CHECK(hipMemcpyAsync(st->amp, ck, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+                ctx->adj_reanchors++;
+            }
+        }
+        const PlanOp &op = pops[g];
+        const sc_op &o = ops[g];
+        double *pslot = slot[g] >= 0 ? part + (uint64_t)slot[g] * B * kParts * 2 : nullptr;
+        const double *dinvp = (const double *)dinv.p + 2 * (o.mat >= 0 ? o.mat : 0);
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 64.0 * (double)total);
+        switch (op.kind) {
+        case SC_OP_SINGLE:
+        case SC_OP_MCU:
+        case SC_OP_CNOT: {
+            int target = op.kind == SC_OP_CNOT ? op.q1 : op.q0;
+            uint64_t controls = op.controls;
+            int lg = n - 1 - __builtin_popcountll(controls);
+            hipLaunchKernelGGL(k_adj_pair, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                               target, controls | (1ULL << target), controls, lg, dinvp, inv_stride,
+                               op.is_x ? 1 : 0, slot[g] >= 0 ? o.gen : 0, pslot);
+            break;
+        }
+        case SC_OP_TWO:
+            hipLaunchKernelGGL(k_adj_two, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                               op.q0, op.q1, dinvp, inv_stride);
+            break;
+        case SC_OP_DIAG2:
+            hipLaunchKernelGGL(k_adj_diag2, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                               op.q0, op.q1, dinvp, inv_stride, slot[g] >= 0 ? o.gen : 0, pslot);
+            break;
+        case SC_OP_SWAP:
+            hipLaunchKernelGGL(k_adj_swap, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                               op.q0, op.q1);
+            break;
+        }
+        SC_HIP_CHECK(hipGetLastError());
+    }
+    // ---- gradients
+    std::fill(grads, grads + B * (uint64_t)n_params, 0.0);
+    if (n_slots > 0) {
+        std::vector<double> res(2 * (uint64_t)n_slots * B);
+        if ((rc = finish_parts(ctx, part, (uint64_t)n_slots * B, res.data()))) return rc;
+        for (uint64_t i = 0; i < n_ops; i++) {
+            if (slot[i] < 0) continue;
+            for (uint64_t b = 0; b < B; b++)
+                grads[b * n_params + ops[i].param] += 2.0 * ops[i].coeff * res[2 * ((uint64_t)slot[i] * B + b) + 1];
+        }
+    }
+    // restore psi to the forward output (the reverse sweep uncomputed it)
+    rc = apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, mats_per_element ? n_mat_complex : 0,
+                        mats_per_element != 0);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
+                                     uint64_t n_mat_complex, const uint8_t *noise_after, double p,
+                                     const double *uniforms, uint64_t n_draws) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (n_ops > 0 && (!ops || !noise_after)) return fail(SC_E_NULL, "ops is null");
+    if (!uniforms) return fail(SC_E_NULL, "uniforms is null");
+    if (!(p >= 0.0 && p <= 1.0)) return fail(SC_E_ARG, "probability must be in [0,1], got " + std::to_string(p));
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    if ((rc = analyze_ops(st->n, ops, n_ops, elems, n_mat_complex, pops))) return rc;
+    if (pops.size() != n_ops) return fail(SC_E_ARG, "trajectory circuits must not contain measurements");
+    uint64_t draws = 0;
+    for (uint64_t i = 0; i < n_ops; i++)
+        if (noise_after[i]) draws += (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
+    if (draws != n_draws)
+        return fail(SC_E_ARG, "expected " + std::to_string(draws) + " uniforms per trajectory, got " +
+                                  std::to_string(n_draws));
+    // DepolarizingChannel (core/.../noise/DepolarizingChannel.java:49-76): sqrt(1-p) I, sqrt(p/3) X, Y, Z
+    const double a = std::sqrt(1.0 - p), s = std::sqrt(p / 3.0);
+    const double kraus[32] = {a, 0, 0, 0, 0, 0, a, 0,  0, 0, s, 0,  s, 0, 0, 0,
+                              0, 0, 0, -s, 0, s, 0, 0, s, 0, 0, 0, 0, 0, -s, 0};
+    const uint64_t B = (uint64_t)st->batch;
+    DevBuf du, dwork;
+    if ((rc = du.alloc(B * std::max<uint64_t>(n_draws, 1) * sizeof(double)))) return rc;
+    if ((rc = dwork.alloc(noise_work_bytes((int)B)))) return rc;
+    if (n_draws > 0)
+        SC_HIP_CHECK(hipMemcpyAsync(du.p, uniforms, B * n_draws * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SC_HIP_CHECK(hipMemsetAsync(noise_err_flag(dwork.p, (int)B), 0, sizeof(int), ctx->stream));
+    uint64_t draw = 0, seg_begin = 0;
+    for (uint64_t i = 0; i < n_ops; i++) {
+        if (!noise_after[i] && i + 1 < n_ops) continue;
+        // unitary segment [seg_begin, i]
+        rc = apply_ops_impl(st, ops + seg_begin, i + 1 - seg_begin, mats, n_mat_complex, 0, false);
+        if (rc) return rc;
+        seg_begin = i + 1;
+        if (!noise_after[i]) continue;
+        int qs[2] = {ops[i].q0, ops[i].q1};
+        int nq = (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
+        for (int j = 0; j < nq; j++) {
+            rc = launch_channel_batched(ctx, st, qs[j], kraus, 4, (const double *)du.p, n_draws, draw++, dwork.p);
+            if (rc) return rc;
+        }
+    }
+    int err = 0;
+    SC_HIP_CHECK(hipMemcpyAsync(&err, noise_err_flag(dwork.p, (int)B), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (err) return fail(SC_E_STATE, "invalid Kraus branch weights in a trajectory");
+    return SC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,696 @@
+// sc_capi.hip -- the C-ABI of the gfx950 statevector engine (include/statecraft_hip.h).
+//
+// Host orchestration only: validation, planning (sc_plan.cpp), HBM ownership,
+// uploads, launches and the reference's host-side rules (normalise threshold,
+// zero-probability and invalid-norm errors).  Kernels live in sc_gates.hip,
+// sc_sweep.hip, sc_measure.hip, sc_noise.hip and sc_adjoint.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sc_internal.hpp"
+#include "sc_plan.hpp"
+
+namespace sc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- profiler
+ProfScope::ProfScope(Profiler *prof, hipStream_t stream, int family, double algo_bytes)
+    : p(prof), s(stream), fam(family), bytes(algo_bytes) {
+    if (!p->enabled) return;
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!p->pool.empty()) {
+            e = p->pool.back();
+            p->pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    };
+    a = take();
+    b = take();
+    if (a) (void)hipEventRecord(a, s);
+}
+
+ProfScope::~ProfScope() {
+    if (!p->enabled || !a || !b) return;
+    (void)hipEventRecord(b, s);
+    p->pending.push_back({a, b, fam, bytes});
+}
+
+int drain_profiler(Context *ctx) {
+    Profiler &p = ctx->prof;
+    if (p.pending.empty()) return SC_OK;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto &pe : p.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+            p.launches[pe.fam] += 1;
+            p.ms[pe.fam] += ms;
+            p.bytes[pe.fam] += pe.bytes;
+        }
+        p.pool.push_back(pe.a);
+        p.pool.push_back(pe.b);
+    }
+    p.pending.clear();
+    return SC_OK;
+}
+
+int ensure_scratch(Context *ctx, size_t bytes) {
+    if (bytes <= ctx->scratch_bytes) return SC_OK;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch) SC_HIP_CHECK(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SC_E_OOM, "device scratch allocation of " + std::to_string(want) + " bytes failed");
+    }
+    ctx->scratch_bytes = want;
+    return SC_OK;
+}
+
+int ensure_host_scratch(Context *ctx, size_t bytes) {
+    if (bytes <= ctx->scratch_host_bytes) return SC_OK;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->scratch_host) SC_HIP_CHECK(hipHostFree(ctx->scratch_host));
+    ctx->scratch_host = nullptr;
+    ctx->scratch_host_bytes = 0;
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    SC_HIP_CHECK(hipHostMalloc(&ctx->scratch_host, want, hipHostMallocDefault));
+    ctx->scratch_host_bytes = want;
+    return SC_OK;
+}
+
+int DevBuf::alloc(size_t bytes) {
+    if (hipMalloc(&p, std::max(bytes, (size_t)16)) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return fail(SC_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return SC_OK;
+}
+
+namespace {
+
+__global__ void k_set_basis(double2 *amp, int n, int batch, uint64_t index) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch) amp[((uint64_t)b << n) + index] = make_double2(1.0, 0.0);
+}
+
+int set_device(Context *ctx) {
+    SC_HIP_CHECK(hipSetDevice(ctx->device));
+    return SC_OK;
+}
+
+// ---- execution of an analysed op list ----------------------------------------
+int exec_direct(Context *ctx, State *st, const std::vector<PlanOp> &ops, const double *mats) {
+    for (const PlanOp &op : ops) {
+        const double *m = op.mat >= 0 ? mats + 2 * (int64_t)op.mat : nullptr;
+        int rc = SC_OK;
+        switch (op.kind) {
+        case SC_OP_SINGLE:
+        case SC_OP_MCU: {
+            if (op.is_x) {
+                rc = launch_x(ctx, st, op.q0, op.controls);
+            } else {
+                Mat2 g;
+                std::memcpy(g.v, m, sizeof(g.v));
+                rc = launch_single(ctx, st, op.q0, g, op.controls);
+            }
+            break;
+        }
+        case SC_OP_CNOT: rc = launch_x(ctx, st, op.q1, 1ULL << op.q0); break;
+        case SC_OP_TWO: {
+            Mat4 g;
+            std::memcpy(g.v, m, sizeof(g.v));
+            rc = launch_two(ctx, st, op.q0, op.q1, g);
+            break;
+        }
+        case SC_OP_DIAG2: rc = launch_diag2(ctx, st, op.q0, op.q1, m); break;
+        case SC_OP_SWAP: rc = launch_swap(ctx, st, op.q0, op.q1); break;
+        }
+        if (rc) return rc;
+        ctx->last_launches++;
+        ctx->last_sweeps++;
+    }
+    return SC_OK;
+}
+
+int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const double *mats) {
+    std::vector<FusedBlock> blocks;
+    plan_blocks(st->n, ops, std::max(1, std::min(ctx->max_fused, 5)), blocks);
+    // matrices of all blocks in one upload
+    std::vector<double> pool;
+    std::vector<size_t> offs(blocks.size(), 0);
+    for (size_t b = 0; b < blocks.size(); b++) {
+        if (blocks[b].standalone) continue;
+        std::vector<double> m;
+        block_matrix(blocks[b], ops, mats, m);
+        offs[b] = pool.size();
+        pool.insert(pool.end(), m.begin(), m.end());
+    }
+    DevBuf dpool;
+    int rc = dpool.alloc(pool.size() * sizeof(double));
+    if (rc) return rc;
+    if (!pool.empty())
+        SC_HIP_CHECK(hipMemcpyAsync(dpool.p, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    for (size_t b = 0; b < blocks.size(); b++) {
+        const FusedBlock &fb = blocks[b];
+        if (fb.standalone) {
+            std::vector<PlanOp> one{ops[fb.ops[0]]};
+            rc = exec_direct(ctx, st, one, mats);
+        } else {
+            rc = launch_dense(ctx, st, (int)fb.qubits.size(), fb.qubits.data(), (double *)dpool.p + offs[b]);
+            ctx->last_launches++;
+            ctx->last_sweeps++;
+            ctx->last_blocks++;
+        }
+        if (rc) return rc;
+    }
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // dpool lifetime
+    return SC_OK;
+}
+
+// mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
+int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
+    SweepPlan plan;
+    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, plan);
+    if (rc) return rc;
+    if (plan.passes.empty()) return SC_OK;
+    const size_t pass_bytes = plan.passes.size() * sizeof(SweepPass);
+    const size_t ops_bytes = std::max<size_t>(plan.ops.size(), 1) * sizeof(SweepOp);
+    const size_t nelem_pool = mats_elems.size();
+    const size_t pool_bytes = std::max<uint64_t>(plan.pool_complex, 1) * 16 * nelem_pool;
+    const size_t total = pass_bytes + ops_bytes + pool_bytes + 1024;
+    rc = ensure_host_scratch(ctx, total);
+    if (rc) return rc;
+    char *h = (char *)ctx->scratch_host;
+    std::memcpy(h, plan.passes.data(), pass_bytes);
+    if (!plan.ops.empty()) std::memcpy(h + pass_bytes, plan.ops.data(), plan.ops.size() * sizeof(SweepOp));
+    double *hpool = (double *)(h + pass_bytes + ops_bytes);
+    for (size_t e = 0; e < nelem_pool; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e);
+    DevBuf d;
+    rc = d.alloc(total);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemcpyAsync(d.p, h, pass_bytes + ops_bytes + pool_bytes, hipMemcpyHostToDevice, ctx->stream));
+    const SweepPass *dpass = (const SweepPass *)d.p;
+    const SweepOp *dops = (const SweepOp *)((char *)d.p + pass_bytes);
+    const double *dpool = (const double *)((char *)d.p + pass_bytes + ops_bytes);
+    const size_t stride = nelem_pool > 1 ? plan.pool_complex : 0;
+    // keep each launch's grid under ~2^20 blocks
+    const int log_chunks = st->n - kChunkBits;
+    int per_launch = std::max(1, (int)std::min<uint64_t>((uint64_t)st->batch, (1ULL << 20) >> log_chunks));
+    for (size_t p = 0; p < plan.passes.size(); p++) {
+        for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
+            int cnt = std::min(per_launch, st->batch - e0);
+            rc = launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride);
+            if (rc) return rc;
+            ctx->last_launches++;
+        }
+        ctx->last_sweeps++;
+    }
+    ctx->last_blocks += plan.fused_blocks;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // plan buffers die with this call
+    return SC_OK;
+}
+
+}  // namespace
+
+int apply_ops_impl(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                   uint64_t mat_stride, bool per_elem) {
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::vector<const double *> elems;
+    if (per_elem) {
+        for (int b = 0; b < st->batch; b++) elems.push_back(mats + 2 * mat_stride * (uint64_t)b);
+    } else {
+        elems.push_back(mats);
+    }
+    std::vector<PlanOp> plan_ops;
+    rc = analyze_ops(st->n, ops, n_ops, elems, per_elem ? mat_stride : n_mat_complex, plan_ops);
+    if (rc) return rc;
+    ctx->last_sweeps = ctx->last_launches = ctx->last_blocks = 0;
+    if (plan_ops.empty()) return SC_OK;
+    int mode = ctx->exec_mode;
+    if (mode == SC_EXEC_SWEEP && st->n < kChunkBits) mode = SC_EXEC_FUSED;
+    if (per_elem && mode != SC_EXEC_SWEEP) {
+        // element-by-element with a one-element view of the state
+        for (int b = 0; b < st->batch; b++) {
+            State view = *st;
+            view.amp = st->amp + ((uint64_t)b << st->n);
+            view.batch = 1;
+            rc = mode == SC_EXEC_FUSED ? exec_fused(ctx, &view, plan_ops, elems[b])
+                                       : exec_direct(ctx, &view, plan_ops, elems[b]);
+            if (rc) return rc;
+        }
+    } else if (mode == SC_EXEC_DIRECT) {
+        rc = exec_direct(ctx, st, plan_ops, mats);
+    } else if (mode == SC_EXEC_FUSED) {
+        rc = exec_fused(ctx, st, plan_ops, mats);
+    } else {
+        rc = exec_sweep(ctx, st, plan_ops, elems);
+    }
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+}  // namespace sc
+
+using namespace sc;
+
+#define SC_REQUIRE(ptr, name)                                                   \
+    do {                                                                        \
+        if (!(ptr)) return fail(SC_E_NULL, std::string(name) + " is null");     \
+    } while (0)
+
+extern "C" {
+
+const char *sc_last_error(void) { return g_last_error.c_str(); }
+int sc_version(void) { return 10000; }
+
+int sc_ctx_create(int device, sc_ctx **out) {
+    SC_REQUIRE(out, "out");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return fail(SC_E_HIP, "no HIP device is visible");
+    }
+    if (device < 0 || device >= count) return fail(SC_E_ARG, "device index out of range: " + std::to_string(device));
+    SC_HIP_CHECK(hipSetDevice(device));
+    std::unique_ptr<sc_ctx> ctx(new sc_ctx());
+    ctx->device = device;
+    SC_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->cu_count = cus;
+    *out = ctx.release();
+    return SC_OK;
+}
+
+int sc_ctx_destroy(sc_ctx *ctx) {
+    if (!ctx) return SC_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &pe : ctx->prof.pending) {
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->scratch_host) (void)hipHostFree(ctx->scratch_host);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SC_OK;
+}
+
+int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
+    SC_REQUIRE(ctx, "ctx");
+    switch (option) {
+    case SC_OPT_EXEC_MODE:
+        if (value < SC_EXEC_DIRECT || value > SC_EXEC_SWEEP) return fail(SC_E_ARG, "unknown execution mode");
+        ctx->exec_mode = (int)value;
+        return SC_OK;
+    case SC_OPT_MAX_FUSED:
+        if (value < 1 || value > 5) return fail(SC_E_ARG, "max fused qubits must be 1..5");
+        ctx->max_fused = (int)value;
+        return SC_OK;
+    case SC_OPT_PROFILE: {
+        int rc = drain_profiler(ctx);
+        if (rc) return rc;
+        ctx->prof.enabled = value != 0;
+        return SC_OK;
+    }
+    case SC_OPT_LOW_QUBITS:
+        if (value < 0 || value > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+        ctx->low_qubits = (int)value;
+        return SC_OK;
+    case SC_OPT_TEST_DRIFT:
+        if (value < 0) return fail(SC_E_ARG, "test drift must be non-negative");
+        ctx->test_drift = value;
+        return SC_OK;
+    default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
+    }
+}
+
+int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
+    SC_REQUIRE(ctx, "ctx");
+    SC_REQUIRE(value, "value");
+    switch (option) {
+    case SC_OPT_EXEC_MODE: *value = ctx->exec_mode; return SC_OK;
+    case SC_OPT_MAX_FUSED: *value = ctx->max_fused; return SC_OK;
+    case SC_OPT_PROFILE: *value = ctx->prof.enabled ? 1 : 0; return SC_OK;
+    case SC_OPT_LOW_QUBITS: *value = ctx->low_qubits; return SC_OK;
+    case SC_OPT_TEST_DRIFT: *value = ctx->test_drift; return SC_OK;
+    default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
+    }
+}
+
+int sc_ctx_synchronize(sc_ctx *ctx) {
+    SC_REQUIRE(ctx, "ctx");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_ctx_device_info(sc_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes, int *cu_count) {
+    SC_REQUIRE(ctx, "ctx");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    SC_HIP_CHECK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    if (cu_count) *cu_count = ctx->cu_count;
+    return SC_OK;
+}
+
+int sc_state_create(sc_ctx *ctx, int num_qubits, int batch, sc_state **out) {
+    SC_REQUIRE(ctx, "ctx");
+    SC_REQUIRE(out, "out");
+    *out = nullptr;
+    if (num_qubits <= 0) return fail(SC_E_ARG, "qubitCount must be positive");
+    if (num_qubits > 34)
+        return fail(SC_E_UNSUPPORTED,
+                    "statevector-hip engine supports up to 34 qubits in dense mode, got " + std::to_string(num_qubits));
+    if (batch <= 0) return fail(SC_E_ARG, "batch must be positive");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    std::unique_ptr<sc_state> st(new sc_state());
+    st->ctx = ctx;
+    st->n = num_qubits;
+    st->batch = batch;
+    const size_t bytes = ((size_t)batch << num_qubits) * sizeof(double2);
+    if (hipMalloc((void **)&st->amp, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SC_E_OOM, "cannot allocate " + std::to_string(bytes) + " bytes of HBM for " +
+                                  std::to_string(num_qubits) + " qubits x " + std::to_string(batch));
+    }
+    sc_state *raw = st.release();
+    rc = sc_state_set_basis(raw, 0);
+    if (rc) {
+        (void)hipFree(raw->amp);
+        delete raw;
+        return rc;
+    }
+    *out = raw;
+    return SC_OK;
+}
+
+int sc_state_destroy(sc_state *st) {
+    if (!st) return SC_OK;
+    (void)hipSetDevice(st->ctx->device);
+    (void)hipStreamSynchronize(st->ctx->stream);
+    if (st->amp) (void)hipFree(st->amp);
+    delete st;
+    return SC_OK;
+}
+
+int sc_state_qubits(const sc_state *st, int *num_qubits, int *batch) {
+    SC_REQUIRE(st, "state");
+    if (num_qubits) *num_qubits = st->n;
+    if (batch) *batch = st->batch;
+    return SC_OK;
+}
+
+int sc_state_set_basis(sc_state *st, uint64_t index) {
+    SC_REQUIRE(st, "state");
+    if (st->n < 64 && (index >> st->n) != 0) return fail(SC_E_ARG, "basis index out of range");
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemsetAsync(st->amp, 0, ((size_t)st->batch << st->n) * sizeof(double2), ctx->stream));
+    hipLaunchKernelGGL(k_set_basis, dim3((st->batch + 255) / 256), dim3(256), 0, ctx->stream, st->amp, st->n, st->batch,
+                       index);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_state_upload(sc_state *st, const double *interleaved, uint64_t n_complex) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(interleaved, "data");
+    if (n_complex != ((uint64_t)st->batch << st->n))
+        return fail(SC_E_ARG, "Expected state vector of length " + std::to_string((uint64_t)st->batch << st->n) +
+                                  ", got " + std::to_string(n_complex));
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemcpy(st->amp, interleaved, n_complex * sizeof(double2), hipMemcpyHostToDevice));
+    return SC_OK;
+}
+
+int sc_state_download(sc_state *st, double *interleaved, uint64_t n_complex) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(interleaved, "data");
+    if (n_complex != ((uint64_t)st->batch << st->n))
+        return fail(SC_E_ARG, "Expected state vector of length " + std::to_string((uint64_t)st->batch << st->n) +
+                                  ", got " + std::to_string(n_complex));
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+    SC_HIP_CHECK(hipMemcpy(interleaved, st->amp, n_complex * sizeof(double2), hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+int sc_state_copy(sc_state *dst, const sc_state *src) {
+    SC_REQUIRE(dst, "dst");
+    SC_REQUIRE(src, "src");
+    if (dst->n != src->n || dst->batch != src->batch) return fail(SC_E_ARG, "state shapes differ");
+    int rc = set_device(dst->ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemcpyAsync(dst->amp, src->amp, ((size_t)src->batch << src->n) * sizeof(double2),
+                                hipMemcpyDeviceToDevice, dst->ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
+    return SC_OK;
+}
+
+int sc_apply_ops(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex) {
+    SC_REQUIRE(st, "state");
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    return apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, 0, false);
+}
+
+int sc_apply_ops_batched(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t mat_stride) {
+    SC_REQUIRE(st, "state");
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (mat_stride > 0) SC_REQUIRE(mats, "mats");
+    return apply_ops_impl(st, ops, n_ops, mats, mat_stride, mat_stride, true);
+}
+
+// ---- measurement ----------------------------------------------------------------
+static int check_measured(const sc_state *st, int elem, const int32_t *measured, int m) {
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    if (m <= 0) return fail(SC_E_ARG, "measured qubit list must not be empty");
+    if (m > st->n) return fail(SC_E_ARG, "more measured qubits than qubits");
+    for (int k = 0; k < m; k++) {
+        if (measured[k] < 0 || measured[k] >= st->n)
+            return fail(SC_E_ARG, "measurement qubit out of range: " + std::to_string(measured[k]));
+        if (k > 0 && measured[k] <= measured[k - 1])
+            return fail(SC_E_ARG, "measured qubits must be sorted ascending and distinct");
+    }
+    return SC_OK;
+}
+
+static int probabilities_dev(sc_state *st, int elem, const int32_t *measured, int m, double *out_dev) {
+    bool identity = (m == st->n);
+    for (int k = 0; identity && k < m; k++) identity = measured[k] == k;
+    if (identity) return launch_probabilities_full(st->ctx, st, elem, out_dev);
+    return launch_probabilities_marginal(st->ctx, st, elem, measured, m, out_dev);
+}
+
+int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, double *out) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(measured, "measured");
+    SC_REQUIRE(out, "out");
+    int rc = check_measured(st, elem, measured, m);
+    if (rc) return rc;
+    rc = set_device(st->ctx);
+    if (rc) return rc;
+    DevBuf p;
+    const uint64_t count = 1ULL << m;
+    if ((rc = p.alloc(count * sizeof(double)))) return rc;
+    if ((rc = probabilities_dev(st, elem, measured, m, (double *)p.p))) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+    SC_HIP_CHECK(hipMemcpy(out, p.p, count * sizeof(double), hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const double *uniforms, int64_t shots,
+              int64_t *outcomes, double *boundary_gap) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(measured, "measured");
+    if (shots <= 0) return fail(SC_E_ARG, "shots must be positive");
+    SC_REQUIRE(uniforms, "uniforms");
+    SC_REQUIRE(outcomes, "outcomes");
+    int rc = check_measured(st, elem, measured, m);
+    if (rc) return rc;
+    Context *ctx = st->ctx;
+    if ((rc = set_device(ctx))) return rc;
+    const uint64_t count = 1ULL << m;
+    DevBuf p, io;
+    if ((rc = p.alloc(count * sizeof(double)))) return rc;
+    if ((rc = io.alloc((size_t)shots * 24))) return rc;
+    if ((rc = probabilities_dev(st, elem, measured, m, (double *)p.p))) return rc;
+    // StatevectorEngine.normalize (:298-312)
+    double sum = 0.0;
+    if ((rc = device_sum(ctx, (double *)p.p, count, &sum))) return rc;
+    if (sum == 0.0) return fail(SC_E_STATE, "State collapsed to zero probability amplitudes");
+    if (std::fabs(sum - 1.0) > 1e-12)
+        if ((rc = launch_scale(ctx, (double *)p.p, count, 1.0 / sum))) return rc;
+    // cumulative (:314-323)
+    if ((rc = launch_cumulative(ctx, (double *)p.p, count))) return rc;
+    double *u_dev = (double *)io.p;
+    int64_t *o_dev = (int64_t *)((char *)io.p + (size_t)shots * 8);
+    double *g_dev = (double *)((char *)io.p + (size_t)shots * 16);
+    SC_HIP_CHECK(hipMemcpyAsync(u_dev, uniforms, (size_t)shots * 8, hipMemcpyHostToDevice, ctx->stream));
+    {
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
+        if ((rc = launch_search(ctx, (double *)p.p, count, u_dev, shots, o_dev, g_dev))) return rc;
+    }
+    SC_HIP_CHECK(hipMemcpyAsync(outcomes, o_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> gaps;
+    if (boundary_gap) {
+        gaps.resize((size_t)shots);
+        SC_HIP_CHECK(hipMemcpyAsync(gaps.data(), g_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (boundary_gap) *boundary_gap = gaps.empty() ? 0.0 : *std::min_element(gaps.begin(), gaps.end());
+    return SC_OK;
+}
+
+int sc_norm_sq(sc_state *st, int elem, double *out) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(out, "out");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    return launch_norm_sq(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), out);
+}
+
+int sc_probability_checksum(sc_state *st, int elem, double out[2]) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(out, "out");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    return launch_checksum(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), out);
+}
+
+// ---- noise ------------------------------------------------------------------------
+int sc_kraus_weights(sc_state *st, int elem, int target, const double *kraus, int k, double *weights) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(kraus, "kraus");
+    SC_REQUIRE(weights, "weights");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    if (target < 0 || target >= st->n) return fail(SC_E_ARG, "noise channel qubit out of range: " + std::to_string(target));
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    return launch_kraus_weights(st->ctx, st->amp + ((uint64_t)elem << st->n), st->n, target, kraus, k, weights);
+}
+
+int sc_apply_kraus(sc_state *st, int elem, int target, const double *kraus_op, double *norm_sq_out) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(kraus_op, "kraus_op");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    if (target < 0 || target >= st->n) return fail(SC_E_ARG, "noise channel qubit out of range: " + std::to_string(target));
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    State view = *st;
+    view.amp = st->amp + ((uint64_t)elem << st->n);
+    view.batch = 1;
+    Mat2 g;
+    std::memcpy(g.v, kraus_op, sizeof(g.v));
+    if ((rc = launch_single(ctx, &view, target, g, 0))) return rc;
+    // renormalizeState (StatevectorEngine.java:590-600)
+    double normSq = 0.0;
+    if ((rc = launch_norm_sq(ctx, view.amp, st->dim(), &normSq))) return rc;
+    if (norm_sq_out) *norm_sq_out = normSq;
+    if (!std::isfinite(normSq) || normSq == 0.0)
+        return fail(SC_E_STATE, "state collapsed to invalid norm after Kraus application: normSq=" + std::to_string(normSq));
+    if (std::fabs(normSq - 1.0) > 1e-12) {
+        if ((rc = launch_scale_state(ctx, view.amp, st->dim(), 1.0 / std::sqrt(normSq)))) return rc;
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return SC_OK;
+}
+
+// ---- tracing ----------------------------------------------------------------------
+int sc_profile_reset(sc_ctx *ctx) {
+    SC_REQUIRE(ctx, "ctx");
+    int rc = drain_profiler(ctx);
+    if (rc) return rc;
+    for (int f = 0; f < SC_KFAM_COUNT; f++) {
+        ctx->prof.launches[f] = 0;
+        ctx->prof.ms[f] = 0;
+        ctx->prof.bytes[f] = 0;
+    }
+    return SC_OK;
+}
+
+int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_ms, double *bytes) {
+    SC_REQUIRE(ctx, "ctx");
+    if (family < 0 || family >= SC_KFAM_COUNT) return fail(SC_E_ARG, "unknown kernel family");
+    int rc = drain_profiler(ctx);
+    if (rc) return rc;
+    if (launches) *launches = ctx->prof.launches[family];
+    if (total_ms) *total_ms = ctx->prof.ms[family];
+    if (bytes) *bytes = ctx->prof.bytes[family];
+    return SC_OK;
+}
+
+int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                        uint64_t n_mat_complex, int low_qubits, uint64_t *passes, uint64_t *stages,
+                        uint64_t *fused_blocks, double *state_inout) {
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
+    if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
+    if (rc) return rc;
+    SweepPlan plan;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, plan))) return rc;
+    uint64_t ns = 0;
+    for (const auto &p : plan.passes) ns += (uint64_t)p.nstages;
+    if (passes) *passes = plan.passes.size();
+    if (stages) *stages = ns;
+    if (fused_blocks) *fused_blocks = (uint64_t)plan.fused_blocks;
+    if (state_inout) {
+        std::vector<double> pool(2 * std::max<uint64_t>(plan.pool_complex, 1));
+        build_pool(plan, mats, pool.data());
+        execute_plan_host(num_qubits, plan, pool.data(), state_inout);
+    }
+    return SC_OK;
+}
+
+int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks) {
+    SC_REQUIRE(ctx, "ctx");
+    if (sweeps) *sweeps = ctx->last_sweeps;
+    if (kernel_launches) *kernel_launches = ctx->last_launches;
+    if (fused_blocks) *fused_blocks = ctx->last_blocks;
+    return SC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,897 @@
+// sc_plan.cpp -- validation, gate fusion and sweep planning (host only).
+//
+// Dependency model.  Every op acts on each of its qubits in one of two roles:
+// "target" (non-diagonal action) or "diagonal" (controls, phases).  Two ops
+// commute when every qubit they share is diagonal for both.  Planners scan
+// the remaining ops in program order and take an op into the current unit
+// (fused block, sweep pass, or stage) only when no earlier op left behind
+// conflicts with it; the units are therefore dependency-closed and executing
+// them in order reproduces the reference's sequential op loop
+// (StatevectorEngine.java:90-151) up to floating-point reassociation.
+#include "sc_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+namespace sc {
+
+namespace {
+
+int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+bool is_exact(double v, double want) { return v == want; }
+
+// 2x2 complex g (8 doubles): diagonal when both off-diagonal entries are exactly zero.
+bool mat_is_diag(const double *g) { return g[2] == 0.0 && g[3] == 0.0 && g[4] == 0.0 && g[5] == 0.0; }
+bool mat_is_x(const double *g) {
+    return is_exact(g[0], 0.0) && is_exact(g[1], 0.0) && is_exact(g[2], 1.0) && is_exact(g[3], 0.0) &&
+           is_exact(g[4], 1.0) && is_exact(g[5], 0.0) && is_exact(g[6], 0.0) && is_exact(g[7], 0.0);
+}
+
+int mat_count(int kind) {
+    switch (kind) {
+    case SC_OP_SINGLE:
+    case SC_OP_MCU:
+    case SC_OP_DIAG2: return 4;
+    case SC_OP_TWO: return 16;
+    default: return 0;
+    }
+}
+
+struct Blocker {
+    uint64_t bt = 0, bd = 0;
+    bool blocked(const PlanOp &op) const { return ((op.targets | op.diag) & bt) || (op.targets & bd); }
+    void mark(const PlanOp &op) {
+        bt |= op.targets;
+        bd |= op.diag;
+    }
+};
+
+}  // namespace
+
+bool ops_conflict(const PlanOp &a, const PlanOp &b) {
+    return (a.targets & (b.targets | b.diag)) || (b.targets & a.diag);
+}
+
+int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const double *> &mats_all,
+                uint64_t n_mat_complex, std::vector<PlanOp> &out) {
+    out.clear();
+    out.reserve(n_ops);
+    bool measured = false;
+    auto range = [&](int q, const char *what) -> int {
+        if (q >= n) return fail(SC_E_ARG, std::string(what) + " out of range: " + std::to_string(q));
+        return SC_OK;
+    };
+    for (uint64_t i = 0; i < n_ops; i++) {
+        const sc_op &o = ops[i];
+        PlanOp p;
+        p.kind = o.kind;
+        p.q0 = o.q0;
+        p.q1 = o.q1;
+        p.mat = o.mat;
+        p.src = (int)i;
+        if (o.kind == SC_OP_MEASURE) {
+            // MeasureOperation.validateTargets (QuantumCircuit.java:538-573)
+            if (o.controls == 0) return fail(SC_E_ARG, "at least one qubit must be measured");
+            if (n < 64 && (o.controls >> n) != 0)
+                return fail(SC_E_ARG, "measured qubit out of range: " + std::to_string(63 - __builtin_clzll(o.controls)));
+            measured = true;
+            continue;
+        }
+        if (o.kind < SC_OP_SINGLE || o.kind > SC_OP_MCU)
+            return fail(SC_E_STATE, "Unsupported operation type: " + std::to_string(o.kind));
+        if (measured)
+            return fail(SC_E_UNSUPPORTED, "Unitary operations cannot follow measurement operations in the circuit");
+        int rc = SC_OK;
+        switch (o.kind) {
+        case SC_OP_SINGLE:  // SingleGateOperation :368-383
+            if (o.q0 < 0) return fail(SC_E_ARG, "qubit index must be non-negative");
+            if ((rc = range(o.q0, "target qubit"))) return rc;
+            break;
+        case SC_OP_CNOT:  // CnotOperation :385-409
+            if (o.q0 < 0) return fail(SC_E_ARG, "control qubit index must be non-negative");
+            if (o.q1 < 0) return fail(SC_E_ARG, "target qubit index must be non-negative");
+            if (o.q0 == o.q1) return fail(SC_E_ARG, "control and target qubits must differ");
+            if ((rc = range(o.q0, "control qubit")) || (rc = range(o.q1, "target qubit"))) return rc;
+            break;
+        case SC_OP_TWO:    // TwoQubitGateOperation :411-438
+        case SC_OP_DIAG2:  // TwoQubitDiagonalOperation :440-468
+            if (o.q0 < 0 || o.q1 < 0) return fail(SC_E_ARG, "qubit indices must be non-negative");
+            if (o.q0 == o.q1) return fail(SC_E_ARG, "qubits must be distinct");
+            if ((rc = range(o.q0, "first qubit")) || (rc = range(o.q1, "second qubit"))) return rc;
+            break;
+        case SC_OP_SWAP:  // SwapOperation :470-490
+            if (o.q0 < 0 || o.q1 < 0) return fail(SC_E_ARG, "qubit indices must be non-negative");
+            if (o.q0 == o.q1) return fail(SC_E_ARG, "cannot swap qubit with itself");
+            if ((rc = range(o.q0, "first qubit")) || (rc = range(o.q1, "second qubit"))) return rc;
+            break;
+        case SC_OP_MCU:  // MultiControlOperation :492-536
+            if (o.q0 < 0) return fail(SC_E_ARG, "target qubit index must be non-negative");
+            if (o.controls == 0) return fail(SC_E_ARG, "at least one control qubit is required");
+            if ((o.controls >> o.q0) & 1ULL) return fail(SC_E_ARG, "control and target qubits must differ");
+            if ((rc = range(o.q0, "target qubit"))) return rc;
+            if (n < 64 && (o.controls >> n) != 0)
+                return fail(SC_E_ARG, "control qubit out of range: " + std::to_string(63 - __builtin_clzll(o.controls)));
+            p.controls = o.controls;
+            break;
+        }
+        int cnt = mat_count(o.kind);
+        if (cnt > 0) {
+            if (o.mat < 0 || (uint64_t)o.mat + (uint64_t)cnt > n_mat_complex)
+                return fail(SC_E_ARG, "matrix offset out of range for op " + std::to_string(i));
+        }
+        // roles
+        switch (o.kind) {
+        case SC_OP_SINGLE:
+        case SC_OP_MCU: {
+            bool diag = true, x = true;
+            for (const double *m : mats_all) {
+                const double *g = m + 2 * (int64_t)o.mat;
+                diag = diag && mat_is_diag(g);
+                x = x && mat_is_x(g);
+            }
+            p.is_diag = diag;
+            p.is_x = x;
+            uint64_t tq = 1ULL << o.q0;
+            if (diag) {
+                p.diag = tq | p.controls;
+            } else {
+                p.targets = tq;
+                p.diag = p.controls;
+            }
+            break;
+        }
+        case SC_OP_CNOT:
+            p.is_x = true;
+            p.targets = 1ULL << o.q1;
+            p.diag = 1ULL << o.q0;
+            p.controls = 1ULL << o.q0;
+            break;
+        case SC_OP_TWO:
+        case SC_OP_SWAP: p.targets = (1ULL << o.q0) | (1ULL << o.q1); break;
+        case SC_OP_DIAG2:
+            p.is_diag = true;
+            p.diag = (1ULL << o.q0) | (1ULL << o.q1);
+            break;
+        }
+        out.push_back(p);
+    }
+    return SC_OK;
+}
+
+// ------------------------------------------------------------------ dense block fusion
+void plan_blocks(int n, const std::vector<PlanOp> &ops, int max_k, std::vector<FusedBlock> &blocks) {
+    (void)n;
+    blocks.clear();
+    std::vector<int> remaining(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) remaining[i] = (int)i;
+    while (!remaining.empty()) {
+        uint64_t Q = 0;
+        FusedBlock b;
+        Blocker blk;
+        std::vector<int> next;
+        bool restarted = false;
+        for (size_t ri = 0; ri < remaining.size(); ri++) {
+            int i = remaining[ri];
+            const PlanOp &op = ops[i];
+            uint64_t qs = op.targets | op.diag;
+            bool blocked = blk.blocked(op);
+            if (popc(qs) > max_k) {
+                if (b.ops.empty() && !blocked) {
+                    FusedBlock s;
+                    s.standalone = true;
+                    s.ops.push_back(i);
+                    blocks.push_back(s);
+                    for (size_t rj = ri + 1; rj < remaining.size(); rj++) next.push_back(remaining[rj]);
+                    restarted = true;
+                    break;
+                }
+                blk.mark(op);
+                next.push_back(i);
+                continue;
+            }
+            if (blocked || popc(Q | qs) > max_k) {
+                blk.mark(op);
+                next.push_back(i);
+                continue;
+            }
+            Q |= qs;
+            b.ops.push_back(i);
+        }
+        if (!b.ops.empty()) {
+            for (int q = 0; q < 64; q++)
+                if ((Q >> q) & 1ULL) b.qubits.push_back(q);
+            blocks.push_back(b);
+        }
+        (void)restarted;
+        remaining.swap(next);
+    }
+}
+
+namespace {
+
+// Apply one reference op to a small vector whose local bit j is qubit qubits[j].
+void apply_local(const PlanOp &op, const double *mats, const std::vector<int> &qubits, std::vector<double> &v) {
+    const int k = (int)qubits.size();
+    const uint64_t D = 1ULL << k;
+    auto lp = [&](int q) -> uint64_t {
+        for (int j = 0; j < k; j++)
+            if (qubits[j] == q) return 1ULL << j;
+        return 0;
+    };
+    uint64_t cmask = 0;
+    for (int q = 0; q < 64; q++)
+        if ((op.controls >> q) & 1ULL) cmask |= lp(q);
+    const double *m = op.mat >= 0 ? mats + 2 * (int64_t)op.mat : nullptr;
+    auto c = [&](uint64_t i) { return &v[2 * i]; };
+    switch (op.kind) {
+    case SC_OP_SINGLE:
+    case SC_OP_MCU:
+    case SC_OP_CNOT: {
+        uint64_t t = op.kind == SC_OP_CNOT ? lp(op.q1) : lp(op.q0);
+        for (uint64_t i = 0; i < D; i++) {
+            if ((i & t) || (i & cmask) != cmask) continue;
+            double *a0 = c(i), *a1 = c(i | t);
+            double x0r = a0[0], x0i = a0[1], x1r = a1[0], x1i = a1[1];
+            if (op.kind == SC_OP_CNOT) {
+                a0[0] = x1r, a0[1] = x1i, a1[0] = x0r, a1[1] = x0i;
+            } else {
+                a0[0] = m[0] * x0r - m[1] * x0i + m[2] * x1r - m[3] * x1i;
+                a0[1] = m[0] * x0i + m[1] * x0r + m[2] * x1i + m[3] * x1r;
+                a1[0] = m[4] * x0r - m[5] * x0i + m[6] * x1r - m[7] * x1i;
+                a1[1] = m[4] * x0i + m[5] * x0r + m[6] * x1i + m[7] * x1r;
+            }
+        }
+        break;
+    }
+    case SC_OP_TWO: {
+        uint64_t fm = lp(op.q0), sm = lp(op.q1);
+        for (uint64_t b = 0; b < D; b++) {
+            if ((b & fm) || (b & sm)) continue;
+            uint64_t idx[4] = {b, b | sm, b | fm, b | fm | sm};
+            double xr[4], xi[4];
+            for (int j = 0; j < 4; j++) xr[j] = c(idx[j])[0], xi[j] = c(idx[j])[1];
+            for (int r = 0; r < 4; r++) {
+                double re = 0, im = 0;
+                for (int j = 0; j < 4; j++) {
+                    double mr = m[8 * r + 2 * j], mi = m[8 * r + 2 * j + 1];
+                    re += mr * xr[j] - mi * xi[j];
+                    im += mr * xi[j] + mi * xr[j];
+                }
+                c(idx[r])[0] = re, c(idx[r])[1] = im;
+            }
+        }
+        break;
+    }
+    case SC_OP_DIAG2: {
+        uint64_t fm = lp(op.q0), sm = lp(op.q1);
+        for (uint64_t i = 0; i < D; i++) {
+            int l = ((i & fm) ? 2 : 0) | ((i & sm) ? 1 : 0);
+            double dr = m[2 * l], di = m[2 * l + 1];
+            double xr = c(i)[0], xi = c(i)[1];
+            c(i)[0] = dr * xr - di * xi;
+            c(i)[1] = dr * xi + di * xr;
+        }
+        break;
+    }
+    case SC_OP_SWAP: {
+        uint64_t fm = lp(op.q0), sm = lp(op.q1);
+        for (uint64_t b = 0; b < D; b++) {
+            if ((b & fm) || (b & sm)) continue;
+            std::swap(c(b | fm)[0], c(b | sm)[0]);
+            std::swap(c(b | fm)[1], c(b | sm)[1]);
+        }
+        break;
+    }
+    }
+}
+
+}  // namespace
+
+void block_matrix(const FusedBlock &b, const std::vector<PlanOp> &ops, const double *mats, std::vector<double> &out) {
+    const int k = (int)b.qubits.size();
+    const uint64_t D = 1ULL << k;
+    out.assign(2 * D * D, 0.0);
+    std::vector<double> col(2 * D);
+    for (uint64_t cidx = 0; cidx < D; cidx++) {
+        std::fill(col.begin(), col.end(), 0.0);
+        col[2 * cidx] = 1.0;
+        for (int oi : b.ops) apply_local(ops[oi], mats, b.qubits, col);
+        for (uint64_t r = 0; r < D; r++) {
+            out[2 * (r * D + cidx)] = col[2 * r];
+            out[2 * (r * D + cidx) + 1] = col[2 * r + 1];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sweeps
+namespace {
+
+struct StageBuild {
+    uint64_t R = 0;  // global qubits that are register bits (targets)
+    std::vector<int> ops;
+};
+
+// Rank over GF(2) of the given 4-bit (or narrower) column vectors.
+int gf2_rank(std::vector<uint32_t> cols) {
+    int rank = 0;
+    for (int bit = 3; bit >= 0; bit--) {
+        int piv = -1;
+        for (size_t i = rank; i < cols.size(); i++)
+            if ((cols[i] >> bit) & 1u) {
+                piv = (int)i;
+                break;
+            }
+        if (piv < 0) continue;
+        std::swap(cols[rank], cols[piv]);
+        for (size_t i = 0; i < cols.size(); i++)
+            if ((int)i != rank && ((cols[i] >> bit) & 1u)) cols[i] ^= cols[rank];
+        rank++;
+    }
+    return rank;
+}
+
+// Register-level op semantics on 16 complex values (mirror of the device ops, all conditions true).
+void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */) {
+    auto cm = [](const double *g, const double *x, double *y) {  // y = g * x
+        y[0] = g[0] * x[0] - g[1] * x[1];
+        y[1] = g[0] * x[1] + g[1] * x[0];
+    };
+    switch (op.kind) {
+    case SW_U1:
+    case SW_X: {
+        for (int r0 = 0; r0 < kRegs; r0++) {
+            if (r0 & (1 << op.k0)) continue;
+            if (((uint32_t)r0 & op.rctrl) != op.rctrl) continue;
+            int r1 = r0 | (1 << op.k0);
+            double x[2] = {a[2 * r0], a[2 * r0 + 1]}, y[2] = {a[2 * r1], a[2 * r1 + 1]};
+            if (op.kind == SW_X) {
+                a[2 * r0] = y[0], a[2 * r0 + 1] = y[1], a[2 * r1] = x[0], a[2 * r1 + 1] = x[1];
+            } else {
+                double t0[2], t1[2], t2[2], t3[2];
+                cm(m + 0, x, t0);
+                cm(m + 2, y, t1);
+                cm(m + 4, x, t2);
+                cm(m + 6, y, t3);
+                a[2 * r0] = t0[0] + t1[0], a[2 * r0 + 1] = t0[1] + t1[1];
+                a[2 * r1] = t2[0] + t3[0], a[2 * r1 + 1] = t2[1] + t3[1];
+            }
+        }
+        break;
+    }
+    case SW_U2: {
+        for (int b = 0; b < kRegs; b++) {
+            if (b & ((1 << op.k0) | (1 << op.k1))) continue;
+            int idx[4] = {b, b | (1 << op.k1), b | (1 << op.k0), b | (1 << op.k0) | (1 << op.k1)};
+            double x[8];
+            for (int c = 0; c < 4; c++) x[2 * c] = a[2 * idx[c]], x[2 * c + 1] = a[2 * idx[c] + 1];
+            for (int r = 0; r < 4; r++) {
+                double re = 0, im = 0;
+                for (int c = 0; c < 4; c++) {
+                    double t[2];
+                    cm(m + 2 * (4 * r + c), x + 2 * c, t);
+                    re += t[0], im += t[1];
+                }
+                a[2 * idx[r]] = re, a[2 * idx[r] + 1] = im;
+            }
+        }
+        break;
+    }
+    case SW_SWAP: {
+        for (int r = 0; r < kRegs; r++)
+            if ((r & (1 << op.k0)) && !(r & (1 << op.k1))) {
+                int s = r ^ (1 << op.k0) ^ (1 << op.k1);
+                std::swap(a[2 * r], a[2 * s]);
+                std::swap(a[2 * r + 1], a[2 * s + 1]);
+            }
+        break;
+    }
+    case SW_DIAG: {
+        for (int r = 0; r < kRegs; r++) {
+            if (((uint32_t)r & op.rctrl) != op.rctrl) continue;
+            int ba = ((uint32_t)r & op.da.rmask) ? 1 : 0;
+            int bb = ((uint32_t)r & op.db.rmask) ? 1 : 0;
+            int l = op.nd == 1 ? ba : 2 * ba + bb;
+            double x[2] = {a[2 * r], a[2 * r + 1]}, y[2];
+            cm(m + 2 * l, x, y);
+            a[2 * r] = y[0], a[2 * r + 1] = y[1];
+        }
+        break;
+    }
+    default: break;
+    }
+}
+
+double op_cost(const SweepOp &op) {
+    switch (op.kind) {
+    case SW_U1: return 8.0;
+    case SW_U2: return 16.0;
+    case SW_DIAG: return 3.0;
+    case SW_X: return 1.0;
+    case SW_SWAP: return 0.5;
+    default: return 64.0;
+    }
+}
+
+bool register_internal(const SweepOp &op) {
+    if (op.tctrl != 0u || op.octrl != 0ULL) return false;
+    if (op.kind == SW_DIAG) {
+        bool a_ok = op.da.tmask == 0u && op.da.omask_lo == 0u && op.da.omask_hi == 0u;
+        bool b_ok = op.nd == 1 || (op.db.tmask == 0u && op.db.omask_lo == 0u && op.db.omask_hi == 0u);
+        return a_ok && b_ok;
+    }
+    return op.kind == SW_U1 || op.kind == SW_X || op.kind == SW_U2 || op.kind == SW_SWAP;
+}
+
+}  // namespace
+
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan &plan) {
+    plan = SweepPlan();
+    if (n < kChunkBits) return fail(SC_E_ARG, "sweep plans need at least 12 qubits");
+    const uint64_t full = (n == 64) ? ~0ULL : ((1ULL << n) - 1ULL);
+    const int lb = std::max(0, std::min(low_qubits, kChunkBits));
+    const uint64_t lowmask = (1ULL << lb) - 1ULL;
+
+    std::map<std::pair<int, int>, int> recipe_index;  // (kind, mat) -> recipe
+    auto recipe_for = [&](int kind, int mat, int count) -> int {
+        auto key = std::make_pair(kind, mat);
+        auto it = recipe_index.find(key);
+        if (it != recipe_index.end()) return it->second;
+        PoolRecipe r;
+        r.kind = kind;
+        r.mat = mat;
+        r.count = count;
+        plan.pool.push_back(r);
+        int idx = (int)plan.pool.size() - 1;
+        recipe_index[key] = idx;
+        return idx;
+    };
+    std::vector<uint64_t> recipe_off;  // filled at the end
+
+    std::vector<int> remaining(ops.size());
+    for (size_t i = 0; i < ops.size(); i++) remaining[i] = (int)i;
+    // SweepOps hold recipe indices in .mat until offsets are known
+    while (!remaining.empty()) {
+        // ---- choose the pass's op set and local qubits
+        uint64_t L = lowmask;
+        Blocker blk;
+        std::vector<int> sched, next;
+        for (int i : remaining) {
+            const PlanOp &op = ops[i];
+            if (blk.blocked(op) || popc(L | op.targets) > kChunkBits || popc(op.targets) > kRegBits) {
+                blk.mark(op);
+                next.push_back(i);
+                continue;
+            }
+            L |= op.targets;
+            sched.push_back(i);
+        }
+        // ---- split into stages
+        std::vector<StageBuild> stages;
+        std::vector<int> rest = sched;
+        while (!rest.empty() && (int)stages.size() < kMaxStages) {
+            StageBuild sb;
+            Blocker sblk;
+            std::vector<int> rest2;
+            for (int i : rest) {
+                const PlanOp &op = ops[i];
+                if (sblk.blocked(op) || popc(sb.R | op.targets) > kRegBits) {
+                    sblk.mark(op);
+                    rest2.push_back(i);
+                    continue;
+                }
+                sb.R |= op.targets;
+                sb.ops.push_back(i);
+            }
+            stages.push_back(sb);
+            rest.swap(rest2);
+        }
+        if (!rest.empty()) {  // truncated: return to the pool in program order
+            std::vector<int> merged;
+            std::merge(rest.begin(), rest.end(), next.begin(), next.end(), std::back_inserter(merged));
+            next.swap(merged);
+        }
+        // ---- local qubits: fill to 12 with the lowest unused qubits
+        for (int q = 0; q < n && popc(L) < kChunkBits; q++) L |= 1ULL << q;
+        int lq[kChunkBits];
+        int lp[64];
+        for (int q = 0; q < 64; q++) lp[q] = -1;
+        {
+            int j = 0;
+            for (int q = 0; q < n; q++)
+                if ((L >> q) & 1ULL) {
+                    lq[j] = q;
+                    lp[q] = j;
+                    j++;
+                }
+        }
+        SweepPass pass;
+        std::memset(&pass, 0, sizeof(pass));
+        pass.outer_mask = full & ~L;
+        pass.nstages = (int)stages.size();
+
+        // per stage: register positions / thread positions
+        struct StageLayout {
+            int rpos[kRegBits];
+            int tpos[kThreadBits];
+        };
+        std::vector<StageLayout> layouts(stages.size());
+        for (size_t s = 0; s < stages.size(); s++) {
+            uint32_t rp = 0;
+            for (int q = 0; q < n; q++)
+                if ((stages[s].R >> q) & 1ULL) rp |= 1u << lp[q];
+            for (int p = kChunkBits - 1; p >= 0 && popc(rp) < kRegBits; p--)
+                if (!((rp >> p) & 1u)) rp |= 1u << p;
+            int rj = 0, tj = 0;
+            for (int p = 0; p < kChunkBits; p++) {
+                if ((rp >> p) & 1u) layouts[s].rpos[rj++] = p;
+                else layouts[s].tpos[tj++] = p;
+            }
+        }
+        // ---- swizzle: hv[p] for local bits p >= kSwzBits, chosen so every stage's lanes 0..3 (and 0..2)
+        // land on distinct low slot bits.
+        uint32_t hv[kChunkBits] = {0};
+        {
+            uint64_t rng = 0x243F6A8885A308D3ULL;
+            auto next_rand = [&]() {
+                rng ^= rng << 13;
+                rng ^= rng >> 7;
+                rng ^= rng << 17;
+                return rng;
+            };
+            int best_bad = 1 << 30;
+            uint32_t best[kChunkBits] = {0};
+            for (int attempt = 0; attempt < 4000 && best_bad > 0; attempt++) {
+                uint32_t cand[kChunkBits] = {0};
+                for (int p = kSwzBits; p < kChunkBits; p++) cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & 15u);
+                int bad = 0;
+                for (size_t s = 0; s < stages.size(); s++) {
+                    std::vector<uint32_t> c16, c8;
+                    for (int i = 0; i < 4; i++) {
+                        int p = layouts[s].tpos[i];
+                        uint32_t col = p < kSwzBits ? (1u << p) : cand[p];
+                        c16.push_back(col);
+                        if (i < 3) c8.push_back(col & 7u);
+                    }
+                    if (gf2_rank(c16) < 4) bad += 2;
+                    if (gf2_rank(c8) < 3) bad += 1;
+                }
+                if (bad < best_bad) {
+                    best_bad = bad;
+                    std::memcpy(best, cand, sizeof(best));
+                }
+            }
+            std::memcpy(hv, best, sizeof(hv));
+        }
+        auto slot_of = [&](uint32_t l) -> uint32_t {
+            uint32_t h = 0;
+            for (int p = kSwzBits; p < kChunkBits; p++)
+                if ((l >> p) & 1u) h ^= hv[p];
+            return l ^ h;
+        };
+        // load / store phase layout: thread bit i -> local i, register bit j -> local 8 + j
+        for (int i = 0; i < kThreadBits; i++) {
+            pass.glob_thread[i] = 1ULL << lq[i];
+            pass.io_thread_slot[i] = (uint16_t)slot_of(1u << i);
+        }
+        for (int r = 0; r < kRegs; r++) {
+            uint64_t g = 0;
+            uint32_t l = 0;
+            for (int j = 0; j < kRegBits; j++)
+                if ((r >> j) & 1) {
+                    g |= 1ULL << lq[kThreadBits + j];
+                    l |= 1u << (kThreadBits + j);
+                }
+            pass.glob_reg[r] = g;
+            pass.io_reg_slot[r] = (uint16_t)slot_of(l);
+        }
+        // ---- stage ops
+        for (size_t s = 0; s < stages.size(); s++) {
+            SweepStage &stg = pass.stages[s];
+            const StageLayout &lay = layouts[s];
+            for (int i = 0; i < kThreadBits; i++) stg.thread_slot[i] = (uint16_t)slot_of(1u << lay.tpos[i]);
+            for (int r = 0; r < kRegs; r++) {
+                uint32_t l = 0;
+                for (int j = 0; j < kRegBits; j++)
+                    if ((r >> j) & 1) l |= 1u << lay.rpos[j];
+                stg.lds_off[r] = (uint16_t)slot_of(l);
+            }
+            auto reg_of = [&](int q) -> int {
+                int p = lp[q];
+                for (int j = 0; j < kRegBits; j++)
+                    if (lay.rpos[j] == p) return j;
+                return -1;
+            };
+            auto src_of = [&](int q) -> BitSrc {
+                BitSrc b;
+                std::memset(&b, 0, sizeof(b));
+                if (lp[q] >= 0) {
+                    int j = reg_of(q);
+                    if (j >= 0) {
+                        b.rmask = 1u << j;
+                    } else {
+                        for (int i = 0; i < kThreadBits; i++)
+                            if (lay.tpos[i] == lp[q]) b.tmask = 1u << i;
+                    }
+                } else {
+                    uint64_t om = 1ULL << q;
+                    b.omask_lo = (uint32_t)om;
+                    b.omask_hi = (uint32_t)(om >> 32);
+                }
+                return b;
+            };
+            auto add_controls = [&](SweepOp &so, uint64_t controls) {
+                for (int q = 0; q < n; q++) {
+                    if (!((controls >> q) & 1ULL)) continue;
+                    BitSrc b = src_of(q);
+                    so.rctrl |= b.rmask;
+                    so.tctrl |= b.tmask;
+                    so.octrl |= ((uint64_t)b.omask_hi << 32) | b.omask_lo;
+                }
+            };
+            std::vector<SweepOp> sops;
+            for (int oi : stages[s].ops) {
+                const PlanOp &op = ops[oi];
+                SweepOp so;
+                std::memset(&so, 0, sizeof(so));
+                so.mat = -1;
+                switch (op.kind) {
+                case SC_OP_SINGLE:
+                case SC_OP_MCU:
+                    if (op.is_diag) {
+                        so.kind = SW_DIAG;
+                        so.nd = 1;
+                        so.da = src_of(op.q0);
+                        so.mat = recipe_for(1, op.mat, 2);
+                        add_controls(so, op.controls);
+                    } else if (op.is_x) {
+                        so.kind = SW_X;
+                        so.k0 = reg_of(op.q0);
+                        add_controls(so, op.controls);
+                    } else {
+                        so.kind = SW_U1;
+                        so.k0 = reg_of(op.q0);
+                        so.mat = recipe_for(0, op.mat, 4);
+                        add_controls(so, op.controls);
+                    }
+                    break;
+                case SC_OP_CNOT:
+                    so.kind = SW_X;
+                    so.k0 = reg_of(op.q1);
+                    add_controls(so, 1ULL << op.q0);
+                    break;
+                case SC_OP_TWO:
+                    so.kind = SW_U2;
+                    so.k0 = reg_of(op.q0);
+                    so.k1 = reg_of(op.q1);
+                    so.mat = recipe_for(0, op.mat, 16);
+                    break;
+                case SC_OP_DIAG2:
+                    so.kind = SW_DIAG;
+                    so.nd = 2;
+                    so.da = src_of(op.q0);
+                    so.db = src_of(op.q1);
+                    so.mat = recipe_for(0, op.mat, 4);
+                    break;
+                case SC_OP_SWAP:
+                    so.kind = SW_SWAP;
+                    so.k0 = reg_of(op.q0);
+                    so.k1 = reg_of(op.q1);
+                    break;
+                }
+                sops.push_back(so);
+            }
+            // ---- dense fusion of register-internal runs when cheaper than op-by-op
+            std::vector<SweepOp> fused;
+            size_t i = 0;
+            while (i < sops.size()) {
+                if (!register_internal(sops[i])) {
+                    fused.push_back(sops[i]);
+                    i++;
+                    continue;
+                }
+                size_t j = i;
+                double cost = 0.0;
+                while (j < sops.size() && register_internal(sops[j])) cost += op_cost(sops[j++]);
+                if (cost > 72.0 && j - i >= 2) {
+                    PoolRecipe r;
+                    r.kind = 2;
+                    r.mat = -1;
+                    r.count = kRegs * kRegs;
+                    for (size_t t = i; t < j; t++) {
+                        r.fused_ops.push_back(sops[t]);
+                        r.fused_srcs.push_back(sops[t].mat);
+                    }
+                    plan.pool.push_back(r);
+                    SweepOp d;
+                    std::memset(&d, 0, sizeof(d));
+                    d.kind = SW_DENSE4;
+                    d.mat = (int)plan.pool.size() - 1;
+                    fused.push_back(d);
+                    plan.fused_blocks++;
+                } else {
+                    for (size_t t = i; t < j; t++) fused.push_back(sops[t]);
+                }
+                i = j;
+            }
+            stg.op_begin = (int32_t)plan.ops.size();
+            for (auto &so : fused) plan.ops.push_back(so);
+            stg.op_end = (int32_t)plan.ops.size();
+        }
+        plan.passes.push_back(pass);
+        remaining.swap(next);
+    }
+    // ---- pool offsets: replace recipe indices by complex offsets
+    recipe_off.resize(plan.pool.size());
+    uint64_t off = 0;
+    for (size_t r = 0; r < plan.pool.size(); r++) {
+        recipe_off[r] = off;
+        off += (uint64_t)plan.pool[r].count;
+    }
+    plan.pool_complex = off;
+    for (auto &so : plan.ops)
+        if (so.mat >= 0) so.mat = (int32_t)recipe_off[so.mat];
+    // fused recipes keep recipe indices in fused_srcs; record their offsets in the SweepOps' .mat for build_pool
+    for (auto &r : plan.pool)
+        for (size_t t = 0; t < r.fused_ops.size(); t++) r.fused_ops[t].mat = r.fused_srcs[t];
+    return SC_OK;
+}
+
+namespace {
+
+bool host_bit_from(const BitSrc &s, uint32_t tid, uint64_t chunk_base) {
+    uint64_t om = ((uint64_t)s.omask_hi << 32) | s.omask_lo;
+    return ((tid & s.tmask) != 0u) || ((chunk_base & om) != 0ULL);
+}
+
+// Mirror of the device run_op (sc_sweep.hip) on one thread's 16 registers.
+void host_run_op(const SweepOp &op, const double *pool, double *a, uint32_t tid, uint64_t chunk_base) {
+    const double *m = op.mat >= 0 ? pool + 2 * (int64_t)op.mat : nullptr;
+    const bool ctrl = (op.rctrl | op.tctrl) != 0u || op.octrl != 0ULL;
+    const bool tcond = ((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl);
+    auto cm = [](const double *g, const double *x, double *y) {
+        y[0] = g[0] * x[0] - g[1] * x[1];
+        y[1] = g[0] * x[1] + g[1] * x[0];
+    };
+    switch (op.kind) {
+    case SW_U1:
+    case SW_X: {
+        if (!tcond) return;
+        SweepOp r = op;
+        if (!ctrl) r.rctrl = 0;
+        host_reg_op(r, m, a);
+        return;
+    }
+    case SW_DIAG: {
+        bool ta = host_bit_from(op.da, tid, chunk_base), tb = host_bit_from(op.db, tid, chunk_base);
+        for (int r = 0; r < kRegs; r++) {
+            if (ctrl && (((uint32_t)r & op.rctrl) != op.rctrl || !tcond)) continue;
+            int ba = (((uint32_t)r & op.da.rmask) != 0u || ta) ? 1 : 0;
+            int bb = (((uint32_t)r & op.db.rmask) != 0u || tb) ? 1 : 0;
+            int l = op.nd == 1 ? ba : 2 * ba + bb;
+            double x[2] = {a[2 * r], a[2 * r + 1]}, y[2];
+            cm(m + 2 * l, x, y);
+            a[2 * r] = y[0], a[2 * r + 1] = y[1];
+        }
+        return;
+    }
+    case SW_DENSE4: {
+        double y[2 * kRegs];
+        for (int r = 0; r < kRegs; r++) {
+            double re = 0, im = 0;
+            for (int c = 0; c < kRegs; c++) {
+                double t[2];
+                cm(m + 2 * (r * kRegs + c), a + 2 * c, t);
+                re += t[0], im += t[1];
+            }
+            y[2 * r] = re, y[2 * r + 1] = im;
+        }
+        std::memcpy(a, y, sizeof(y));
+        return;
+    }
+    default: host_reg_op(op, m, a); return;
+    }
+}
+
+}  // namespace
+
+void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state) {
+    const int log_chunks = n - kChunkBits;
+    std::vector<double> lds(2 * kChunk);
+    double a[2 * kRegs];
+    for (const SweepPass &pass : plan.passes) {
+        for (uint64_t chunk = 0; chunk < (1ULL << log_chunks); chunk++) {
+            uint64_t chunk_base = 0, om = pass.outer_mask, c = chunk;
+            while (om) {
+                uint64_t low = om & (~om + 1ULL);
+                if (c & 1ULL) chunk_base |= low;
+                c >>= 1;
+                om &= om - 1ULL;
+            }
+            for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                uint64_t g = 0;
+                uint32_t s = 0;
+                for (int i = 0; i < kThreadBits; i++)
+                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
+                for (int r = 0; r < kRegs; r++) {
+                    uint64_t gi = chunk_base + g + pass.glob_reg[r];
+                    uint32_t sl = s ^ pass.io_reg_slot[r];
+                    lds[2 * sl] = state[2 * gi];
+                    lds[2 * sl + 1] = state[2 * gi + 1];
+                }
+            }
+            for (int st = 0; st < pass.nstages; st++) {
+                const SweepStage &stg = pass.stages[st];
+                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                    uint32_t ts = 0;
+                    for (int i = 0; i < kThreadBits; i++)
+                        if ((tid >> i) & 1u) ts ^= stg.thread_slot[i];
+                    for (int r = 0; r < kRegs; r++) {
+                        uint32_t sl = ts ^ stg.lds_off[r];
+                        a[2 * r] = lds[2 * sl], a[2 * r + 1] = lds[2 * sl + 1];
+                    }
+                    for (int o = stg.op_begin; o < stg.op_end; o++) host_run_op(plan.ops[o], pool, a, tid, chunk_base);
+                    for (int r = 0; r < kRegs; r++) {
+                        uint32_t sl = ts ^ stg.lds_off[r];
+                        lds[2 * sl] = a[2 * r], lds[2 * sl + 1] = a[2 * r + 1];
+                    }
+                }
+            }
+            for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                uint64_t g = 0;
+                uint32_t s = 0;
+                for (int i = 0; i < kThreadBits; i++)
+                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
+                for (int r = 0; r < kRegs; r++) {
+                    uint64_t gi = chunk_base + g + pass.glob_reg[r];
+                    uint32_t sl = s ^ pass.io_reg_slot[r];
+                    state[2 * gi] = lds[2 * sl];
+                    state[2 * gi + 1] = lds[2 * sl + 1];
+                }
+            }
+        }
+    }
+}
+
+void build_pool(const SweepPlan &plan, const double *mats, double *out) {
+    // first pass: plain recipes (kinds 0, 1) at their offsets
+    std::vector<uint64_t> offs(plan.pool.size());
+    uint64_t off = 0;
+    for (size_t r = 0; r < plan.pool.size(); r++) {
+        offs[r] = off;
+        off += (uint64_t)plan.pool[r].count;
+    }
+    for (size_t r = 0; r < plan.pool.size(); r++) {
+        const PoolRecipe &rc = plan.pool[r];
+        double *dst = out + 2 * offs[r];
+        if (rc.kind == 0) {
+            std::memcpy(dst, mats + 2 * (int64_t)rc.mat, sizeof(double) * 2 * rc.count);
+        } else if (rc.kind == 1) {
+            const double *g = mats + 2 * (int64_t)rc.mat;
+            dst[0] = g[0], dst[1] = g[1], dst[2] = g[6], dst[3] = g[7];
+        }
+    }
+    // second pass: fused dense4 blocks from the plain entries
+    for (size_t r = 0; r < plan.pool.size(); r++) {
+        const PoolRecipe &rc = plan.pool[r];
+        if (rc.kind != 2) continue;
+        double *dst = out + 2 * offs[r];
+        for (int c = 0; c < kRegs; c++) {
+            double col[2 * kRegs] = {0};
+            col[2 * c] = 1.0;
+            for (size_t t = 0; t < rc.fused_ops.size(); t++) {
+                const SweepOp &so = rc.fused_ops[t];
+                const double *m = so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr;
+                host_reg_op(so, m, col);
+            }
+            for (int row = 0; row < kRegs; row++) {
+                dst[2 * (row * kRegs + c)] = col[2 * row];
+                dst[2 * (row * kRegs + c) + 1] = col[2 * row + 1];
+            }
+        }
+    }
+}
+
+}  // namespace sc

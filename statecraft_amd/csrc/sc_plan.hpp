@@ -1,0 +1,70 @@
+// sc_plan.hpp -- host-side circuit analysis, gate fusion and sweep planning.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sc_internal.hpp"
+
+namespace sc {
+
+// One reference op after validation, with its qubit roles.
+struct PlanOp {
+    int kind = 0;           // SC_OP_*
+    int q0 = -1, q1 = -1;
+    uint64_t controls = 0;  // SC_OP_MCU
+    int mat = -1;           // complex offset into the caller's matrix array
+    uint64_t targets = 0;   // qubits acted on non-diagonally
+    uint64_t diag = 0;      // qubits acted on diagonally (controls, diagonal gates)
+    bool is_x = false;      // gate is exactly Pauli X (pure exchange)
+    bool is_diag = false;   // whole op is diagonal
+    int src = -1;           // index into the caller's op array
+};
+
+// Validate ops against n qubits (QuantumCircuit.Operation.validateTargets, QuantumCircuit.java:362-574) and
+// classify them.  `mats_all` lists every batch element's matrix array (one entry when shared) so diagonal /
+// exchange classification holds for all of them.  Measurement records are dropped; a unitary after one fails
+// with SC_E_UNSUPPORTED (StatevectorEngine.java:92-95).
+int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const double *> &mats_all,
+                uint64_t n_mat_complex, std::vector<PlanOp> &out);
+
+bool ops_conflict(const PlanOp &a, const PlanOp &b);
+
+// ---- dense 2^k block fusion (SC_EXEC_FUSED) --------------------------------
+struct FusedBlock {
+    std::vector<int> qubits;  // ascending; local bit j <-> qubits[j]
+    std::vector<int> ops;     // indices into the PlanOp list, program order
+    bool standalone = false;  // op too wide for a block: launched with its own kernel
+};
+void plan_blocks(int n, const std::vector<PlanOp> &ops, int max_k, std::vector<FusedBlock> &blocks);
+// Row-major 2^k x 2^k complex matrix of a block for one matrix array.
+void block_matrix(const FusedBlock &b, const std::vector<PlanOp> &ops, const double *mats, std::vector<double> &out);
+
+// ---- LDS-blocked sweeps (SC_EXEC_SWEEP) ------------------------------------
+// A matrix-pool entry: how to build `count` complex values from the caller's matrices.
+struct PoolRecipe {
+    int kind;        // 0 copy `count` entries from mat; 1 diag of a 2x2 (entries 0 and 3); 2 fused dense4 of ops
+    int mat;         // source offset (kinds 0, 1)
+    int count;
+    std::vector<SweepOp> fused_ops;  // kind 2: register ops in order (their .mat index into `fused_mats`)
+    std::vector<int> fused_srcs;     // kind 2: per fused op, recipe index of its matrix (-1 none)
+};
+
+struct SweepPlan {
+    std::vector<SweepPass> passes;
+    std::vector<SweepOp> ops;      // all passes' ops (stage ranges index this)
+    std::vector<PoolRecipe> pool;  // recipes; SweepOp.mat = complex offset of the recipe's output
+    uint64_t pool_complex = 0;
+    std::vector<int> standalone;   // unused for now (every op kind fits a sweep)
+    int fused_blocks = 0;
+};
+
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan &plan);
+// Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
+void build_pool(const SweepPlan &plan, const double *mats, double *out);
+// Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS
+// slots, per-thread register sets, stage ops, store phase).  Used to validate plans on the CPU.
+void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state);
+
+}  // namespace sc

@@ -1,0 +1,207 @@
+"""GPU parity for noisy trajectories, Pauli expectations and adjoint gradients.
+
+Noise: the engine's per-channel path and the batched trajectory kernels against the oracle's restatement of
+StatevectorEngine.applyErrorChannel with the same SplittableRandom streams (1e-12).
+Expectations: the oracle's Pauli-sum evaluation (1e-12).
+Gradients: parameter-shift on the oracle (1e-10) and central finite differences h=1e-6 (1e-7, the accuracy
+a double-precision central difference can reach).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from circuits import random_circuit, random_state
+from statecraft_amd import capi, workloads
+from statecraft_amd.circuit import Hadamard, PauliX, QuantumCircuit, lower
+from statecraft_amd.engine import HipStatevectorEngine, MeasurementInstruction, SimulationRequest
+from statecraft_amd.noise import ErrorChannel, NoiseModel
+from statecraft_amd.rng import SplittableRandom
+
+pytestmark = pytest.mark.gpu
+EPS = 1e-12
+
+
+def oracle_noisy(circuit, model, seed):
+    """Oracle run of a noisy circuit with the schedule NoiseModel.scheduledChannelsAfter yields."""
+    n = circuit.qubit_count
+    ops, mats = lower(circuit)
+    s = oracle.zero_state(n)
+    holder = [seed]
+    for i, op in enumerate(circuit.operations):
+        if op.kind == capi.OP_MEASURE:
+            continue
+        s = oracle.run_ops(n, ops[i:i + 1], mats, s)
+        for sch in model.scheduled_channels_after(op):
+            chans = sch.channel.channels if hasattr(sch.channel, "channels") else [sch.channel]
+            for ch in chans:
+                targets = sch.target_qubits if not hasattr(sch.channel, "channels") or len(sch.target_qubits) == 1 \
+                    else ch.affected_qubits()
+                for t in targets:
+                    oracle.apply_channel_1q(s, t, ch.kraus, holder)
+    return s
+
+
+def test_engine_noisy_simulation_matches_oracle():
+    circuit = (QuantumCircuit(5).append(Hadamard(), 0).append(Hadamard(), 3).append_cnot(3, 1).append_swap(0, 4)
+               .append_controlled_phase(math.pi / 5.0, 4, 2).append_multi_control(PauliX(), 2, 0, 1, 3))
+    model = (NoiseModel().after_all_gates(ErrorChannel.phase_flip(0.35, 0))
+             .after_all_gates(ErrorChannel.amplitude_damping(0.2, 3))
+             .on_qubits(ErrorChannel.depolarizing(0.3, 0), 4, 1))
+    eng = HipStatevectorEngine()
+    for seed in (99, 100, 101, 7):
+        res = eng.simulate(SimulationRequest.zero_state(circuit).with_noise_model(model).with_noise_seed(seed))
+        want = oracle_noisy(circuit, model, seed)
+        assert np.max(np.abs(res.final_state - want)) <= EPS, seed
+
+
+def test_engine_golden_noise_histogram():
+    # StatevectorNoiseTest golden: X then amplitude damping gamma=1 always relaxes to |0>: counts {0: 16}
+    circuit = QuantumCircuit(1).append(PauliX(), 0)
+    model = NoiseModel().after_all_gates(ErrorChannel.amplitude_damping(1.0, 0))
+    req = (SimulationRequest.zero_state(circuit).with_noise_model(model).with_noise_seed(3)
+           .with_measurement(MeasurementInstruction.counts_all(16).with_seed(1)))
+    res = HipStatevectorEngine().simulate(req)
+    assert res.measurement.counts == {0: 16}
+
+
+def test_engine_measurement_counts_bit_identical():
+    circ, _ = workloads.ry_qft(12, 42)
+    req = SimulationRequest.zero_state(circ).with_measurement(MeasurementInstruction.counts_all(4096).with_seed(42))
+    res = HipStatevectorEngine().simulate(req)
+    ops, mats = lower(circ)
+    s = oracle.run_ops(12, ops, mats)
+    out = oracle.sample(s, list(range(12)), SplittableRandom(42).next_doubles(4096))
+    vals, cnts = np.unique(out, return_counts=True)
+    assert res.measurement.counts == {int(v): int(c) for v, c in zip(vals, cnts)}
+
+
+@pytest.mark.parametrize("n,B,p", [(12, 8, 0.1), (14, 5, 0.3)])
+def test_batched_trajectories(hip_ctx, n, B, p):
+    circuit = workloads.ghz(n)
+    ops, mats = lower(circuit)
+    noise_after = (ops["kind"] == capi.OP_CNOT).astype(np.uint8)
+    n_draws = 2 * int(noise_after.sum())
+    seeds = [1000 + b for b in range(B)]
+    u = np.stack([SplittableRandom(s).next_doubles(n_draws) for s in seeds])
+    st = capi.State(hip_ctx, n, B)
+    st.run_depolarizing_trajectories(ops, mats, noise_after, p, u)
+    got = st.download()
+    dep = ErrorChannel.depolarizing(p, 0).kraus
+    for b in range(B):
+        s = oracle.zero_state(n)
+        holder = [seeds[b]]
+        for i in range(len(ops)):
+            s = oracle.run_ops(n, ops[i:i + 1], mats, s)
+            if noise_after[i]:
+                oracle.apply_channel_1q(s, int(ops[i]["q0"]), dep, holder)
+                oracle.apply_channel_1q(s, int(ops[i]["q1"]), dep, holder)
+        assert np.max(np.abs(got[b] - s)) <= EPS, b
+    # trajectory-averaged GHZ parity <Z^n>
+    par = st.expectation([0], [(1 << n) - 1], [1.0])
+    for b in range(B):
+        assert abs(par[b] - oracle.expect_pauli(got[b], [0], [(1 << n) - 1], [1.0])) <= EPS
+    st.close()
+
+
+def test_pauli_expectations(hip_ctx):
+    rng = np.random.default_rng(17)
+    n, B = 13, 3
+    states = np.stack([random_state(n, rng) for _ in range(B)])
+    xs = [int(x) for x in rng.integers(0, 1 << n, 12)]
+    zs = [int(z) for z in rng.integers(0, 1 << n, 12)]
+    cs = rng.normal(size=12)
+    st = capi.State(hip_ctx, n, B)
+    st.upload(states)
+    got = st.expectation(xs, zs, cs)
+    for b in range(B):
+        assert abs(got[b] - oracle.expect_pauli(states[b], xs, zs, cs)) <= EPS
+    st.close()
+
+
+def hea_energy(n, layers, thetas, obs):
+    ops, mats = lower(workloads.hea(n, layers, thetas))
+    return oracle.expect_pauli(oracle.run_ops(n, ops, mats), *obs)
+
+
+@pytest.mark.parametrize("n,layers,ckpt", [(6, 2, 0), (12, 2, 10), (13, 3, 25)])
+def test_adjoint_hea_vs_parameter_shift(hip_ctx, n, layers, ckpt):
+    rng = np.random.default_rng(n)
+    P = 2 * n * layers
+    B = 2
+    thetas = rng.uniform(0, 2 * math.pi, size=(B, P))
+    obs = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
+    ops, _ = lower(workloads.hea(n, layers, thetas[0]))
+    mats = np.stack([lower(workloads.hea(n, layers, thetas[b]))[1] for b in range(B)])
+    st = capi.State(hip_ctx, n, B)
+    grads, energies = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=ckpt, per_element=True)
+    st.close()
+    for b in range(B):
+        e0 = hea_energy(n, layers, thetas[b], obs)
+        assert abs(energies[b] - e0) <= EPS
+        for p in range(0, P, max(1, P // 12)):
+            tp, tm = thetas[b].copy(), thetas[b].copy()
+            tp[p] += math.pi / 2
+            tm[p] -= math.pi / 2
+            ps = 0.5 * (hea_energy(n, layers, tp, obs) - hea_energy(n, layers, tm, obs))
+            assert abs(grads[b, p] - ps) <= 1e-10, (b, p)
+            h = 1e-6
+            tp, tm = thetas[b].copy(), thetas[b].copy()
+            tp[p] += h
+            tm[p] -= h
+            fd = (hea_energy(n, layers, tp, obs) - hea_energy(n, layers, tm, obs)) / (2 * h)
+            assert abs(grads[b, p] - fd) <= 1e-7, (b, p)
+
+
+def test_adjoint_checkpoint_reanchor(hip_ctx):
+    n, layers = 12, 2
+    rng = np.random.default_rng(3)
+    P = 2 * n * layers
+    thetas = rng.uniform(0, 2 * math.pi, size=P)
+    obs = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
+    ops, mats = lower(workloads.hea(n, layers, thetas))
+    st = capi.State(hip_ctx, n, 1)
+    g0, _ = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=16)
+    assert hip_ctx.adjoint_stats()["reanchors"] == 0
+    hip_ctx.set_option(capi.OPT_TEST_DRIFT, 1000)  # inject 1e-9 drift before every checkpoint check
+    try:
+        st.set_basis(0)
+        g1, _ = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=16)
+    finally:
+        hip_ctx.set_option(capi.OPT_TEST_DRIFT, 0)
+    stats = hip_ctx.adjoint_stats()
+    assert stats["reanchors"] > 0 and stats["max_drift"] > 1e-10
+    assert np.max(np.abs(g1 - g0)) <= 1e-10
+    st.close()
+
+
+def test_adjoint_qaoa_vs_finite_differences(hip_ctx):
+    n, p = 12, 2
+    rng = np.random.default_rng(44)
+    gam, bet = rng.uniform(0, math.pi, p), rng.uniform(0, math.pi, p)
+    w = rng.uniform(0.5, 1.5, n)
+    const, xm, zm, cf = workloads.maxcut_observable(n, w)
+    circ = workloads.qaoa_ring(n, p, gam, bet, w)
+    ops, mats = lower(circ)
+    st = capi.State(hip_ctx, n, 1)
+    grads, energies = st.adjoint_gradient(ops, mats, 2 * p, xm, zm, cf)
+    st.close()
+
+    def energy(g, b):
+        o, m = lower(workloads.qaoa_ring(n, p, g, b, w))
+        return oracle.expect_pauli(oracle.run_ops(n, o, m), xm, zm, cf)
+
+    assert abs(energies[0] - energy(gam, bet)) <= EPS
+    h = 1e-6
+    for l in range(p):
+        for which in (0, 1):
+            gp, gm, bp, bm = gam.copy(), gam.copy(), bet.copy(), bet.copy()
+            if which == 0:
+                gp[l] += h
+                gm[l] -= h
+            else:
+                bp[l] += h
+                bm[l] -= h
+            fd = (energy(gp, bp) - energy(gm, bm)) / (2 * h)
+            assert abs(grads[0, 2 * l + which] - fd) <= 1e-7
