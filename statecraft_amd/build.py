@@ -65,9 +65,32 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_engine()
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources", flush=True)
+        print(f"built {LIB} from {len(sources)} sources, {ENGINE_LIB.name} and {ENGINE_TEST.name}", flush=True)
     return LIB
+
+
+ENGINE_SRC = PKG / "engine" / "statecraft.cpp"
+ENGINE_LIB = LIBDIR / "libstatecraft_engine.so"
+ENGINE_TEST_SRC = PKG.parent / "tests" / "cpp" / "engine_test.cpp"
+ENGINE_TEST = LIBDIR / "engine_test"
+
+
+def build_engine() -> None:
+    """The C++ engine mirror (host code only) over the C-ABI, and its test program."""
+    cxx = os.environ.get("CXX", "g++")
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{PKG.parent}"]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    cmd = [cxx, *flags, "-shared", "-o", str(ENGINE_LIB), str(ENGINE_SRC), f"-L{LIBDIR}", "-lstatecraft_hip", rpath]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"engine build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    cmd = [cxx, *flags, "-o", str(ENGINE_TEST), str(ENGINE_TEST_SRC), f"-L{LIBDIR}", "-lstatecraft_engine",
+           "-lstatecraft_hip", rpath]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"engine test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
 
 
 def main(argv=None) -> int:
