@@ -96,6 +96,7 @@ typedef struct sc_op {
 #define SC_OPT_PROFILE 3        /* 1: time every sweep/gate launch with HIP events */
 #define SC_OPT_LOW_QUBITS 4     /* qubits always resident in a sweep chunk for coalescing (0..8), default 4 */
 #define SC_OPT_TEST_DRIFT 5     /* test hook: adds value*1e-12 to psi[0] at every adjoint checkpoint, default 0 */
+#define SC_OPT_REMAP 6          /* 1: sweep passes relabel qubits in place to keep coalesced low positions useful, default 1 */
 
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
@@ -208,7 +209,7 @@ int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches,
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
                         uint64_t *fused_blocks, double *state_inout);
 
 #ifdef __cplusplus

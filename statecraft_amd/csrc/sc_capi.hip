@@ -190,7 +190,7 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 // mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
 int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
     SweepPlan plan;
-    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, plan);
+    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan);
     if (rc) return rc;
     if (plan.passes.empty()) return SC_OK;
     const size_t pass_bytes = plan.passes.size() * sizeof(SweepPass);
@@ -347,6 +347,9 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value < 0) return fail(SC_E_ARG, "test drift must be non-negative");
         ctx->test_drift = value;
         return SC_OK;
+    case SC_OPT_REMAP:
+        ctx->remap = value != 0;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -360,6 +363,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_PROFILE: *value = ctx->prof.enabled ? 1 : 0; return SC_OK;
     case SC_OPT_LOW_QUBITS: *value = ctx->low_qubits; return SC_OK;
     case SC_OPT_TEST_DRIFT: *value = ctx->test_drift; return SC_OK;
+    case SC_OPT_REMAP: *value = ctx->remap; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -660,7 +664,7 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 }
 
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
                         uint64_t *fused_blocks, double *state_inout) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
@@ -671,7 +675,7 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
     if (rc) return rc;
     SweepPlan plan;
-    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, plan))) return rc;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan))) return rc;
     uint64_t ns = 0;
     for (const auto &p : plan.passes) ns += (uint64_t)p.nstages;
     if (passes) *passes = plan.passes.size();

@@ -58,6 +58,7 @@ struct Context {
     int exec_mode = SC_EXEC_SWEEP;
     int max_fused = 4;
     int low_qubits = 4;
+    int remap = 1;
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)
@@ -146,8 +147,12 @@ struct SweepPass {
     int32_t pad;
     uint64_t glob_thread[kThreadBits];  // load/store phase: global offset of thread-index bit i
     uint64_t glob_reg[kRegs];       // load/store phase: global offset of register index r
-    uint16_t io_thread_slot[kThreadBits];  // load/store phase: swizzled slot of thread-index bit i
+    uint16_t io_thread_slot[kThreadBits];  // load phase: swizzled slot of thread-index bit i
     uint16_t io_reg_slot[kRegs];
+    // Store phase: thread bit i / register bit j still address chunk positions i / 8+j in HBM, but read the
+    // LDS slot of the local qubit the pass moves there.  Equal to io_* when the pass does not remap qubits.
+    uint16_t st_thread_slot[kThreadBits];
+    uint16_t st_reg_slot[kRegs];
     SweepStage stages[kMaxStages];
 };
 

@@ -427,7 +427,7 @@ bool register_internal(const SweepOp &op) {
 
 }  // namespace
 
-int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan &plan) {
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan) {
     plan = SweepPlan();
     if (n < kChunkBits) return fail(SC_E_ARG, "sweep plans need at least 12 qubits");
     const uint64_t full = (n == 64) ? ~0ULL : ((1ULL << n) - 1ULL);
@@ -452,10 +452,24 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
 
     std::vector<int> remaining(ops.size());
     for (size_t i = 0; i < ops.size(); i++) remaining[i] = (int)i;
+    // Qubit placement: logical qubit q lives at physical index bit phys[q]; logi is the inverse.  Ops are in
+    // logical qubits, kernels address physical bits.  A pass may permute the qubits of its chunk (the store
+    // phase writes each local qubit to another of the chunk's positions, in place) so that the qubits the
+    // next pass needs occupy the low, coalesced positions.  The placement is the identity again when the
+    // plan ends (fix-up passes without ops restore it if needed).
+    int phys[64], logi[64];
+    for (int q = 0; q < 64; q++) phys[q] = logi[q] = q;
+    auto identity = [&]() {
+        for (int q = 0; q < n; q++)
+            if (phys[q] != q) return false;
+        return true;
+    };
     // SweepOps hold recipe indices in .mat until offsets are known
-    while (!remaining.empty()) {
-        // ---- choose the pass's op set and local qubits
-        uint64_t L = lowmask;
+    while (!remaining.empty() || !identity()) {
+        const bool fixup = remaining.empty();
+        // ---- choose the pass's op set and local (logical) qubits; the low physical positions are forced
+        uint64_t L = 0;
+        for (int p = 0; p < lb; p++) L |= 1ULL << logi[p];
         Blocker blk;
         std::vector<int> sched, next;
         for (int i : remaining) {
@@ -467,6 +481,15 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
             }
             L |= op.targets;
             sched.push_back(i);
+        }
+        if (fixup) {
+            // bring logical 0..lb-1 home, then misplaced qubits together with the occupant of their home
+            for (int q = 0; q < lb; q++) L |= 1ULL << q;
+            for (int q = 0; q < n; q++) {
+                if (phys[q] == q) continue;
+                uint64_t add = (1ULL << q) | (1ULL << logi[q]);
+                if (popc(L | add) <= kChunkBits) L |= add;
+            }
         }
         // ---- split into stages
         std::vector<StageBuild> stages;
@@ -493,24 +516,90 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
             std::merge(rest.begin(), rest.end(), next.begin(), next.end(), std::back_inserter(merged));
             next.swap(merged);
         }
-        // ---- local qubits: fill to 12 with the lowest unused qubits
-        for (int q = 0; q < n && popc(L) < kChunkBits; q++) L |= 1ULL << q;
-        int lq[kChunkBits];
-        int lp[64];
+        // ---- local qubits: fill to 12 with the qubits at the lowest unused physical positions
+        for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
+        int lq[kChunkBits];  // local position j -> physical position (ascending)
+        int lp[64];          // logical qubit -> local position, -1 when outer
+        uint64_t Lphys = 0;
         for (int q = 0; q < 64; q++) lp[q] = -1;
         {
             int j = 0;
-            for (int q = 0; q < n; q++)
-                if ((L >> q) & 1ULL) {
-                    lq[j] = q;
-                    lp[q] = j;
+            for (int p = 0; p < n; p++)
+                if ((L >> logi[p]) & 1ULL) {
+                    lq[j] = p;
+                    lp[logi[p]] = j;
+                    Lphys |= 1ULL << p;
                     j++;
                 }
         }
         SweepPass pass;
         std::memset(&pass, 0, sizeof(pass));
-        pass.outer_mask = full & ~L;
+        pass.outer_mask = full & ~Lphys;
         pass.nstages = (int)stages.size();
+
+        // ---- placement after the pass: out_q[k] is the logical qubit written to chunk position k.
+        int out_q[kChunkBits];
+        {
+            bool used[kChunkBits] = {false};
+            uint64_t placed = 0;
+            std::vector<int> front;  // qubits wanted at the low positions, in priority order
+            if (!remap) {
+                for (int k = 0; k < kChunkBits; k++) {
+                    out_q[k] = logi[lq[k]];
+                    used[k] = true;
+                    placed |= 1ULL << out_q[k];
+                }
+            } else if (fixup || next.empty()) {
+                // restore: every local qubit whose home is one of the chunk's positions goes home
+                for (int k = 0; k < kChunkBits; k++)
+                    if ((L >> lq[k]) & 1ULL) {
+                        out_q[k] = lq[k];
+                        used[k] = true;
+                        placed |= 1ULL << lq[k];
+                    }
+            } else {
+                // lookahead: the targets of the next pass, in order of first use
+                Blocker b2;
+                uint64_t T = 0;
+                for (int i : next) {
+                    const PlanOp &op = ops[i];
+                    if (b2.blocked(op) || popc(T | op.targets) > kChunkBits) {
+                        b2.mark(op);
+                        continue;
+                    }
+                    for (int q : {op.q0, op.q1})
+                        if (q >= 0 && ((op.targets >> q) & 1ULL) && !((T >> q) & 1ULL) && ((L >> q) & 1ULL))
+                            front.push_back(q);
+                    T |= op.targets;
+                }
+                for (int k = 0; k < lb && k < (int)front.size(); k++) {
+                    out_q[k] = front[k];
+                    used[k] = true;
+                    placed |= 1ULL << front[k];
+                }
+                // everything else stays where it is when its position is free
+                for (int k = 0; k < kChunkBits; k++) {
+                    int q = logi[lq[k]];
+                    if (!used[k] && !((placed >> q) & 1ULL)) {
+                        out_q[k] = q;
+                        used[k] = true;
+                        placed |= 1ULL << q;
+                    }
+                }
+            }
+            // leftovers fill the remaining positions in order
+            int k = 0;
+            for (int j = 0; j < kChunkBits; j++) {
+                int q = logi[lq[j]];
+                if ((placed >> q) & 1ULL) continue;
+                while (used[k]) k++;
+                out_q[k] = q;
+                used[k] = true;
+                placed |= 1ULL << q;
+            }
+        }
+        int pi[kChunkBits];  // chunk position k reads local position pi[k]
+        for (int k = 0; k < kChunkBits; k++) pi[k] = lp[out_q[k]];
 
         // per stage: register positions / thread positions
         struct StageLayout {
@@ -547,10 +636,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
                 uint32_t cand[kChunkBits] = {0};
                 for (int p = kSwzBits; p < kChunkBits; p++) cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & 15u);
                 int bad = 0;
-                for (size_t s = 0; s < stages.size(); s++) {
+                for (size_t s = 0; s <= stages.size(); s++) {  // every stage, then the store phase
                     std::vector<uint32_t> c16, c8;
                     for (int i = 0; i < 4; i++) {
-                        int p = layouts[s].tpos[i];
+                        int p = s < stages.size() ? layouts[s].tpos[i] : pi[i];
                         uint32_t col = p < kSwzBits ? (1u << p) : cand[p];
                         c16.push_back(col);
                         if (i < 3) c8.push_back(col & 7u);
@@ -587,6 +676,14 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
             pass.glob_reg[r] = g;
             pass.io_reg_slot[r] = (uint16_t)slot_of(l);
         }
+        // store phase: chunk position k takes the value of local position pi[k]
+        for (int i = 0; i < kThreadBits; i++) pass.st_thread_slot[i] = (uint16_t)slot_of(1u << pi[i]);
+        for (int r = 0; r < kRegs; r++) {
+            uint32_t l = 0;
+            for (int j = 0; j < kRegBits; j++)
+                if ((r >> j) & 1) l |= 1u << pi[kThreadBits + j];
+            pass.st_reg_slot[r] = (uint16_t)slot_of(l);
+        }
         // ---- stage ops
         for (size_t s = 0; s < stages.size(); s++) {
             SweepStage &stg = pass.stages[s];
@@ -616,7 +713,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
                             if (lay.tpos[i] == lp[q]) b.tmask = 1u << i;
                     }
                 } else {
-                    uint64_t om = 1ULL << q;
+                    uint64_t om = 1ULL << phys[q];
                     b.omask_lo = (uint32_t)om;
                     b.omask_hi = (uint32_t)(om >> 32);
                 }
@@ -721,6 +818,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan
             stg.op_end = (int32_t)plan.ops.size();
         }
         plan.passes.push_back(pass);
+        for (int k = 0; k < kChunkBits; k++) {
+            phys[out_q[k]] = lq[k];
+            logi[lq[k]] = out_q[k];
+        }
         remaining.swap(next);
     }
     // ---- pool offsets: replace recipe indices by complex offsets
@@ -843,10 +944,10 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 uint64_t g = 0;
                 uint32_t s = 0;
                 for (int i = 0; i < kThreadBits; i++)
-                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
+                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.st_thread_slot[i];
                 for (int r = 0; r < kRegs; r++) {
                     uint64_t gi = chunk_base + g + pass.glob_reg[r];
-                    uint32_t sl = s ^ pass.io_reg_slot[r];
+                    uint32_t sl = s ^ pass.st_reg_slot[r];
                     state[2 * gi] = lds[2 * sl];
                     state[2 * gi + 1] = lds[2 * sl + 1];
                 }

@@ -60,7 +60,7 @@ struct SweepPlan {
     int fused_blocks = 0;
 };
 
-int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, SweepPlan &plan);
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan);
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS

@@ -302,9 +302,13 @@ __global__ void __launch_bounds__(kThreads) k_sweep(double2 *__restrict__ amp, i
         __syncthreads();
     }
 
-    // ---- store phase
+    // ---- store phase (possibly remapping which local qubit lands on which chunk position)
+    uint32_t sout = 0;
 #pragma unroll
-    for (int r = 0; r < kRegs; r++) a[r] = lds[sthread ^ pass->io_reg_slot[r]];
+    for (int i = 0; i < kThreadBits; i++)
+        if ((tid >> i) & 1u) sout ^= pass->st_thread_slot[i];
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) a[r] = lds[sout ^ pass->st_reg_slot[r]];
 #pragma unroll
     for (int r = 0; r < kRegs; r++) base[gthread + pass->glob_reg[r]] = a[r];
 }
