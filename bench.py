@@ -66,6 +66,8 @@ def bench_c2(args):
     ops, mats = lower(circ)
     credit = workloads.amp_update_credit(circ)
     ctx = capi.Context(0, MODES[args.mode])
+    ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
+    ctx.set_option(capi.OPT_REMAP, 0 if args.no_remap else 1)
     free, total, cus = ctx.device_info()
     st = capi.State(ctx, n)
     measured = list(range(n))
@@ -104,6 +106,17 @@ def bench_c2(args):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     sweeps = plan["sweeps"]
     circuit_ms = ms / args.steps
+    # measured device-copy bandwidth (plain copy kernel, same byte count per launch as one sweep)
+    copy_gbs = None
+    if not args.no_copy_reference:
+        dst = capi.State(ctx, n)
+        ctx.set_option(capi.OPT_PROFILE, 1)
+        for _ in range(3):
+            dst.copy_from(st)
+        ctx.set_option(capi.OPT_PROFILE, 0)
+        c_n, c_ms, c_bytes = ctx.profile_read(capi.KFAM_COPY)
+        copy_gbs = (c_bytes / c_n) / (c_ms / c_n * 1e-3) / 1e9 if c_n else None
+        dst.close()
     result = {
         "metric": METRIC,
         "value": value,
@@ -134,6 +147,11 @@ def bench_c2(args):
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "launches": launches,
+            "copy_kernel_gbs": copy_gbs,
+            "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None,
+            "fp64_ops_per_amp_per_step": capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None,
+                                                               not args.no_remap)[0]["fp64_ops_per_amp"]
+            if args.mode == "sweep" else None,
         },
         "amp_updates_per_step": credit,
         "gate_ms_per_step": circuit_ms,
@@ -198,6 +216,9 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", default="sweep", choices=list(MODES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-copy-reference", action="store_true")
+    ap.add_argument("--low-qubits", type=int, default=4)
+    ap.add_argument("--no-remap", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args(argv)
     if args.gpus != 1:

@@ -199,7 +199,8 @@ int sc_adjoint_stats(sc_ctx *ctx, uint64_t *reanchors, double *max_drift);
 #define SC_KFAM_MEASURE 3  /* probability / sampling kernels */
 #define SC_KFAM_NOISE 4    /* Kraus weight / renormalise kernels */
 #define SC_KFAM_ADJOINT 5  /* adjoint backward kernels */
-#define SC_KFAM_COUNT 6
+#define SC_KFAM_COPY 6     /* sc_state_copy's plain copy kernel (bandwidth reference) */
+#define SC_KFAM_COUNT 7
 int sc_profile_reset(sc_ctx *ctx);
 int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_ms, double *bytes);
 /* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
@@ -210,7 +211,7 @@ int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches,
  * movement, so plans can be validated without a GPU. */
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
                         uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
-                        uint64_t *fused_blocks, double *state_inout);
+                        uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ OP_SINGLE, OP_CNOT, OP_TWO, OP_DIAG2, OP_SWAP, OP_MCU, OP_MEASURE = range(1, 8)
 GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
 OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP = range(1, 7)
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
-KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT = range(6)
+KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 
 OP_DTYPE = np.dtype(
     [
@@ -124,7 +124,7 @@ def _sig(lib):
         "sc_profile_reset": (i32, [P]),
         "sc_profile_read": (i32, [P, i32, U64P, DP, DP]),
         "sc_last_plan_stats": (i32, [P, U64P, U64P, U64P]),
-        "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, i32, U64P, U64P, U64P, DP]),
+        "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, i32, U64P, U64P, U64P, DP, DP]),
     }
     for name, (res, args) in s.items():
         f = getattr(lib, name)
@@ -171,7 +171,7 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
     """Host-only sweep plan statistics; with `state` (complex 2^n) also executes the plan on the host."""
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
-    a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    a, b, c, f = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
     out = None
     ptr = None
     if state is not None:
@@ -179,8 +179,8 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
         ptr = dptr(out.view(np.float64))
     check(lib().sc_plan_sweeps_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(low_qubits),
                                     1 if remap else 0,
-                                    ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ptr))
-    return {"passes": a.value, "stages": b.value, "fused_blocks": c.value}, out
+                                    ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(f), ptr))
+    return {"passes": a.value, "stages": b.value, "fused_blocks": c.value, "fp64_ops_per_amp": f.value}, out
 
 
 class Context:

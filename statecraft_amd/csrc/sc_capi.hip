@@ -481,8 +481,7 @@ int sc_state_copy(sc_state *dst, const sc_state *src) {
     if (dst->n != src->n || dst->batch != src->batch) return fail(SC_E_ARG, "state shapes differ");
     int rc = set_device(dst->ctx);
     if (rc) return rc;
-    SC_HIP_CHECK(hipMemcpyAsync(dst->amp, src->amp, ((size_t)src->batch << src->n) * sizeof(double2),
-                                hipMemcpyDeviceToDevice, dst->ctx->stream));
+    if ((rc = launch_copy(dst->ctx, src->amp, dst->amp, (uint64_t)src->batch << src->n))) return rc;
     SC_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
     return SC_OK;
 }
@@ -665,7 +664,7 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
                         uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
-                        uint64_t *fused_blocks, double *state_inout) {
+                        uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
     if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
@@ -681,6 +680,7 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     if (passes) *passes = plan.passes.size();
     if (stages) *stages = ns;
     if (fused_blocks) *fused_blocks = (uint64_t)plan.fused_blocks;
+    if (fp64_ops_per_amp) *fp64_ops_per_amp = plan.fp64_ops;
     if (state_inout) {
         std::vector<double> pool(2 * std::max<uint64_t>(plan.pool_complex, 1));
         build_pool(plan, mats, pool.data());

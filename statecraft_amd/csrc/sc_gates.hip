@@ -174,6 +174,13 @@ __global__ void __launch_bounds__(kBlock) k_dense(double2 *__restrict__ amp, int
     }
 }
 
+// Plain device copy, 16 B per lane per iteration: the bandwidth reference for the sweep kernel.
+__global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                 uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
 unsigned grid_for(Context *ctx, uint64_t total) {
     uint64_t blocks = (total + kBlock - 1) / kBlock;
     uint64_t cap = (uint64_t)ctx->cu_count * 16;
@@ -231,6 +238,13 @@ int launch_swap(Context *ctx, State *st, int first, int second) {
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_GATE, 32.0 * (double)total * 2.0);
     hipLaunchKernelGGL(k_swap, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, st->n, first,
                        second, log_groups, total);
+    SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
+}
+
+int launch_copy(Context *ctx, const double2 *src, double2 *dst, uint64_t count) {
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_COPY, 32.0 * (double)count);
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx->stream, src, dst, count);
     SC_HIP_CHECK(hipGetLastError());
     return SC_OK;
 }

@@ -111,6 +111,9 @@ enum SweepOpKind : int32_t {
     SW_DIAG = 4,   // 2-entry or 4-entry diagonal on up to two qubits anywhere
     SW_SWAP = 5,   // exact exchange of register bits k0, k1
     SW_DENSE4 = 6, // dense 16x16 on all four register bits (fused block)
+    SW_U1R = 7,    // SW_U1 with a real matrix (imaginary parts zero): half the FP64 work
+    SW_U2R = 8,    // SW_U2 with a real matrix
+    SW_DENSE4R = 9,// SW_DENSE4 with a real matrix
 };
 
 // Where a qubit's bit of an amplitude index comes from inside a stage.
@@ -172,6 +175,7 @@ int launch_two(Context *ctx, State *st, int first, int second, const Mat4 &m);
 int launch_diag2(Context *ctx, State *st, int first, int second, const double *d);
 int launch_swap(Context *ctx, State *st, int first, int second);
 int launch_dense(Context *ctx, State *st, int k, const int *qubits, const double *mat_dev);
+int launch_copy(Context *ctx, const double2 *src, double2 *dst, uint64_t count);
 int launch_sweep(Context *ctx, State *st, const SweepPass *pass_dev, const SweepOp *ops_dev,
                  const double *mats_dev, int elem_begin, int elem_count, size_t mat_stride);
 

@@ -118,9 +118,12 @@ int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const
             break;
         }
         int cnt = mat_count(o.kind);
+        p.is_real = true;
         if (cnt > 0) {
             if (o.mat < 0 || (uint64_t)o.mat + (uint64_t)cnt > n_mat_complex)
                 return fail(SC_E_ARG, "matrix offset out of range for op " + std::to_string(i));
+            for (const double *m : mats_all)
+                for (int e = 0; e < cnt; e++) p.is_real = p.is_real && m[2 * ((int64_t)o.mat + e) + 1] == 0.0;
         }
         // roles
         switch (o.kind) {
@@ -341,6 +344,7 @@ void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */)
     };
     switch (op.kind) {
     case SW_U1:
+    case SW_U1R:
     case SW_X: {
         for (int r0 = 0; r0 < kRegs; r0++) {
             if (r0 & (1 << op.k0)) continue;
@@ -361,7 +365,8 @@ void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */)
         }
         break;
     }
-    case SW_U2: {
+    case SW_U2:
+    case SW_U2R: {
         for (int b = 0; b < kRegs; b++) {
             if (b & ((1 << op.k0) | (1 << op.k1))) continue;
             int idx[4] = {b, b | (1 << op.k1), b | (1 << op.k0), b | (1 << op.k0) | (1 << op.k1)};
@@ -404,14 +409,37 @@ void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */)
     }
 }
 
+// FP64 VALU operations per amplitude of a register op (complex multiply-add = 4).
 double op_cost(const SweepOp &op) {
     switch (op.kind) {
     case SW_U1: return 8.0;
+    case SW_U1R: return 4.0;
     case SW_U2: return 16.0;
-    case SW_DIAG: return 3.0;
+    case SW_U2R: return 8.0;
+    case SW_DIAG: return 4.0;
     case SW_X: return 1.0;
     case SW_SWAP: return 0.5;
+    case SW_DENSE4R: return 32.0;
     default: return 64.0;
+    }
+}
+
+double fused_cost(int bits, bool real) {
+    double c = bits <= 1 ? 8.0 : bits == 2 ? 16.0 : 64.0;
+    return real ? 0.5 * c : c;
+}
+
+// Register bits a register-internal op reads or writes.
+uint32_t reg_support(const SweepOp &op) {
+    switch (op.kind) {
+    case SW_U1:
+    case SW_U1R:
+    case SW_X: return (1u << op.k0) | op.rctrl;
+    case SW_U2:
+    case SW_U2R:
+    case SW_SWAP: return (1u << op.k0) | (1u << op.k1);
+    case SW_DIAG: return op.da.rmask | (op.nd == 2 ? op.db.rmask : 0u) | op.rctrl;
+    default: return (1u << kRegBits) - 1u;
     }
 }
 
@@ -422,7 +450,8 @@ bool register_internal(const SweepOp &op) {
         bool b_ok = op.nd == 1 || (op.db.tmask == 0u && op.db.omask_lo == 0u && op.db.omask_hi == 0u);
         return a_ok && b_ok;
     }
-    return op.kind == SW_U1 || op.kind == SW_X || op.kind == SW_U2 || op.kind == SW_SWAP;
+    return op.kind == SW_U1 || op.kind == SW_U1R || op.kind == SW_X || op.kind == SW_U2 || op.kind == SW_U2R ||
+           op.kind == SW_SWAP;
 }
 
 }  // namespace
@@ -729,6 +758,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 }
             };
             std::vector<SweepOp> sops;
+            std::vector<bool> sreal;  // op's matrix is real
             for (int oi : stages[s].ops) {
                 const PlanOp &op = ops[oi];
                 SweepOp so;
@@ -748,7 +778,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                         so.k0 = reg_of(op.q0);
                         add_controls(so, op.controls);
                     } else {
-                        so.kind = SW_U1;
+                        so.kind = (op.is_real && op.controls == 0) ? SW_U1R : SW_U1;
                         so.k0 = reg_of(op.q0);
                         so.mat = recipe_for(0, op.mat, 4);
                         add_controls(so, op.controls);
@@ -760,7 +790,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     add_controls(so, 1ULL << op.q0);
                     break;
                 case SC_OP_TWO:
-                    so.kind = SW_U2;
+                    so.kind = op.is_real ? SW_U2R : SW_U2;
                     so.k0 = reg_of(op.q0);
                     so.k1 = reg_of(op.q1);
                     so.mat = recipe_for(0, op.mat, 16);
@@ -779,39 +809,86 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     break;
                 }
                 sops.push_back(so);
+                sreal.push_back(op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP || op.is_x);
             }
-            // ---- dense fusion of register-internal runs when cheaper than op-by-op
-            std::vector<SweepOp> fused;
-            size_t i = 0;
-            while (i < sops.size()) {
-                if (!register_internal(sops[i])) {
-                    fused.push_back(sops[i]);
-                    i++;
-                    continue;
+            // ---- fusion: split the stage's op list into consecutive segments, each either left as is or
+            // (when every op in it stays inside the register bits) replaced by one dense block on the
+            // segment's register bits -- 2x2 (8 FP64 ops per amplitude), 4x4 (16) or 16x16 (64) -- choosing
+            // the segmentation with the fewest FP64 operations (dynamic programming over segment ends).
+            const size_t L = sops.size();
+            std::vector<double> best(L + 1, 0.0);
+            std::vector<int> from(L + 1, -1);  // segment start, or -1 for "op left as is"
+            for (size_t e = 1; e <= L; e++) {
+                best[e] = best[e - 1] + op_cost(sops[e - 1]);
+                from[e] = -1;
+                uint32_t sup = 0;
+                bool real = true;
+                for (size_t s0 = e; s0-- > 0;) {
+                    if (!register_internal(sops[s0])) break;
+                    sup |= reg_support(sops[s0]);
+                    real = real && sreal[s0];
+                    if (e - s0 < 2) continue;
+                    double c = best[s0] + fused_cost(popc(sup), real);
+                    if (c < best[e] - 1e-9) {
+                        best[e] = c;
+                        from[e] = (int)s0;
+                    }
                 }
-                size_t j = i;
-                double cost = 0.0;
-                while (j < sops.size() && register_internal(sops[j])) cost += op_cost(sops[j++]);
-                if (cost > 72.0 && j - i >= 2) {
+            }
+            std::vector<std::pair<size_t, size_t>> segs;  // [begin, end) of fused segments
+            std::vector<bool> seg_start(L + 1, false);
+            for (size_t e = L; e > 0;) {
+                if (from[e] < 0) {
+                    e--;
+                } else {
+                    segs.push_back({(size_t)from[e], e});
+                    e = (size_t)from[e];
+                }
+            }
+            std::sort(segs.begin(), segs.end());
+            std::vector<SweepOp> fused;
+            size_t si = 0;
+            for (size_t i = 0; i < L;) {
+                if (si < segs.size() && segs[si].first == i) {
+                    size_t j = segs[si].second;
+                    uint32_t sup = 0;
+                    bool real = true;
+                    for (size_t t = i; t < j; t++) sup |= reg_support(sops[t]), real = real && sreal[t];
                     PoolRecipe r;
                     r.kind = 2;
                     r.mat = -1;
-                    r.count = kRegs * kRegs;
+                    SweepOp d;
+                    std::memset(&d, 0, sizeof(d));
+                    std::vector<int> bits;
+                    for (int b = kRegBits - 1; b >= 0; b--)
+                        if ((sup >> b) & 1u) bits.push_back(b);
+                    if (bits.size() == 1) {
+                        d.kind = real ? SW_U1R : SW_U1;
+                        d.k0 = bits[0];
+                    } else if (bits.size() == 2) {
+                        d.kind = real ? SW_U2R : SW_U2;
+                        d.k0 = bits[0];
+                        d.k1 = bits[1];
+                    } else {
+                        d.kind = real ? SW_DENSE4R : SW_DENSE4;
+                        bits = {3, 2, 1, 0};
+                        plan.fused_blocks++;
+                    }
+                    r.fused_bits = bits;
+                    r.count = 1 << (2 * bits.size());
                     for (size_t t = i; t < j; t++) {
                         r.fused_ops.push_back(sops[t]);
                         r.fused_srcs.push_back(sops[t].mat);
                     }
                     plan.pool.push_back(r);
-                    SweepOp d;
-                    std::memset(&d, 0, sizeof(d));
-                    d.kind = SW_DENSE4;
                     d.mat = (int)plan.pool.size() - 1;
                     fused.push_back(d);
-                    plan.fused_blocks++;
+                    i = j;
+                    si++;
                 } else {
-                    for (size_t t = i; t < j; t++) fused.push_back(sops[t]);
+                    fused.push_back(sops[i]);
+                    i++;
                 }
-                i = j;
             }
             stg.op_begin = (int32_t)plan.ops.size();
             for (auto &so : fused) plan.ops.push_back(so);
@@ -832,6 +909,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         off += (uint64_t)plan.pool[r].count;
     }
     plan.pool_complex = off;
+    for (auto &so : plan.ops) plan.fp64_ops += op_cost(so);
     for (auto &so : plan.ops)
         if (so.mat >= 0) so.mat = (int32_t)recipe_off[so.mat];
     // fused recipes keep recipe indices in fused_srcs; record their offsets in the SweepOps' .mat for build_pool
@@ -858,6 +936,7 @@ void host_run_op(const SweepOp &op, const double *pool, double *a, uint32_t tid,
     };
     switch (op.kind) {
     case SW_U1:
+    case SW_U1R:
     case SW_X: {
         if (!tcond) return;
         SweepOp r = op;
@@ -878,7 +957,8 @@ void host_run_op(const SweepOp &op, const double *pool, double *a, uint32_t tid,
         }
         return;
     }
-    case SW_DENSE4: {
+    case SW_DENSE4:
+    case SW_DENSE4R: {
         double y[2 * kRegs];
         for (int r = 0; r < kRegs; r++) {
             double re = 0, im = 0;
@@ -974,22 +1054,30 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
             dst[0] = g[0], dst[1] = g[1], dst[2] = g[6], dst[3] = g[7];
         }
     }
-    // second pass: fused dense4 blocks from the plain entries
+    // second pass: fused blocks from the plain entries.  Block index l has bit (m-1-j) <-> register bit
+    // fused_bits[j]; the block's ops touch no other register bit, so columns with those bits 0 suffice.
     for (size_t r = 0; r < plan.pool.size(); r++) {
         const PoolRecipe &rc = plan.pool[r];
         if (rc.kind != 2) continue;
         double *dst = out + 2 * offs[r];
-        for (int c = 0; c < kRegs; c++) {
+        const int m = (int)rc.fused_bits.size(), d = 1 << m;
+        auto reg_of = [&](int l) {
+            int reg = 0;
+            for (int j = 0; j < m; j++)
+                if ((l >> (m - 1 - j)) & 1) reg |= 1 << rc.fused_bits[j];
+            return reg;
+        };
+        for (int c = 0; c < d; c++) {
             double col[2 * kRegs] = {0};
-            col[2 * c] = 1.0;
+            col[2 * reg_of(c)] = 1.0;
             for (size_t t = 0; t < rc.fused_ops.size(); t++) {
                 const SweepOp &so = rc.fused_ops[t];
-                const double *m = so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr;
-                host_reg_op(so, m, col);
+                const double *mm = so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr;
+                host_reg_op(so, mm, col);
             }
-            for (int row = 0; row < kRegs; row++) {
-                dst[2 * (row * kRegs + c)] = col[2 * row];
-                dst[2 * (row * kRegs + c) + 1] = col[2 * row + 1];
+            for (int row = 0; row < d; row++) {
+                dst[2 * (row * d + c)] = col[2 * reg_of(row)];
+                dst[2 * (row * d + c) + 1] = col[2 * reg_of(row) + 1];
             }
         }
     }

@@ -19,6 +19,7 @@ struct PlanOp {
     uint64_t diag = 0;      // qubits acted on diagonally (controls, diagonal gates)
     bool is_x = false;      // gate is exactly Pauli X (pure exchange)
     bool is_diag = false;   // whole op is diagonal
+    bool is_real = false;   // every matrix entry has a zero imaginary part (all batch elements)
     int src = -1;           // index into the caller's op array
 };
 
@@ -44,11 +45,12 @@ void block_matrix(const FusedBlock &b, const std::vector<PlanOp> &ops, const dou
 // ---- LDS-blocked sweeps (SC_EXEC_SWEEP) ------------------------------------
 // A matrix-pool entry: how to build `count` complex values from the caller's matrices.
 struct PoolRecipe {
-    int kind;        // 0 copy `count` entries from mat; 1 diag of a 2x2 (entries 0 and 3); 2 fused dense4 of ops
+    int kind;        // 0 copy `count` entries from mat; 1 diag of a 2x2 (entries 0 and 3); 2 fused block of ops
     int mat;         // source offset (kinds 0, 1)
     int count;
     std::vector<SweepOp> fused_ops;  // kind 2: register ops in order (their .mat index into `fused_mats`)
     std::vector<int> fused_srcs;     // kind 2: per fused op, recipe index of its matrix (-1 none)
+    std::vector<int> fused_bits;     // kind 2: register bits of the block, most significant index bit first
 };
 
 struct SweepPlan {
@@ -58,6 +60,7 @@ struct SweepPlan {
     uint64_t pool_complex = 0;
     std::vector<int> standalone;   // unused for now (every op kind fits a sweep)
     int fused_blocks = 0;
+    double fp64_ops = 0.0;  // estimated FP64 VALU operations per amplitude over all passes
 };
 
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan);

@@ -50,6 +50,58 @@ __device__ __forceinline__ void op_u1(double2 (&a)[kRegs], const double2 *__rest
     }
 }
 
+// Real 2x2 (Ry, H, real products): two real multiply-adds per output component.
+template <int K>
+__device__ __forceinline__ void op_u1r(double2 (&a)[kRegs], const double2 *__restrict__ m) {
+    const double g00 = m[0].x, g01 = m[1].x, g10 = m[2].x, g11 = m[3].x;
+#pragma unroll
+    for (int r0 = 0; r0 < kRegs; r0++) {
+        if (r0 & (1 << K)) continue;
+        const int r1 = r0 | (1 << K);
+        double2 x = a[r0], y = a[r1];
+        a[r0] = make_double2(fma(g01, y.x, g00 * x.x), fma(g01, y.y, g00 * x.y));
+        a[r1] = make_double2(fma(g11, y.x, g10 * x.x), fma(g11, y.y, g10 * x.y));
+    }
+}
+
+template <int K0, int K1>
+__device__ __forceinline__ void op_u2r(double2 (&a)[kRegs], const double2 *__restrict__ m) {
+#pragma unroll
+    for (int b = 0; b < kRegs; b++) {
+        if (b & ((1 << K0) | (1 << K1))) continue;
+        const int idx[4] = {b, b | (1 << K1), b | (1 << K0), b | (1 << K0) | (1 << K1)};
+        double2 x[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = a[idx[c]];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            double re = m[4 * r].x * x[0].x, im = m[4 * r].x * x[0].y;
+#pragma unroll
+            for (int c = 1; c < 4; c++) {
+                re = fma(m[4 * r + c].x, x[c].x, re);
+                im = fma(m[4 * r + c].x, x[c].y, im);
+            }
+            a[idx[r]] = make_double2(re, im);
+        }
+    }
+}
+
+__device__ __forceinline__ void op_dense4r(double2 (&a)[kRegs], const double2 *__restrict__ m) {
+    double2 y[kRegs];
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) {
+        double re = m[r * kRegs].x * a[0].x, im = m[r * kRegs].x * a[0].y;
+#pragma unroll
+        for (int c = 1; c < kRegs; c++) {
+            re = fma(m[r * kRegs + c].x, a[c].x, re);
+            im = fma(m[r * kRegs + c].x, a[c].y, im);
+        }
+        y[r] = make_double2(re, im);
+    }
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) a[r] = y[r];
+}
+
 template <int K>
 __device__ __forceinline__ void op_cu1(double2 (&a)[kRegs], const double2 *__restrict__ m, uint32_t rctrl,
                                        bool tcond) {
@@ -241,6 +293,32 @@ __device__ __forceinline__ void run_op(double2 (&a)[kRegs], const SweepOp &op, c
         break;
     }
     case SW_DENSE4: op_dense4(a, m); break;
+    case SW_DENSE4R: op_dense4r(a, m); break;
+    case SW_U1R:
+        switch (op.k0) {
+        case 0: op_u1r<0>(a, m); break;
+        case 1: op_u1r<1>(a, m); break;
+        case 2: op_u1r<2>(a, m); break;
+        default: op_u1r<3>(a, m); break;
+        }
+        break;
+    case SW_U2R: {
+        switch (op.k0 * 4 + op.k1) {
+        case 1: op_u2r<0, 1>(a, m); break;
+        case 2: op_u2r<0, 2>(a, m); break;
+        case 3: op_u2r<0, 3>(a, m); break;
+        case 4: op_u2r<1, 0>(a, m); break;
+        case 6: op_u2r<1, 2>(a, m); break;
+        case 7: op_u2r<1, 3>(a, m); break;
+        case 8: op_u2r<2, 0>(a, m); break;
+        case 9: op_u2r<2, 1>(a, m); break;
+        case 11: op_u2r<2, 3>(a, m); break;
+        case 12: op_u2r<3, 0>(a, m); break;
+        case 13: op_u2r<3, 1>(a, m); break;
+        default: op_u2r<3, 2>(a, m); break;
+        }
+        break;
+    }
     default: break;
     }
 }
