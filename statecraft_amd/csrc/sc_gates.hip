@@ -175,10 +175,21 @@ __global__ void __launch_bounds__(kBlock) k_dense(double2 *__restrict__ amp, int
 }
 
 // Plain device copy, 16 B per lane per iteration: the bandwidth reference for the sweep kernel.
+// Each block moves 4 x 256 consecutive values (4 independent 16 B loads in flight per lane).
 __global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                  uint64_t count) {
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock)
-        dst[i] = src[i];
+    const uint64_t base = (uint64_t)blockIdx.x * (4 * kBlock) + threadIdx.x;
+    double2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint64_t i = base + (uint64_t)j * kBlock;
+        if (i < count) v[j] = src[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint64_t i = base + (uint64_t)j * kBlock;
+        if (i < count) dst[i] = v[j];
+    }
 }
 
 unsigned grid_for(Context *ctx, uint64_t total) {
@@ -244,7 +255,8 @@ int launch_swap(Context *ctx, State *st, int first, int second) {
 
 int launch_copy(Context *ctx, const double2 *src, double2 *dst, uint64_t count) {
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_COPY, 32.0 * (double)count);
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx->stream, src, dst, count);
+    const uint64_t blocks = (count + 4 * kBlock - 1) / (4 * kBlock);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, src, dst, count);
     SC_HIP_CHECK(hipGetLastError());
     return SC_OK;
 }

@@ -207,29 +207,36 @@ __global__ void __launch_bounds__(kBlock) k_tile_sums(const double *__restrict__
                                                       double *__restrict__ sums) {
     uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     double acc[1] = {0.0};
-    // thread t sums its kScanPer consecutive values, in order
-    uint64_t start = base + (uint64_t)threadIdx.x * kScanPer;
+    // coalesced: lane t sums elements t, t + 256, ... of the tile (fixed order)
 #pragma unroll
     for (int j = 0; j < kScanPer; j++) {
-        uint64_t i = start + j;
+        uint64_t i = base + threadIdx.x + (uint64_t)j * kBlock;
         if (i < count) acc[0] += x[i];
     }
     block_reduce<1>(acc, sums + blockIdx.x);
 }
 
 // In-place inclusive scan of each tile, plus offsets[blockIdx] (exclusive prefix of tile sums) when non-null.
+// The tile is loaded and stored coalesced through LDS (one pad slot per 16 values keeps the per-thread
+// runs of 16 on distinct banks); each thread scans 16 consecutive values, then the thread totals are scanned.
 __global__ void __launch_bounds__(kBlock) k_tile_scan(double *__restrict__ x, uint64_t count,
                                                       const double *__restrict__ offsets) {
     __shared__ double sh[kBlock];
+    __shared__ double tile[kScanTile + kScanTile / kScanPer];
     uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    uint64_t start = base + (uint64_t)threadIdx.x * kScanPer;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        int l = threadIdx.x + j * kBlock;
+        uint64_t i = base + l;
+        tile[l + l / kScanPer] = i < count ? x[i] : 0.0;
+    }
+    __syncthreads();
     double v[kScanPer];
     double run = 0.0;
 #pragma unroll
     for (int j = 0; j < kScanPer; j++) {
-        uint64_t i = start + j;
-        double xi = i < count ? x[i] : 0.0;
-        run += xi;
+        int l = threadIdx.x * kScanPer + j;
+        run += tile[l + l / kScanPer];
         v[j] = run;
     }
     // Hillis-Steele scan of per-thread totals (fixed order).
@@ -245,8 +252,15 @@ __global__ void __launch_bounds__(kBlock) k_tile_scan(double *__restrict__ x, ui
     if (offsets) prefix += (blockIdx.x > 0 ? offsets[blockIdx.x - 1] : 0.0);
 #pragma unroll
     for (int j = 0; j < kScanPer; j++) {
-        uint64_t i = start + j;
-        if (i < count) x[i] = prefix + v[j];
+        int l = threadIdx.x * kScanPer + j;
+        tile[l + l / kScanPer] = prefix + v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        int l = threadIdx.x + j * kBlock;
+        uint64_t i = base + l;
+        if (i < count) x[i] = tile[l + l / kScanPer];
     }
 }
 

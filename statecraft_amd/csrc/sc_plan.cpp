@@ -409,15 +409,17 @@ void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */)
     }
 }
 
-// FP64 VALU operations per amplitude of a register op (complex multiply-add = 4).
+// VALU operations per amplitude of a register op (complex multiply-add = 4).  Ops with a thread-index control
+// or diagonal source pay per-lane selects on top; the rest resolve their conditions per workgroup.
 double op_cost(const SweepOp &op) {
+    const bool lane = (op.tctrl | op.da.tmask | op.db.tmask) != 0u;
     switch (op.kind) {
-    case SW_U1: return 8.0;
+    case SW_U1: return lane ? 12.0 : 8.0;
     case SW_U1R: return 4.0;
     case SW_U2: return 16.0;
     case SW_U2R: return 8.0;
-    case SW_DIAG: return 4.0;
-    case SW_X: return 1.0;
+    case SW_DIAG: return lane ? 12.0 : 6.0;
+    case SW_X: return lane ? 4.0 : 1.0;
     case SW_SWAP: return 0.5;
     case SW_DENSE4R: return 32.0;
     default: return 64.0;

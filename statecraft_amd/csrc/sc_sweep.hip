@@ -225,14 +225,21 @@ __device__ __forceinline__ void dispatch_u1(double2 (&a)[kRegs], const double2 *
     else op_u1<K>(a, m);
 }
 
-__device__ __forceinline__ void run_op(double2 (&a)[kRegs], const SweepOp &op, const double2 *__restrict__ mats,
-                                       uint32_t tid, uint64_t chunk_base) {
+__device__ __forceinline__ void run_op_fixed_impl(double2 (&a)[kRegs], const SweepOp &op,
+                                                  const double2 *__restrict__ m);
+
+// LANE = false: no control or diagonal source is a thread-index bit, so every condition is uniform across
+// the workgroup (register bits are compile-time per element, chunk-fixed bits come from chunk_base) and the
+// per-element selects fold away.  LANE = true only reaches the op kinds that can carry thread-bit sources.
+template <bool LANE>
+__device__ __forceinline__ void run_op_impl(double2 (&a)[kRegs], const SweepOp &op, const double2 *__restrict__ mats,
+                                            uint32_t tid, uint64_t chunk_base) {
     const double2 *m = mats + op.mat;
     switch (op.kind) {
     case SW_U1:
     case SW_X: {
         const bool ctrl = (op.rctrl | op.tctrl) != 0u || op.octrl != 0ULL;
-        const bool tcond = ((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl);
+        const bool tcond = LANE ? (((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl)) : true;
         if (op.kind == SW_U1) {
             switch (op.k0) {
             case 0: dispatch_u1<0>(a, m, op, ctrl, tcond); break;
@@ -250,6 +257,38 @@ __device__ __forceinline__ void run_op(double2 (&a)[kRegs], const SweepOp &op, c
         }
         break;
     }
+    case SW_DIAG: {
+        bool ta = LANE ? bit_from(op.da, tid, chunk_base) : bit_from(op.da, 0u, chunk_base);
+        bool tb = LANE ? bit_from(op.db, tid, chunk_base) : bit_from(op.db, 0u, chunk_base);
+        const bool ctrl = (op.rctrl | op.tctrl) != 0u || op.octrl != 0ULL;
+        if (!ctrl) {
+            op_diag(a, m, op, ta, tb);
+        } else {
+            const bool tcond = LANE ? (((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl)) : true;
+            op_cdiag(a, m, op, ta, tcond);
+        }
+        break;
+    }
+    default:
+        if constexpr (!LANE) run_op_fixed_impl(a, op, m);
+        break;
+    }
+}
+
+__device__ __forceinline__ void run_op(double2 (&a)[kRegs], const SweepOp &op, const double2 *__restrict__ mats,
+                                       uint32_t tid, uint64_t chunk_base) {
+    if ((op.tctrl | op.da.tmask | op.db.tmask) != 0u) {
+        run_op_impl<true>(a, op, mats, tid, chunk_base);
+    } else {
+        if ((chunk_base & op.octrl) != op.octrl) return;  // outer controls unset: identity on this chunk
+        run_op_impl<false>(a, op, mats, tid, chunk_base);
+    }
+}
+
+// Register-only op kinds (no controls, no diagonal sources): identical for every lane.
+__device__ __forceinline__ void run_op_fixed_impl(double2 (&a)[kRegs], const SweepOp &op,
+                                                  const double2 *__restrict__ m) {
+    switch (op.kind) {
     case SW_U2: {
         switch (op.k0 * 4 + op.k1) {
         case 1: op_u2<0, 1>(a, m); break;
@@ -264,18 +303,6 @@ __device__ __forceinline__ void run_op(double2 (&a)[kRegs], const SweepOp &op, c
         case 12: op_u2<3, 0>(a, m); break;
         case 13: op_u2<3, 1>(a, m); break;
         default: op_u2<3, 2>(a, m); break;
-        }
-        break;
-    }
-    case SW_DIAG: {
-        bool ta = bit_from(op.da, tid, chunk_base);
-        bool tb = bit_from(op.db, tid, chunk_base);
-        const bool ctrl = (op.rctrl | op.tctrl) != 0u || op.octrl != 0ULL;
-        if (!ctrl) {
-            op_diag(a, m, op, ta, tb);
-        } else {
-            const bool tcond = ((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl);
-            op_cdiag(a, m, op, ta, tcond);
         }
         break;
     }
