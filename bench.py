@@ -204,12 +204,166 @@ def bench_c1(args):
             "config": {"workload": "ry-qft-20q", "plan": ctx.plan_stats()}}
 
 
+def _line(metric, value, unit, args, ms, config, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+         "data": "synthetic", "config": config}
+    d.update(extra)
+    return d
+
+
+def bench_c3(args):
+    """Grover search, 33 qubits (128 GiB state), 8 iterations; marked probability vs the closed form."""
+    n = args.qubits if args.qubits != 30 else 33
+    circ, marked = workloads.grover(n, 8, 33)
+    ops, mats = lower(circ)
+    credit = workloads.amp_update_credit(circ)
+    ctx = capi.Context(0, MODES[args.mode])
+    st = capi.State(ctx, n)
+    want = workloads.grover_marked_probability(n, 8)
+
+    def step():
+        st.set_basis(0)
+        st.apply(ops, mats)
+        return abs(st.read(marked)[0]) ** 2
+
+    for _ in range(args.warmup):
+        step()
+    ctx.profile_reset()
+    ctx.set_option(capi.OPT_PROFILE, 1)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        p = step()
+        log(f"c3 step {i + 1}/{args.steps}: p(marked) = {p:.17g} (closed form {want:.17g})")
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.set_option(capi.OPT_PROFILE, 0)
+    launches, ms, nbytes = ctx.profile_read(capi.KFAM_SWEEP)
+    achieved = (nbytes / launches) / (ms / launches * 1e-3) / 1e9 if launches else 0.0
+    u = SplittableRandom(33).next_doubles(1024)
+    out, gap = st.sample(list(range(n)), u)
+    res = _line(f"{n}q grover amp-updates/s", credit * args.steps / el, "amp-updates/s", args,
+                1e3 * el / args.steps,
+                {"workload": f"grover-{n}q-8-iterations", "qubits": n, "ops": len(ops), "marked": marked,
+                 "plan": ctx.plan_stats()},
+                marked_probability=p, closed_form=want, rel_error=abs(p - want) / want,
+                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "kernel": "k_sweep"},
+                shots_1024_distinct=int(len(np.unique(out))))
+    st.close()
+    ctx.close()
+    return res
+
+
+def bench_c4(args):
+    """Batched QAOA MaxCut rings (instances of 12..22 qubits, p = 4) and depolarising GHZ-20 trajectories."""
+    rng = SplittableRandom(4)
+    n_inst, p = args.instances, 4
+    sizes = (rng.next_longs(n_inst) % np.uint64(11)).astype(np.int64) + 12
+    ctx = capi.Context(0, MODES[args.mode])
+    t0 = time.perf_counter()
+    mean_c = []
+    for n in range(12, 23):
+        idx = np.nonzero(sizes == n)[0]
+        if len(idx) == 0:
+            continue
+        B = len(idx)
+        g = rng.next_doubles(B * p).reshape(B, p) * math.pi
+        b = rng.next_doubles(B * p).reshape(B, p) * math.pi
+        w = 0.5 + rng.next_doubles(B * n).reshape(B, n)
+        ops, _ = lower(workloads.qaoa_ring(n, p, g[0], b[0], w[0]))
+        mats = np.stack([lower(workloads.qaoa_ring(n, p, g[k], b[k], w[k]))[1] for k in range(B)])
+        st = capi.State(ctx, n, B)
+        st.apply_batched(ops, mats)
+        zm = np.array([(1 << i) | (1 << ((i + 1) % n)) for i in range(n)], dtype=np.uint64)
+        # <C> = sum_e w_e (1 - <Z_i Z_j>_e) / 2 per instance
+        zz = np.stack([st.expectation(np.zeros(1, np.uint64), zm[e:e + 1], [1.0]) for e in range(n)])  # [n, B]
+        mean_c.extend(list(0.5 * np.sum(w.T * (1.0 - zz), axis=0)))
+        st.close()
+    ctx.synchronize()
+    t_qaoa = time.perf_counter() - t0
+    # GHZ-20 trajectories with depolarising p = 0.01 on both operands after every CNOT
+    nt, nq = args.trajectories, 20
+    gops, gmats = lower(workloads.ghz(nq))
+    noise_after = (gops["kind"] == capi.OP_CNOT).astype(np.uint8)
+    draws = 2 * int(noise_after.sum())
+    u = np.stack([SplittableRandom(1000 + t).next_doubles(draws) for t in range(nt)])
+    t1 = time.perf_counter()
+    st = capi.State(ctx, nq, nt)
+    st.run_depolarizing_trajectories(gops, gmats, noise_after, 0.01, u)
+    parity = st.expectation([0], [(1 << nq) - 1], [1.0])
+    ctx.synchronize()
+    t_traj = time.perf_counter() - t1
+    st.close()
+    return _line("batched QAOA instances/s (12-22 q, p=4) + GHZ-20 depolarising trajectories/s",
+                 n_inst / t_qaoa, "instances/s", args, 1e3 * (t_qaoa + t_traj),
+                 {"workload": "qaoa-ring-batched+ghz20-trajectories", "instances": n_inst,
+                  "trajectories": nt},
+                 qaoa_seconds=t_qaoa, mean_cut_expectation=float(np.mean(mean_c)),
+                 trajectories_per_s=nt / t_traj, trajectory_seconds=t_traj,
+                 ghz_parity_mean=float(np.mean(parity)))
+
+
+def bench_c5(args):
+    """Adjoint gradients of a 24-qubit, 8-layer hardware-efficient ansatz (384 parameters) for a batch of
+    parameter sets, against parameter shift on the same engine (timed on a sample, extrapolated)."""
+    n, layers, batch = 24, 8, args.param_sets
+    P = 2 * n * layers
+    rng = np.random.default_rng(5)
+    thetas = SplittableRandom(5).next_doubles(batch * P).reshape(batch, P) * 2 * math.pi
+    obs = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
+    ops, _ = lower(workloads.hea(n, layers, thetas[0]))
+    mats = np.stack([lower(workloads.hea(n, layers, thetas[k]))[1] for k in range(batch)])
+    ctx = capi.Context(0, MODES[args.mode])
+    st = capi.State(ctx, n, batch)
+    st.adjoint_gradient(ops, mats[:1].repeat(batch, 0), P, *obs, checkpoint_every=0, per_element=True)  # warm
+    st.set_basis(0)
+    t0 = time.perf_counter()
+    grads, energies = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=args.checkpoint, per_element=True)
+    ctx.synchronize()
+    t_adj = time.perf_counter() - t0
+    st.close()
+    # parameter shift on the plain engine: E(theta +- pi/2) per parameter, one state per evaluation
+    one = capi.State(ctx, n, 1)
+    sample = list(range(0, P, max(1, P // args.shift_sample)))[:args.shift_sample]
+    ps = []
+    t1 = time.perf_counter()
+    for p in sample:
+        vals = []
+        for sgn in (1, -1):
+            th = thetas[0].copy()
+            th[p] += sgn * math.pi / 2
+            o, m = lower(workloads.hea(n, layers, th))
+            one.set_basis(0)
+            one.apply(o, m)
+            vals.append(one.expectation(*obs)[0])
+        ps.append(0.5 * (vals[0] - vals[1]))
+    ctx.synchronize()
+    t_ps = time.perf_counter() - t1
+    one.close()
+    per_grad_ps = t_ps / len(sample)
+    adj_rate = batch * P / t_adj
+    ps_rate = 1.0 / per_grad_ps
+    err = float(np.max(np.abs(np.array(ps) - grads[0, sample])))
+    return _line("adjoint gradients/s, 24q HEA 384 params", adj_rate, "gradients/s", args, 1e3 * t_adj,
+                 {"workload": "hea-24q-8-layers-384-params", "param_sets": batch,
+                  "checkpoint_every": args.checkpoint},
+                 param_shift_gradients_per_s=ps_rate, speedup_vs_param_shift=adj_rate / ps_rate,
+                 param_shift_sample=len(sample), max_abs_diff_vs_param_shift=err,
+                 adjoint_stats=ctx.adjoint_stats(), energy_0=float(energies[0]))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--trajectories", type=int, default=1024)
+    ap.add_argument("--param-sets", type=int, default=64)
+    ap.add_argument("--checkpoint", type=int, default=144)
+    ap.add_argument("--shift-sample", type=int, default=8)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--shots", type=int, default=8192)
@@ -225,7 +379,7 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",
               file=sys.stderr)
         return 2
-    res = bench_c2(args) if args.workload == "c2" else bench_c1(args)
+    res = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
     return 0
 

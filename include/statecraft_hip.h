@@ -130,6 +130,9 @@ int sc_state_upload(sc_state *st, const double *interleaved, uint64_t n_complex)
 int sc_state_download(sc_state *st, double *interleaved, uint64_t n_complex);
 /* Device-to-device copy of all amplitudes (same shape). */
 int sc_state_copy(sc_state *dst, const sc_state *src);
+/* Reads amplitudes [offset, offset + count) of batch element `elem` (StateVector.amplitude access without
+ * copying a whole 2^n state to the host). */
+int sc_state_read_range(sc_state *st, int elem, uint64_t offset, uint64_t count, double *interleaved);
 
 /* ---- gate application ----------------------------------------------------- */
 /* The op loop of StatevectorEngine.simulate :90-151 for every unitary op kind;

@@ -76,7 +76,7 @@ EXPORTED = [
     "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
     "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_expectation_pauli",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
-    "sc_plan_sweeps_host",
+    "sc_plan_sweeps_host", "sc_state_read_range",
 ]
 
 _lib = None
@@ -109,6 +109,7 @@ def _sig(lib):
         "sc_state_upload": (i32, [P, DP, u64]),
         "sc_state_download": (i32, [P, DP, u64]),
         "sc_state_copy": (i32, [P, P]),
+        "sc_state_read_range": (i32, [P, i32, u64, u64, DP]),
         "sc_apply_ops": (i32, [P, P, u64, DP, u64]),
         "sc_apply_ops_batched": (i32, [P, P, u64, DP, u64]),
         "sc_probabilities": (i32, [P, i32, I32P, i32, DP]),
@@ -265,6 +266,11 @@ class State:
 
     def copy_from(self, other: "State") -> None:
         check(lib().sc_state_copy(self.h, other.h))
+
+    def read(self, offset: int, count: int = 1, elem: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=np.complex128)
+        check(lib().sc_state_read_range(self.h, elem, int(offset), int(count), dptr(out.view(np.float64))))
+        return out
 
     def apply(self, ops: np.ndarray, mats) -> None:
         ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)

@@ -486,6 +486,20 @@ int sc_state_copy(sc_state *dst, const sc_state *src) {
     return SC_OK;
 }
 
+int sc_state_read_range(sc_state *st, int elem, uint64_t offset, uint64_t count, double *interleaved) {
+    SC_REQUIRE(st, "state");
+    if (count > 0) SC_REQUIRE(interleaved, "data");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    if (offset > st->dim() || count > st->dim() - offset) return fail(SC_E_ARG, "amplitude range out of bounds");
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
+    if (count > 0)
+        SC_HIP_CHECK(hipMemcpy(interleaved, st->amp + ((uint64_t)elem << st->n) + offset, count * sizeof(double2),
+                               hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
 int sc_apply_ops(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex) {
     SC_REQUIRE(st, "state");
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
