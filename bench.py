@@ -39,6 +39,14 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def make_context(args) -> capi.Context:
+    ctx = capi.Context(0, MODES[args.mode])
+    ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
+    ctx.set_option(capi.OPT_REMAP, 0 if args.no_remap else 1)
+    ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
+    return ctx
+
+
 def cpu_baseline(n, ops, mats, seconds: float):
     """The oracle (C port of StatevectorOps, 1 thread) on the first ops of the same circuit until `seconds`."""
     import oracle
@@ -65,9 +73,7 @@ def bench_c2(args):
     circ = workloads.random_layered(n, depth, args.seed)
     ops, mats = lower(circ)
     credit = workloads.amp_update_credit(circ)
-    ctx = capi.Context(0, MODES[args.mode])
-    ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
-    ctx.set_option(capi.OPT_REMAP, 0 if args.no_remap else 1)
+    ctx = make_context(args)
     free, total, cus = ctx.device_info()
     st = capi.State(ctx, n)
     measured = list(range(n))
@@ -135,6 +141,7 @@ def bench_c2(args):
             "qubits": n, "depth": depth, "ops": int(len(ops)), "shots": args.shots, "exec_mode": args.mode,
             "sweeps_per_step": sweeps, "kernel_launches_per_step": plan["launches"],
             "fused_16x16_blocks": plan["fused_blocks"], "parallelism": "1 GPU",
+            "low_qubits": args.low_qubits, "remap": not args.no_remap, "blocks_per_cu": args.blocks_per_cu,
         },
         "roofline": {
             "bound": "hbm",
@@ -178,7 +185,7 @@ def bench_c1(args):
     circ, _ = workloads.ry_qft(n, 42)
     ops, mats = lower(circ)
     credit = workloads.amp_update_credit(circ)
-    ctx = capi.Context(0, MODES[args.mode])
+    ctx = make_context(args)
     st = capi.State(ctx, n)
     u = SplittableRandom(42).next_doubles(4096)
     amps = np.empty(1 << n, dtype=np.complex128)
@@ -218,7 +225,7 @@ def bench_c3(args):
     circ, marked = workloads.grover(n, 8, 33)
     ops, mats = lower(circ)
     credit = workloads.amp_update_credit(circ)
-    ctx = capi.Context(0, MODES[args.mode])
+    ctx = make_context(args)
     st = capi.State(ctx, n)
     want = workloads.grover_marked_probability(n, 8)
 
@@ -260,7 +267,7 @@ def bench_c4(args):
     rng = SplittableRandom(4)
     n_inst, p = args.instances, 4
     sizes = (rng.next_longs(n_inst) % np.uint64(11)).astype(np.int64) + 12
-    ctx = capi.Context(0, MODES[args.mode])
+    ctx = make_context(args)
     t0 = time.perf_counter()
     mean_c = []
     for n in range(12, 23):
@@ -314,7 +321,7 @@ def bench_c5(args):
     obs = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
     ops, _ = lower(workloads.hea(n, layers, thetas[0]))
     mats = np.stack([lower(workloads.hea(n, layers, thetas[k]))[1] for k in range(batch)])
-    ctx = capi.Context(0, MODES[args.mode])
+    ctx = make_context(args)
     st = capi.State(ctx, n, batch)
     st.adjoint_gradient(ops, mats[:1].repeat(batch, 0), P, *obs, checkpoint_every=0, per_element=True)  # warm
     st.set_basis(0)
@@ -373,6 +380,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-copy-reference", action="store_true")
     ap.add_argument("--low-qubits", type=int, default=4)
     ap.add_argument("--no-remap", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args(argv)
     if args.gpus != 1:

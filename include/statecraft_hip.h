@@ -92,12 +92,13 @@ typedef struct sc_op {
 
 /* ---- execution options (sc_ctx_set_option) -------------------------------- */
 #define SC_OPT_EXEC_MODE 1      /* SC_EXEC_*; default SC_EXEC_SWEEP */
-#define SC_OPT_MAX_FUSED 2      /* max qubits of a dense fused block (1..5), default 4 */
+#define SC_OPT_MAX_FUSED 2      /* max qubits of a dense fused block (0..5; sweeps fuse at most 4, 0 = no sweep
+                                   fusion, fused mode at least 1), default 4 */
 #define SC_OPT_PROFILE 3        /* 1: time every sweep/gate launch with HIP events */
 #define SC_OPT_LOW_QUBITS 4     /* qubits always resident in a sweep chunk for coalescing (0..8), default 4 */
 #define SC_OPT_TEST_DRIFT 5     /* test hook: adds value*1e-12 to psi[0] at every adjoint checkpoint, default 0 */
 #define SC_OPT_REMAP 6          /* 1: sweep passes relabel qubits in place to keep coalesced low positions useful, default 1 */
-
+#define SC_OPT_SWEEP_BLOCKS_PER_CU 7 /* resident sweep workgroups per CU (0: as many as fit), default 0 */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */

@@ -42,9 +42,13 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libstatecraft_hip.so")
 
 
+# Per-source extra flags.
+EXTRA: dict = {}
+
+
 def _compile(src: Path) -> Path:
     obj = OBJDIR / (src.name + ".o")
-    cmd = [hipcc(), *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [hipcc(), *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -8,8 +8,10 @@ There is no fallback: if the library is missing, loading fails loudly.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -20,7 +22,8 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libstatecraft_hip.so"
 SC_OK, SC_E_NULL, SC_E_ARG, SC_E_UNSUPPORTED, SC_E_STATE, SC_E_OOM, SC_E_HIP = range(7)
 OP_SINGLE, OP_CNOT, OP_TWO, OP_DIAG2, OP_SWAP, OP_MCU, OP_MEASURE = range(1, 8)
 GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
-OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP = range(1, 7)
+(OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP,
+ OPT_SWEEP_BLOCKS_PER_CU) = range(1, 8)
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
 KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 
@@ -191,6 +194,8 @@ class Context:
         h = P()
         check(lib().sc_ctx_create(device, ctypes.byref(h)))
         self.h = h
+        self._states = weakref.WeakSet()  # states to release before the context
+        _live_contexts.add(self)
         if mode is not None:
             self.set_option(OPT_EXEC_MODE, mode)
 
@@ -229,13 +234,30 @@ class Context:
         return {"reanchors": a.value, "max_drift": b.value}
 
     def close(self) -> None:
+        """Release the context; its states that are still open are released first."""
         if getattr(self, "h", None):
+            for s in list(self._states):
+                s.close()
             lib().sc_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+_live_contexts: "weakref.WeakSet[Context]" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all() -> None:
+    # Device memory and streams are released while the HIP runtime is still up, whatever order the
+    # interpreter would finalise module globals in.
+    for c in list(_live_contexts):
+        try:
+            c.close()
         except Exception:
             pass
 
@@ -247,6 +269,7 @@ class State:
         h = P()
         check(lib().sc_state_create(ctx.h, int(n), int(batch), ctypes.byref(h)))
         self.h, self.ctx, self.n, self.batch = h, ctx, int(n), int(batch)
+        ctx._states.add(self)
 
     @property
     def size(self) -> int:

@@ -190,7 +190,7 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 // mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
 int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
     SweepPlan plan;
-    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan);
+    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan, std::min(ctx->max_fused, kRegBits));
     if (rc) return rc;
     if (plan.passes.empty()) return SC_OK;
     const size_t pass_bytes = plan.passes.size() * sizeof(SweepPass);
@@ -330,7 +330,7 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         ctx->exec_mode = (int)value;
         return SC_OK;
     case SC_OPT_MAX_FUSED:
-        if (value < 1 || value > 5) return fail(SC_E_ARG, "max fused qubits must be 1..5");
+        if (value < 0 || value > 5) return fail(SC_E_ARG, "max fused qubits must be 0..5");
         ctx->max_fused = (int)value;
         return SC_OK;
     case SC_OPT_PROFILE: {
@@ -350,6 +350,10 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
     case SC_OPT_REMAP:
         ctx->remap = value != 0;
         return SC_OK;
+    case SC_OPT_SWEEP_BLOCKS_PER_CU:
+        if (value < 0 || value > 8) return fail(SC_E_ARG, "sweep blocks per CU must be 0..8");
+        ctx->sweep_blocks_per_cu = (int)value;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -364,6 +368,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_LOW_QUBITS: *value = ctx->low_qubits; return SC_OK;
     case SC_OPT_TEST_DRIFT: *value = ctx->test_drift; return SC_OK;
     case SC_OPT_REMAP: *value = ctx->remap; return SC_OK;
+    case SC_OPT_SWEEP_BLOCKS_PER_CU: *value = ctx->sweep_blocks_per_cu; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/statecraft_hip.h"
+#include "sc_ops.hpp"
 
 namespace sc {
 
@@ -59,6 +60,7 @@ struct Context {
     int max_fused = 4;
     int low_qubits = 4;
     int remap = 1;
+    int sweep_blocks_per_cu = 0;
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)
@@ -95,53 +97,30 @@ struct Mat4 {
 // One chunk = 2^12 amplitudes: 12 "local" qubits resident in LDS; every
 // other qubit is fixed per workgroup.  A pass runs stages; in a stage each of
 // 256 threads holds 16 amplitudes that differ in 4 "register" local bits.
-constexpr int kChunkBits = 12;
-constexpr int kChunk = 1 << kChunkBits;
-constexpr int kRegBits = 4;
-constexpr int kRegs = 1 << kRegBits;
-constexpr int kThreadBits = kChunkBits - kRegBits;  // 8
-constexpr int kThreads = 1 << kThreadBits;          // 256
 constexpr int kMaxStages = 24;
 constexpr int kSwzBits = 4;  // LDS slot bits permuted by the XOR swizzle
 
-enum SweepOpKind : int32_t {
-    SW_U1 = 1,     // (controlled) 2x2 on register bit k0
-    SW_X = 2,      // (controlled) exact swap on register bit k0 (CNOT / MC-X)
-    SW_U2 = 3,     // dense 4x4 on register bits (k0 = first = MSB, k1 = second)
-    SW_DIAG = 4,   // 2-entry or 4-entry diagonal on up to two qubits anywhere
-    SW_SWAP = 5,   // exact exchange of register bits k0, k1
-    SW_DENSE4 = 6, // dense 16x16 on all four register bits (fused block)
-    SW_U1R = 7,    // SW_U1 with a real matrix (imaginary parts zero): half the FP64 work
-    SW_U2R = 8,    // SW_U2 with a real matrix
-    SW_DENSE4R = 9,// SW_DENSE4 with a real matrix
-};
 
-// Where a qubit's bit of an amplitude index comes from inside a stage.
-struct BitSrc {
-    uint32_t rmask;  // register-index mask (one bit) or 0
-    uint32_t tmask;  // thread-index mask (one bit) or 0
-    uint32_t outer;  // 1 if the bit is an outer qubit that is set in this chunk (resolved in-kernel)
-    uint32_t omask_lo, omask_hi;  // outer qubit mask (global) when the qubit is outer, else 0
-};
-
+// 32 bytes: the kernel reads a whole record with one scalar load.
 struct SweepOp {
-    int32_t kind;
-    int32_t k0, k1;     // register bit positions
-    int32_t mat;        // complex offset into the pass matrix pool
-    // controls (SW_U1 / SW_X): register part, thread part, outer part
-    uint32_t rctrl;     // register-index mask
-    uint32_t tctrl;     // thread-index mask
-    uint64_t octrl;     // global mask of outer controls
-    // SW_DIAG: qubit a (MSB of the 2-bit diagonal index) and b (LSB); nd = 1 or 2 entries bits
-    int32_t nd;
-    int32_t pad;
-    BitSrc da, db;
+    int8_t kind;
+    int8_t k0, k1;   // register bit positions (-1: none)
+    int8_t pad0;
+    int32_t mat;     // complex offset of the op's matrix / factor table in the pass pool
+    uint16_t rctrl;  // register-index control mask
+    uint16_t tctrl;  // thread-index control mask
+    uint8_t ta, tb;  // SW_DIAGL: thread-index masks of the diagonal's bit sources
+    int8_t oa, ob;   // SW_DIAGL: outer (global) qubit positions of the diagonal's bit sources, -1: none
+    uint64_t octrl;  // outer (chunk-fixed) control mask, global qubit positions
+    uint64_t pad1;
 };
+static_assert(sizeof(SweepOp) == 32, "sweep op records are 32 bytes");
 
+// Slot tables are 32-bit so that the kernel reads them with scalar dword loads.
 struct SweepStage {
     int32_t op_begin, op_end;
-    uint16_t lds_off[kRegs];      // swizzled LDS slot offset per register index
-    uint16_t thread_slot[kThreadBits];  // swizzled slot contribution of each thread-index bit
+    uint32_t lds_off[kRegs];            // swizzled LDS slot offset per register index
+    uint32_t thread_slot[kThreadBits];  // swizzled slot contribution of each thread-index bit
 };
 
 struct SweepPass {
@@ -150,12 +129,12 @@ struct SweepPass {
     int32_t pad;
     uint64_t glob_thread[kThreadBits];  // load/store phase: global offset of thread-index bit i
     uint64_t glob_reg[kRegs];       // load/store phase: global offset of register index r
-    uint16_t io_thread_slot[kThreadBits];  // load phase: swizzled slot of thread-index bit i
-    uint16_t io_reg_slot[kRegs];
+    uint32_t io_thread_slot[kThreadBits];  // load phase: swizzled slot of thread-index bit i
+    uint32_t io_reg_slot[kRegs];
     // Store phase: thread bit i / register bit j still address chunk positions i / 8+j in HBM, but read the
     // LDS slot of the local qubit the pass moves there.  Equal to io_* when the pass does not remap qubits.
-    uint16_t st_thread_slot[kThreadBits];
-    uint16_t st_reg_slot[kRegs];
+    uint32_t st_thread_slot[kThreadBits];
+    uint32_t st_reg_slot[kRegs];
     SweepStage stages[kMaxStages];
 };
 

@@ -336,99 +336,146 @@ int gf2_rank(std::vector<uint32_t> cols) {
     return rank;
 }
 
-// Register-level op semantics on 16 complex values (mirror of the device ops, all conditions true).
-void host_reg_op(const SweepOp &op, const double *m, double *a /* 32 doubles */) {
+// Mirror of the device run_op (sc_sweep.hip): applies one register op to the 16 complex values (32 doubles)
+// of thread `tid` in the chunk whose outer offset is cb.  m is the op's matrix / factor table.  SW_DIAGL may
+// also carry the planner's form with both diagonal sources in register bits (k1 >= 0); plans hand such ops to
+// the device only inside a SW_DIAG16 table.
+void host_apply(const SweepOp &op, const double *m, double *a, uint32_t tid, uint64_t cb) {
+    if ((cb & op.octrl) != op.octrl) return;
     auto cm = [](const double *g, const double *x, double *y) {  // y = g * x
         y[0] = g[0] * x[0] - g[1] * x[1];
         y[1] = g[0] * x[1] + g[1] * x[0];
     };
+    const bool lane = (tid & op.tctrl) == op.tctrl;
+    double out[2 * kRegs];
+    std::memcpy(out, a, sizeof(out));
     switch (op.kind) {
     case SW_U1:
     case SW_U1R:
-    case SW_X: {
+    case SW_U1C:
+    case SW_X:
+    case SW_XL:
+    case SW_CX: {
+        if (!lane) break;
+        const bool x = op.kind == SW_X || op.kind == SW_XL || op.kind == SW_CX;
+        const uint32_t rc = op.kind == SW_CX ? 1u << op.k1 : op.rctrl;
         for (int r0 = 0; r0 < kRegs; r0++) {
             if (r0 & (1 << op.k0)) continue;
-            if (((uint32_t)r0 & op.rctrl) != op.rctrl) continue;
-            int r1 = r0 | (1 << op.k0);
-            double x[2] = {a[2 * r0], a[2 * r0 + 1]}, y[2] = {a[2 * r1], a[2 * r1 + 1]};
-            if (op.kind == SW_X) {
-                a[2 * r0] = y[0], a[2 * r0 + 1] = y[1], a[2 * r1] = x[0], a[2 * r1 + 1] = x[1];
+            if (((uint32_t)r0 & rc) != rc) continue;
+            const int r1 = r0 | (1 << op.k0);
+            const double *xv = a + 2 * r0, *yv = a + 2 * r1;
+            if (x) {
+                out[2 * r0] = yv[0], out[2 * r0 + 1] = yv[1], out[2 * r1] = xv[0], out[2 * r1 + 1] = xv[1];
             } else {
                 double t0[2], t1[2], t2[2], t3[2];
-                cm(m + 0, x, t0);
-                cm(m + 2, y, t1);
-                cm(m + 4, x, t2);
-                cm(m + 6, y, t3);
-                a[2 * r0] = t0[0] + t1[0], a[2 * r0 + 1] = t0[1] + t1[1];
-                a[2 * r1] = t2[0] + t3[0], a[2 * r1 + 1] = t2[1] + t3[1];
+                cm(m + 0, xv, t0);
+                cm(m + 2, yv, t1);
+                cm(m + 4, xv, t2);
+                cm(m + 6, yv, t3);
+                out[2 * r0] = t0[0] + t1[0], out[2 * r0 + 1] = t0[1] + t1[1];
+                out[2 * r1] = t2[0] + t3[0], out[2 * r1 + 1] = t2[1] + t3[1];
             }
         }
         break;
     }
     case SW_U2:
-    case SW_U2R: {
+    case SW_U2R:
         for (int b = 0; b < kRegs; b++) {
             if (b & ((1 << op.k0) | (1 << op.k1))) continue;
-            int idx[4] = {b, b | (1 << op.k1), b | (1 << op.k0), b | (1 << op.k0) | (1 << op.k1)};
-            double x[8];
-            for (int c = 0; c < 4; c++) x[2 * c] = a[2 * idx[c]], x[2 * c + 1] = a[2 * idx[c] + 1];
+            const int idx[4] = {b, b | (1 << op.k1), b | (1 << op.k0), b | (1 << op.k0) | (1 << op.k1)};
             for (int r = 0; r < 4; r++) {
                 double re = 0, im = 0;
                 for (int c = 0; c < 4; c++) {
                     double t[2];
-                    cm(m + 2 * (4 * r + c), x + 2 * c, t);
+                    cm(m + 2 * (4 * r + c), a + 2 * idx[c], t);
                     re += t[0], im += t[1];
                 }
-                a[2 * idx[r]] = re, a[2 * idx[r] + 1] = im;
+                out[2 * idx[r]] = re, out[2 * idx[r] + 1] = im;
             }
         }
         break;
-    }
-    case SW_SWAP: {
-        for (int r = 0; r < kRegs; r++)
-            if ((r & (1 << op.k0)) && !(r & (1 << op.k1))) {
-                int s = r ^ (1 << op.k0) ^ (1 << op.k1);
-                std::swap(a[2 * r], a[2 * s]);
-                std::swap(a[2 * r + 1], a[2 * s + 1]);
-            }
+    case SW_SWAP:
+        for (int r = 0; r < kRegs; r++) {
+            const int b0 = (r >> op.k0) & 1, b1 = (r >> op.k1) & 1;
+            const int s = (r & ~((1 << op.k0) | (1 << op.k1))) | (b1 << op.k0) | (b0 << op.k1);
+            out[2 * r] = a[2 * s], out[2 * r + 1] = a[2 * s + 1];
+        }
         break;
-    }
-    case SW_DIAG: {
+    case SW_D4:
+    case SW_D4R:
+        for (int r = 0; r < kRegs; r++) {
+            double re = 0, im = 0;
+            for (int c = 0; c < kRegs; c++) {
+                double t[2];
+                cm(m + 2 * (r * kRegs + c), a + 2 * c, t);
+                re += t[0], im += t[1];
+            }
+            out[2 * r] = re, out[2 * r + 1] = im;
+        }
+        break;
+    case SW_DIAG16:
+        for (int r = 0; r < kRegs; r++) cm(m + 2 * r, a + 2 * r, out + 2 * r);
+        break;
+    case SW_DIAGL: {
+        if (!lane) break;
+        const int bl = ((tid & op.tb) != 0u || (op.ob >= 0 && ((cb >> op.ob) & 1ULL))) ? 1 : 0;
+        const int al = ((tid & op.ta) != 0u || (op.oa >= 0 && ((cb >> op.oa) & 1ULL))) ? 1 : 0;
         for (int r = 0; r < kRegs; r++) {
             if (((uint32_t)r & op.rctrl) != op.rctrl) continue;
-            int ba = ((uint32_t)r & op.da.rmask) ? 1 : 0;
-            int bb = ((uint32_t)r & op.db.rmask) ? 1 : 0;
-            int l = op.nd == 1 ? ba : 2 * ba + bb;
-            double x[2] = {a[2 * r], a[2 * r + 1]}, y[2];
-            cm(m + 2 * l, x, y);
-            a[2 * r] = y[0], a[2 * r + 1] = y[1];
+            const int A = op.k0 >= 0 ? (r >> op.k0) & 1 : al;
+            const int B = op.k1 >= 0 ? (r >> op.k1) & 1 : bl;
+            cm(m + 2 * (2 * A + B), a + 2 * r, out + 2 * r);
         }
         break;
     }
     default: break;
     }
+    std::memcpy(a, out, sizeof(out));
 }
 
 // VALU operations per amplitude of a register op (complex multiply-add = 4).  Ops with a thread-index control
 // or diagonal source pay per-lane selects on top; the rest resolve their conditions per workgroup.
-double op_cost(const SweepOp &op) {
-    const bool lane = (op.tctrl | op.da.tmask | op.db.tmask) != 0u;
-    switch (op.kind) {
-    case SW_U1: return lane ? 12.0 : 8.0;
-    case SW_U1R: return 4.0;
-    case SW_U2: return 16.0;
-    case SW_U2R: return 8.0;
-    case SW_DIAG: return lane ? 12.0 : 6.0;
-    case SW_X: return lane ? 4.0 : 1.0;
-    case SW_SWAP: return 0.5;
-    case SW_DENSE4R: return 32.0;
-    default: return 64.0;
-    }
+// Every op also pays a fixed issue cost that fusing ops saves: scalar loads of its record and matrix and
+// about a dozen dispatch branches, each of which stalls the wavefront's instruction fetch.  Measured on
+// gfx950 at about half the cost of a 2x2 complex gate, i.e. ~8 FP64 operations per amplitude.
+constexpr double kOpOverhead = 8.0;
+
+// Qubits an op needs as register bits of its stage (local = the pass's local qubits): its targets and, for
+// a non-diagonal controlled op whose qubits fit in the register bits, its local controls too -- it then
+// needs no per-lane condition and can fuse with its neighbours.  Chunk-fixed controls are uniform per
+// workgroup and stay outside; diagonal factors and controls of diagonal ops may come from thread-index bits
+// at little cost.
+uint64_t reg_need(const PlanOp &op, uint64_t local) {
+    if (op.is_diag || op.controls == 0) return op.targets;
+    const uint64_t both = op.targets | (op.controls & local);
+    return popc(both) <= kRegBits ? both : op.targets;
 }
 
-double fused_cost(int bits, bool real) {
+double op_cost(const SweepOp &op) {
+    double c;
+    switch (op.kind) {
+    case SW_U1: c = 8.0; break;
+    case SW_U1R: c = 4.0; break;
+    case SW_U1C: c = op.tctrl ? 9.0 : 8.0; break;
+    case SW_X: c = op.rctrl ? 4.0 : 2.0; break;  // register controls: per-pair selects; else moves
+    case SW_CX: c = 2.0; break;
+    case SW_XL: c = 4.0; break;
+    case SW_U2: c = 16.0; break;
+    case SW_U2R: c = 8.0; break;
+    case SW_SWAP: c = 2.0; break;
+    case SW_D4R: c = 32.0; break;
+    case SW_DIAG16: c = 4.0; break;
+    case SW_DIAGL: c = op.rctrl ? 8.0 : 4.0; break;
+    default: c = 64.0; break;
+    }
+    return c + kOpOverhead;
+}
+
+// A fused segment: a diagonal over the register index (any number of bits) or a dense block on its bits.
+double fused_cost(int bits, bool real, bool diag) {
+    if (diag) return 4.0 + kOpOverhead;
     double c = bits <= 1 ? 8.0 : bits == 2 ? 16.0 : 64.0;
-    return real ? 0.5 * c : c;
+    return (real ? 0.5 * c : c) + kOpOverhead;
 }
 
 // Register bits a register-internal op reads or writes.
@@ -436,29 +483,32 @@ uint32_t reg_support(const SweepOp &op) {
     switch (op.kind) {
     case SW_U1:
     case SW_U1R:
+    case SW_U1C:
     case SW_X: return (1u << op.k0) | op.rctrl;
+    case SW_CX:
     case SW_U2:
     case SW_U2R:
     case SW_SWAP: return (1u << op.k0) | (1u << op.k1);
-    case SW_DIAG: return op.da.rmask | (op.nd == 2 ? op.db.rmask : 0u) | op.rctrl;
+    case SW_DIAGL: return (op.k0 >= 0 ? 1u << op.k0 : 0u) | (op.k1 >= 0 ? 1u << op.k1 : 0u) | op.rctrl;
     default: return (1u << kRegBits) - 1u;
     }
 }
 
+bool is_diag_kind(const SweepOp &op) { return op.kind == SW_DIAGL || op.kind == SW_DIAG16; }
+
+// Ops that only read and write register bits (no thread-index or chunk-fixed dependence): they can be folded
+// into a fused block or diagonal table.
 bool register_internal(const SweepOp &op) {
     if (op.tctrl != 0u || op.octrl != 0ULL) return false;
-    if (op.kind == SW_DIAG) {
-        bool a_ok = op.da.tmask == 0u && op.da.omask_lo == 0u && op.da.omask_hi == 0u;
-        bool b_ok = op.nd == 1 || (op.db.tmask == 0u && op.db.omask_lo == 0u && op.db.omask_hi == 0u);
-        return a_ok && b_ok;
-    }
-    return op.kind == SW_U1 || op.kind == SW_U1R || op.kind == SW_X || op.kind == SW_U2 || op.kind == SW_U2R ||
-           op.kind == SW_SWAP;
+    if (op.kind == SW_DIAGL) return op.ta == 0u && op.tb == 0u && op.oa < 0 && op.ob < 0 && op.k0 >= 0;
+    return op.kind == SW_U1 || op.kind == SW_U1R || op.kind == SW_U1C || op.kind == SW_X || op.kind == SW_CX ||
+           op.kind == SW_U2 || op.kind == SW_U2R || op.kind == SW_SWAP;
 }
 
 }  // namespace
 
-int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan) {
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
+                int max_fuse_bits) {
     plan = SweepPlan();
     if (n < kChunkBits) return fail(SC_E_ARG, "sweep plans need at least 12 qubits");
     const uint64_t full = (n == 64) ? ~0ULL : ((1ULL << n) - 1ULL);
@@ -522,6 +572,8 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 if (popc(L | add) <= kChunkBits) L |= add;
             }
         }
+        // ---- local qubits: fill to 12 with the qubits at the lowest unused physical positions
+        for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
         // ---- split into stages
         std::vector<StageBuild> stages;
         std::vector<int> rest = sched;
@@ -531,12 +583,13 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             std::vector<int> rest2;
             for (int i : rest) {
                 const PlanOp &op = ops[i];
-                if (sblk.blocked(op) || popc(sb.R | op.targets) > kRegBits) {
+                const uint64_t need = reg_need(op, L);
+                if (sblk.blocked(op) || popc(sb.R | need) > kRegBits) {
                     sblk.mark(op);
                     rest2.push_back(i);
                     continue;
                 }
-                sb.R |= op.targets;
+                sb.R |= need;
                 sb.ops.push_back(i);
             }
             stages.push_back(sb);
@@ -642,6 +695,21 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             uint32_t rp = 0;
             for (int q = 0; q < n; q++)
                 if ((stages[s].R >> q) & 1ULL) rp |= 1u << lp[q];
+            // spare register bits go to the local qubits the stage's ops use as controls / diagonal sources
+            // (most uses first): such ops then resolve per register index instead of per lane, and can fuse
+            if (popc(rp) < kRegBits) {
+                int uses[kChunkBits] = {0};
+                for (int oi : stages[s].ops)
+                    for (int q = 0; q < n; q++)
+                        if (((ops[oi].diag >> q) & 1ULL) && lp[q] >= 0) uses[lp[q]]++;
+                while (popc(rp) < kRegBits) {
+                    int best = -1;
+                    for (int p = 0; p < kChunkBits; p++)
+                        if (!((rp >> p) & 1u) && uses[p] > 0 && (best < 0 || uses[p] > uses[best])) best = p;
+                    if (best < 0) break;
+                    rp |= 1u << best;
+                }
+            }
             for (int p = kChunkBits - 1; p >= 0 && popc(rp) < kRegBits; p--)
                 if (!((rp >> p) & 1u)) rp |= 1u << p;
             int rj = 0, tj = 0;
@@ -694,7 +762,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // load / store phase layout: thread bit i -> local i, register bit j -> local 8 + j
         for (int i = 0; i < kThreadBits; i++) {
             pass.glob_thread[i] = 1ULL << lq[i];
-            pass.io_thread_slot[i] = (uint16_t)slot_of(1u << i);
+            pass.io_thread_slot[i] = slot_of(1u << i);
         }
         for (int r = 0; r < kRegs; r++) {
             uint64_t g = 0;
@@ -705,26 +773,26 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     l |= 1u << (kThreadBits + j);
                 }
             pass.glob_reg[r] = g;
-            pass.io_reg_slot[r] = (uint16_t)slot_of(l);
+            pass.io_reg_slot[r] = slot_of(l);
         }
         // store phase: chunk position k takes the value of local position pi[k]
-        for (int i = 0; i < kThreadBits; i++) pass.st_thread_slot[i] = (uint16_t)slot_of(1u << pi[i]);
+        for (int i = 0; i < kThreadBits; i++) pass.st_thread_slot[i] = slot_of(1u << pi[i]);
         for (int r = 0; r < kRegs; r++) {
             uint32_t l = 0;
             for (int j = 0; j < kRegBits; j++)
                 if ((r >> j) & 1) l |= 1u << pi[kThreadBits + j];
-            pass.st_reg_slot[r] = (uint16_t)slot_of(l);
+            pass.st_reg_slot[r] = slot_of(l);
         }
         // ---- stage ops
         for (size_t s = 0; s < stages.size(); s++) {
             SweepStage &stg = pass.stages[s];
             const StageLayout &lay = layouts[s];
-            for (int i = 0; i < kThreadBits; i++) stg.thread_slot[i] = (uint16_t)slot_of(1u << lay.tpos[i]);
+            for (int i = 0; i < kThreadBits; i++) stg.thread_slot[i] = slot_of(1u << lay.tpos[i]);
             for (int r = 0; r < kRegs; r++) {
                 uint32_t l = 0;
                 for (int j = 0; j < kRegBits; j++)
                     if ((r >> j) & 1) l |= 1u << lay.rpos[j];
-                stg.lds_off[r] = (uint16_t)slot_of(l);
+                stg.lds_off[r] = slot_of(l);
             }
             auto reg_of = [&](int q) -> int {
                 int p = lp[q];
@@ -732,105 +800,147 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     if (lay.rpos[j] == p) return j;
                 return -1;
             };
-            auto src_of = [&](int q) -> BitSrc {
-                BitSrc b;
-                std::memset(&b, 0, sizeof(b));
+            // where a qubit's index bit comes from in this stage: a register bit, a thread-index bit or an
+            // outer (chunk-fixed) qubit at its physical position
+            struct Src {
+                int reg = -1;     // register bit
+                uint32_t t = 0;   // thread-index mask
+                int opos = -1;    // outer qubit's global position
+            };
+            auto src_of = [&](int q) -> Src {
+                Src b;
                 if (lp[q] >= 0) {
-                    int j = reg_of(q);
-                    if (j >= 0) {
-                        b.rmask = 1u << j;
-                    } else {
+                    b.reg = reg_of(q);
+                    if (b.reg < 0)
                         for (int i = 0; i < kThreadBits; i++)
-                            if (lay.tpos[i] == lp[q]) b.tmask = 1u << i;
-                    }
+                            if (lay.tpos[i] == lp[q]) b.t = 1u << i;
                 } else {
-                    uint64_t om = 1ULL << phys[q];
-                    b.omask_lo = (uint32_t)om;
-                    b.omask_hi = (uint32_t)(om >> 32);
+                    b.opos = phys[q];
                 }
                 return b;
             };
             auto add_controls = [&](SweepOp &so, uint64_t controls) {
                 for (int q = 0; q < n; q++) {
                     if (!((controls >> q) & 1ULL)) continue;
-                    BitSrc b = src_of(q);
-                    so.rctrl |= b.rmask;
-                    so.tctrl |= b.tmask;
-                    so.octrl |= ((uint64_t)b.omask_hi << 32) | b.omask_lo;
+                    const Src b = src_of(q);
+                    if (b.reg >= 0) so.rctrl |= (uint16_t)(1u << b.reg);
+                    so.tctrl |= (uint16_t)b.t;
+                    if (b.opos >= 0) so.octrl |= 1ULL << b.opos;
                 }
+            };
+            // an exchange whose only control is one register bit: a CNOT inside the registers
+            auto set_exchange = [&](SweepOp &so, int target) {
+                so.k0 = (int8_t)reg_of(target);
+                if (so.tctrl) {
+                    so.kind = SW_XL;
+                } else if (so.octrl == 0 && popc(so.rctrl) == 1) {
+                    so.kind = SW_CX;
+                    so.k1 = (int8_t)__builtin_ctz(so.rctrl);
+                    so.rctrl = 0;
+                } else {
+                    so.kind = SW_X;
+                }
+            };
+            auto new_op = []() {
+                SweepOp so;
+                std::memset(&so, 0, sizeof(so));
+                so.mat = -1;
+                so.k0 = so.k1 = -1;
+                so.oa = so.ob = -1;
+                return so;
             };
             std::vector<SweepOp> sops;
             std::vector<bool> sreal;  // op's matrix is real
             for (int oi : stages[s].ops) {
                 const PlanOp &op = ops[oi];
-                SweepOp so;
-                std::memset(&so, 0, sizeof(so));
-                so.mat = -1;
+                SweepOp so = new_op();
                 switch (op.kind) {
                 case SC_OP_SINGLE:
                 case SC_OP_MCU:
+                    add_controls(so, op.controls);
                     if (op.is_diag) {
-                        so.kind = SW_DIAG;
-                        so.nd = 1;
-                        so.da = src_of(op.q0);
-                        so.mat = recipe_for(1, op.mat, 2);
-                        add_controls(so, op.controls);
+                        // entries (g00, g00, g11, g11): factor m[2A + 0] with A the target's bit
+                        const Src a = src_of(op.q0);
+                        so.kind = SW_DIAGL;
+                        so.k0 = (int8_t)a.reg;
+                        so.ta = (uint8_t)a.t;
+                        so.oa = (int8_t)a.opos;
+                        so.mat = recipe_for(RC_DIAG1, op.mat, 4);
                     } else if (op.is_x) {
-                        so.kind = SW_X;
-                        so.k0 = reg_of(op.q0);
-                        add_controls(so, op.controls);
+                        set_exchange(so, op.q0);
                     } else {
-                        so.kind = (op.is_real && op.controls == 0) ? SW_U1R : SW_U1;
-                        so.k0 = reg_of(op.q0);
-                        so.mat = recipe_for(0, op.mat, 4);
-                        add_controls(so, op.controls);
+                        so.kind = (so.rctrl | so.tctrl) ? SW_U1C : (op.is_real ? SW_U1R : SW_U1);
+                        so.k0 = (int8_t)reg_of(op.q0);
+                        so.mat = recipe_for(RC_COPY, op.mat, 4);
                     }
                     break;
                 case SC_OP_CNOT:
-                    so.kind = SW_X;
-                    so.k0 = reg_of(op.q1);
                     add_controls(so, 1ULL << op.q0);
+                    set_exchange(so, op.q1);
                     break;
-                case SC_OP_TWO:
+                case SC_OP_TWO: {
+                    // kernels take the more significant register bit first; the other order transposes the
+                    // matrix's qubit order
+                    const int r0 = reg_of(op.q0), r1 = reg_of(op.q1);
                     so.kind = op.is_real ? SW_U2R : SW_U2;
-                    so.k0 = reg_of(op.q0);
-                    so.k1 = reg_of(op.q1);
-                    so.mat = recipe_for(0, op.mat, 16);
+                    so.k0 = (int8_t)std::max(r0, r1);
+                    so.k1 = (int8_t)std::min(r0, r1);
+                    so.mat = recipe_for(r0 > r1 ? RC_COPY : RC_SWAP4, op.mat, 16);
                     break;
-                case SC_OP_DIAG2:
-                    so.kind = SW_DIAG;
-                    so.nd = 2;
-                    so.da = src_of(op.q0);
-                    so.db = src_of(op.q1);
-                    so.mat = recipe_for(0, op.mat, 4);
+                }
+                case SC_OP_DIAG2: {
+                    // factor m[2A + B]: when only the second qubit is a register bit the table is transposed so
+                    // that A is the register source
+                    Src a = src_of(op.q0), b = src_of(op.q1);
+                    int rk = RC_COPY;
+                    if (a.reg < 0 && b.reg >= 0) {
+                        std::swap(a, b);
+                        rk = RC_SWAPDIAG;
+                    }
+                    so.kind = SW_DIAGL;
+                    so.k0 = (int8_t)a.reg;
+                    so.ta = (uint8_t)a.t;
+                    so.oa = (int8_t)a.opos;
+                    so.k1 = (int8_t)b.reg;
+                    so.tb = (uint8_t)b.t;
+                    so.ob = (int8_t)b.opos;
+                    so.mat = recipe_for(rk, op.mat, 4);
                     break;
-                case SC_OP_SWAP:
+                }
+                case SC_OP_SWAP: {
+                    const int r0 = reg_of(op.q0), r1 = reg_of(op.q1);
                     so.kind = SW_SWAP;
-                    so.k0 = reg_of(op.q0);
-                    so.k1 = reg_of(op.q1);
+                    so.k0 = (int8_t)std::max(r0, r1);
+                    so.k1 = (int8_t)std::min(r0, r1);
                     break;
+                }
                 }
                 sops.push_back(so);
                 sreal.push_back(op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP || op.is_x);
             }
             // ---- fusion: split the stage's op list into consecutive segments, each either left as is or
-            // (when every op in it stays inside the register bits) replaced by one dense block on the
-            // segment's register bits -- 2x2 (8 FP64 ops per amplitude), 4x4 (16) or 16x16 (64) -- choosing
-            // the segmentation with the fewest FP64 operations (dynamic programming over segment ends).
+            // (when every op in it stays inside the register bits) replaced by one op: a diagonal table over
+            // the register index when every op is diagonal (4 VALU ops per amplitude), else a dense block on
+            // the segment's register bits -- 2x2 (8), 4x4 (16) or 16x16 (64), half that when real -- choosing
+            // the segmentation with the lowest cost (dynamic programming over segment ends).  A diagonal on
+            // two register bits has no kernel form of its own and always becomes a table.
             const size_t L = sops.size();
             std::vector<double> best(L + 1, 0.0);
             std::vector<int> from(L + 1, -1);  // segment start, or -1 for "op left as is"
             for (size_t e = 1; e <= L; e++) {
-                best[e] = best[e - 1] + op_cost(sops[e - 1]);
-                from[e] = -1;
+                const SweepOp &last = sops[e - 1];
+                const bool table = last.kind == SW_DIAGL && last.k1 >= 0;
+                best[e] = best[e - 1] + (table ? fused_cost(0, false, true) : op_cost(last));
+                from[e] = table ? (int)(e - 1) : -1;
                 uint32_t sup = 0;
-                bool real = true;
+                bool real = true, diag = true;
                 for (size_t s0 = e; s0-- > 0;) {
                     if (!register_internal(sops[s0])) break;
                     sup |= reg_support(sops[s0]);
                     real = real && sreal[s0];
-                    if (e - s0 < 2) continue;
-                    double c = best[s0] + fused_cost(popc(sup), real);
+                    diag = diag && is_diag_kind(sops[s0]);
+                    if (e - s0 < 2 || max_fuse_bits <= 0 || (!diag && popc(sup) > max_fuse_bits)) continue;
+                    double c = best[s0] + fused_cost(popc(sup), real, diag);
                     if (c < best[e] - 1e-9) {
                         best[e] = c;
                         from[e] = (int)s0;
@@ -854,30 +964,39 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 if (si < segs.size() && segs[si].first == i) {
                     size_t j = segs[si].second;
                     uint32_t sup = 0;
-                    bool real = true;
-                    for (size_t t = i; t < j; t++) sup |= reg_support(sops[t]), real = real && sreal[t];
-                    PoolRecipe r;
-                    r.kind = 2;
-                    r.mat = -1;
-                    SweepOp d;
-                    std::memset(&d, 0, sizeof(d));
-                    std::vector<int> bits;
-                    for (int b = kRegBits - 1; b >= 0; b--)
-                        if ((sup >> b) & 1u) bits.push_back(b);
-                    if (bits.size() == 1) {
-                        d.kind = real ? SW_U1R : SW_U1;
-                        d.k0 = bits[0];
-                    } else if (bits.size() == 2) {
-                        d.kind = real ? SW_U2R : SW_U2;
-                        d.k0 = bits[0];
-                        d.k1 = bits[1];
-                    } else {
-                        d.kind = real ? SW_DENSE4R : SW_DENSE4;
-                        bits = {3, 2, 1, 0};
-                        plan.fused_blocks++;
+                    bool real = true, diag = true;
+                    for (size_t t = i; t < j; t++) {
+                        sup |= reg_support(sops[t]);
+                        real = real && sreal[t];
+                        diag = diag && is_diag_kind(sops[t]);
                     }
-                    r.fused_bits = bits;
-                    r.count = 1 << (2 * bits.size());
+                    PoolRecipe r;
+                    r.mat = -1;
+                    SweepOp d = new_op();
+                    if (diag) {
+                        r.kind = RC_DIAG16;
+                        r.count = kRegs;
+                        d.kind = SW_DIAG16;
+                    } else {
+                        r.kind = RC_FUSED;
+                        std::vector<int> bits;
+                        for (int b = kRegBits - 1; b >= 0; b--)
+                            if ((sup >> b) & 1u) bits.push_back(b);
+                        if (bits.size() == 1) {
+                            d.kind = real ? SW_U1R : SW_U1;
+                            d.k0 = bits[0];
+                        } else if (bits.size() == 2) {
+                            d.kind = real ? SW_U2R : SW_U2;
+                            d.k0 = bits[0];
+                            d.k1 = bits[1];
+                        } else {
+                            d.kind = real ? SW_D4R : SW_D4;
+                            bits = {3, 2, 1, 0};
+                            plan.fused_blocks++;
+                        }
+                        r.fused_bits = bits;
+                        r.count = 1 << (2 * bits.size());
+                    }
                     for (size_t t = i; t < j; t++) {
                         r.fused_ops.push_back(sops[t]);
                         r.fused_srcs.push_back(sops[t].mat);
@@ -911,9 +1030,27 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         off += (uint64_t)plan.pool[r].count;
     }
     plan.pool_complex = off;
-    for (auto &so : plan.ops) plan.fp64_ops += op_cost(so);
+    for (auto &so : plan.ops) {
+        // kernel contract: register bit positions in range, two-bit kinds ordered, one register diagonal source
+        const bool two = so.kind == SW_U2 || so.kind == SW_U2R || so.kind == SW_SWAP;
+        const bool one = so.kind == SW_U1 || so.kind == SW_U1R || so.kind == SW_U1C || so.kind == SW_X ||
+                         so.kind == SW_XL;
+        bool ok = so.kind >= SW_U1 && so.kind <= SW_CX;
+        if (one) ok = ok && so.k0 >= 0 && so.k0 < kRegBits;
+        if (two) ok = ok && so.k1 >= 0 && so.k0 > so.k1 && so.k0 < kRegBits;
+        if (so.kind == SW_CX) ok = ok && so.k0 >= 0 && so.k0 < kRegBits && so.k1 >= 0 && so.k1 < kRegBits &&
+                                    so.k0 != so.k1;
+        if (so.kind == SW_DIAGL) ok = ok && so.k1 < 0 && so.k0 < kRegBits;
+        if (!ok) return fail(SC_E_STATE, "internal error: sweep plan produced an invalid register op");
+        plan.fp64_ops += op_cost(so);
+    }
     for (auto &so : plan.ops)
         if (so.mat >= 0) so.mat = (int32_t)recipe_off[so.mat];
+    // one record past the last op: the kernel reads the record after the current op ahead of time
+    SweepOp sentinel;
+    std::memset(&sentinel, 0, sizeof(sentinel));
+    sentinel.mat = -1;
+    plan.ops.push_back(sentinel);
     // fused recipes keep recipe indices in fused_srcs; record their offsets in the SweepOps' .mat for build_pool
     for (auto &r : plan.pool)
         for (size_t t = 0; t < r.fused_ops.size(); t++) r.fused_ops[t].mat = r.fused_srcs[t];
@@ -922,60 +1059,11 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
 
 namespace {
 
-bool host_bit_from(const BitSrc &s, uint32_t tid, uint64_t chunk_base) {
-    uint64_t om = ((uint64_t)s.omask_hi << 32) | s.omask_lo;
-    return ((tid & s.tmask) != 0u) || ((chunk_base & om) != 0ULL);
-}
-
-// Mirror of the device run_op (sc_sweep.hip) on one thread's 16 registers.
+// Mirror of the device run_op (sc_sweep.hip) on one thread's 16 registers; kernels only see ops whose
+// diagonals have at most one register source.
 void host_run_op(const SweepOp &op, const double *pool, double *a, uint32_t tid, uint64_t chunk_base) {
     const double *m = op.mat >= 0 ? pool + 2 * (int64_t)op.mat : nullptr;
-    const bool ctrl = (op.rctrl | op.tctrl) != 0u || op.octrl != 0ULL;
-    const bool tcond = ((tid & op.tctrl) == op.tctrl) && ((chunk_base & op.octrl) == op.octrl);
-    auto cm = [](const double *g, const double *x, double *y) {
-        y[0] = g[0] * x[0] - g[1] * x[1];
-        y[1] = g[0] * x[1] + g[1] * x[0];
-    };
-    switch (op.kind) {
-    case SW_U1:
-    case SW_U1R:
-    case SW_X: {
-        if (!tcond) return;
-        SweepOp r = op;
-        if (!ctrl) r.rctrl = 0;
-        host_reg_op(r, m, a);
-        return;
-    }
-    case SW_DIAG: {
-        bool ta = host_bit_from(op.da, tid, chunk_base), tb = host_bit_from(op.db, tid, chunk_base);
-        for (int r = 0; r < kRegs; r++) {
-            if (ctrl && (((uint32_t)r & op.rctrl) != op.rctrl || !tcond)) continue;
-            int ba = (((uint32_t)r & op.da.rmask) != 0u || ta) ? 1 : 0;
-            int bb = (((uint32_t)r & op.db.rmask) != 0u || tb) ? 1 : 0;
-            int l = op.nd == 1 ? ba : 2 * ba + bb;
-            double x[2] = {a[2 * r], a[2 * r + 1]}, y[2];
-            cm(m + 2 * l, x, y);
-            a[2 * r] = y[0], a[2 * r + 1] = y[1];
-        }
-        return;
-    }
-    case SW_DENSE4:
-    case SW_DENSE4R: {
-        double y[2 * kRegs];
-        for (int r = 0; r < kRegs; r++) {
-            double re = 0, im = 0;
-            for (int c = 0; c < kRegs; c++) {
-                double t[2];
-                cm(m + 2 * (r * kRegs + c), a + 2 * c, t);
-                re += t[0], im += t[1];
-            }
-            y[2 * r] = re, y[2 * r + 1] = im;
-        }
-        std::memcpy(a, y, sizeof(y));
-        return;
-    }
-    default: host_reg_op(op, m, a); return;
-    }
+    host_apply(op, m, a, tid, chunk_base);
 }
 
 }  // namespace
@@ -1049,19 +1137,46 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
     for (size_t r = 0; r < plan.pool.size(); r++) {
         const PoolRecipe &rc = plan.pool[r];
         double *dst = out + 2 * offs[r];
-        if (rc.kind == 0) {
-            std::memcpy(dst, mats + 2 * (int64_t)rc.mat, sizeof(double) * 2 * rc.count);
-        } else if (rc.kind == 1) {
-            const double *g = mats + 2 * (int64_t)rc.mat;
-            dst[0] = g[0], dst[1] = g[1], dst[2] = g[6], dst[3] = g[7];
+        const double *g = rc.mat >= 0 ? mats + 2 * (int64_t)rc.mat : nullptr;
+        auto put = [&](int k, int src) { dst[2 * k] = g[2 * src], dst[2 * k + 1] = g[2 * src + 1]; };
+        switch (rc.kind) {
+        case RC_COPY: std::memcpy(dst, g, sizeof(double) * 2 * rc.count); break;
+        case RC_DIAG1:  // (g00, g00, g11, g11) of a diagonal 2x2
+            put(0, 0), put(1, 0), put(2, 3), put(3, 3);
+            break;
+        case RC_SWAP4:  // 4x4 with the qubit order exchanged: M'[2b+a][2d+c] = M[2a+b][2c+d]
+            for (int row = 0; row < 4; row++)
+                for (int col = 0; col < 4; col++) {
+                    const int rs = ((row & 1) << 1) | (row >> 1), cs = ((col & 1) << 1) | (col >> 1);
+                    put(4 * row + col, 4 * rs + cs);
+                }
+            break;
+        case RC_SWAPDIAG:  // 4-entry diagonal with the qubit order exchanged
+            put(0, 0), put(1, 2), put(2, 1), put(3, 3);
+            break;
+        default: break;
         }
     }
-    // second pass: fused blocks from the plain entries.  Block index l has bit (m-1-j) <-> register bit
-    // fused_bits[j]; the block's ops touch no other register bit, so columns with those bits 0 suffice.
+    // second pass: diagonal tables and fused blocks from the plain entries.
     for (size_t r = 0; r < plan.pool.size(); r++) {
         const PoolRecipe &rc = plan.pool[r];
-        if (rc.kind != 2) continue;
         double *dst = out + 2 * offs[r];
+        if (rc.kind == RC_DIAG16) {
+            for (int reg = 0; reg < kRegs; reg++) {
+                double col[2 * kRegs] = {0};
+                col[2 * reg] = 1.0;
+                for (size_t t = 0; t < rc.fused_ops.size(); t++) {
+                    const SweepOp &so = rc.fused_ops[t];
+                    host_apply(so, so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr, col, 0u, 0ULL);
+                }
+                dst[2 * reg] = col[2 * reg];
+                dst[2 * reg + 1] = col[2 * reg + 1];
+            }
+            continue;
+        }
+        // Block index l has bit (m-1-j) <-> register bit fused_bits[j]; the block's ops touch no other
+        // register bit, so columns with those bits 0 suffice.
+        if (rc.kind != RC_FUSED) continue;
         const int m = (int)rc.fused_bits.size(), d = 1 << m;
         auto reg_of = [&](int l) {
             int reg = 0;
@@ -1074,8 +1189,7 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
             col[2 * reg_of(c)] = 1.0;
             for (size_t t = 0; t < rc.fused_ops.size(); t++) {
                 const SweepOp &so = rc.fused_ops[t];
-                const double *mm = so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr;
-                host_reg_op(so, mm, col);
+                host_apply(so, so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr, col, 0u, 0ULL);
             }
             for (int row = 0; row < d; row++) {
                 dst[2 * (row * d + c)] = col[2 * reg_of(row)];

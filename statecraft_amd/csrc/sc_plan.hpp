@@ -44,13 +44,22 @@ void block_matrix(const FusedBlock &b, const std::vector<PlanOp> &ops, const dou
 
 // ---- LDS-blocked sweeps (SC_EXEC_SWEEP) ------------------------------------
 // A matrix-pool entry: how to build `count` complex values from the caller's matrices.
+enum RecipeKind : int {
+    RC_COPY = 0,      // `count` entries copied from mat
+    RC_DIAG1 = 1,     // (g00, g00, g11, g11) of the diagonal 2x2 at mat
+    RC_FUSED = 2,     // dense block of fused_ops on fused_bits
+    RC_SWAP4 = 3,     // the 4x4 at mat with its two qubits exchanged
+    RC_SWAPDIAG = 4,  // the 4-entry diagonal at mat with its two qubits exchanged
+    RC_DIAG16 = 5,    // diagonal of the fused (diagonal) fused_ops over the 16 register indices
+};
+
 struct PoolRecipe {
-    int kind;        // 0 copy `count` entries from mat; 1 diag of a 2x2 (entries 0 and 3); 2 fused block of ops
-    int mat;         // source offset (kinds 0, 1)
+    int kind;        // RecipeKind
+    int mat;         // source offset in the caller's matrices (RC_COPY, RC_DIAG1, RC_SWAP4, RC_SWAPDIAG)
     int count;
-    std::vector<SweepOp> fused_ops;  // kind 2: register ops in order (their .mat index into `fused_mats`)
-    std::vector<int> fused_srcs;     // kind 2: per fused op, recipe index of its matrix (-1 none)
-    std::vector<int> fused_bits;     // kind 2: register bits of the block, most significant index bit first
+    std::vector<SweepOp> fused_ops;  // RC_FUSED / RC_DIAG16: register ops in order (.mat = recipe index)
+    std::vector<int> fused_srcs;     // RC_FUSED / RC_DIAG16: per fused op, recipe index of its matrix (-1 none)
+    std::vector<int> fused_bits;     // RC_FUSED: register bits of the block, most significant index bit first
 };
 
 struct SweepPlan {
@@ -63,7 +72,10 @@ struct SweepPlan {
     double fp64_ops = 0.0;  // estimated FP64 VALU operations per amplitude over all passes
 };
 
-int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan);
+// max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
+// register bits needs).
+int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
+                int max_fuse_bits = kRegBits);
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS

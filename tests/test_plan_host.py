@@ -6,10 +6,10 @@ import pytest
 import oracle
 from circuits import random_circuit, random_state
 from statecraft_amd import capi, workloads
-from statecraft_amd.circuit import lower
+from statecraft_amd.circuit import Hadamard, PauliZ, QuantumCircuit, lower, rotation_y, rotation_z, zz_phase
 
 
-@pytest.mark.parametrize("n,low", [(12, 0), (12, 4), (13, 8), (14, 2), (15, 4), (16, 6)])
+@pytest.mark.parametrize("n,low", [(12, 0), (12, 4), (13, 8), (14, 1), (14, 2), (15, 4), (16, 0), (16, 6)])
 def test_plan_matches_oracle(n, low):
     rng = np.random.default_rng(100 * n + low)
     ops, mats = random_circuit(n, 250, rng)
@@ -37,6 +37,40 @@ def test_many_control_ops_plan():
     ops, mats = random_circuit(n, 80, rng, kinds=[capi.OP_MCU, capi.OP_CNOT, capi.OP_SINGLE])
     init = random_state(n, rng)
     _, out = capi.plan_sweeps_host(n, ops, mats, 4, init)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+
+
+def _diag_real_circuit(n: int, rng) -> QuantumCircuit:
+    """Layers that fuse into diagonal tables (Rz, ZZ, controlled phases, multi-controlled Z) and real blocks
+    (Ry, H, real two-qubit unitaries), with CNOT / Toffoli on varying qubits."""
+    c = QuantumCircuit(n)
+    for layer in range(5):
+        for q in range(n):
+            c = c.append(rotation_y(float(rng.uniform(0, 6.3))), q)
+            c = c.append(rotation_z(float(rng.uniform(0, 6.3))), q)
+        for q in range(n):
+            c = c.append_diagonal_two_qubit(zz_phase(float(rng.uniform(0, 3.1))), q, (q + 1 + layer) % n)
+        for _ in range(n // 2):
+            a, b, t = (int(x) for x in rng.choice(n, size=3, replace=False))
+            c = c.append_controlled_phase(float(rng.uniform(0, 6.3)), a, b)
+            c = c.append_toffoli(a, b, t)
+            c = c.append_multi_control(PauliZ(), t, a, b)
+            c = c.append(Hadamard(), t)
+        for q in range(0, n - 1, 2):
+            ry = np.array([[np.cos(0.3 * q), -np.sin(0.3 * q)], [np.sin(0.3 * q), np.cos(0.3 * q)]])
+            c = c.append_two_qubit_unitary(list(np.kron(ry, ry.T).astype(np.complex128).reshape(-1)),
+                                           (q + layer) % n, (q + 1 + 2 * layer) % n)
+        c = c.append_cnot(layer % n, (layer + 5) % n)
+    return c
+
+
+@pytest.mark.parametrize("n,low", [(13, 4), (15, 0), (16, 8)])
+def test_diagonal_tables_and_real_blocks_match_oracle(n, low):
+    rng = np.random.default_rng(17 + n)
+    ops, mats = lower(_diag_real_circuit(n, rng))
+    init = random_state(n, rng)
+    stats, out = capi.plan_sweeps_host(n, ops, mats, low, init)
+    assert stats["passes"] >= 1
     assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
 
 
