@@ -44,6 +44,7 @@ def make_context(args) -> capi.Context:
     ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
     ctx.set_option(capi.OPT_REMAP, 0 if args.no_remap else 1)
     ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
+    ctx.set_option(capi.OPT_JIT, args.jit)
     return ctx
 
 
@@ -142,6 +143,7 @@ def bench_c2(args):
             "sweeps_per_step": sweeps, "kernel_launches_per_step": plan["launches"],
             "fused_16x16_blocks": plan["fused_blocks"], "parallelism": "1 GPU",
             "low_qubits": args.low_qubits, "remap": not args.no_remap, "blocks_per_cu": args.blocks_per_cu,
+            "jit": bool(args.jit),
         },
         "roofline": {
             "bound": "hbm",
@@ -150,14 +152,15 @@ def bench_c2(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": {"sweep": "k_sweep", "fused": "k_dense", "direct": "k_single"}[args.mode],
+            "kernel": {"sweep": "sc_pass_*" if args.jit else "k_sweep", "fused": "k_dense",
+                       "direct": "k_single"}[args.mode],
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
             "launches": launches,
             "copy_kernel_gbs": copy_gbs,
             "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None,
             "fp64_ops_per_amp_per_step": capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None,
-                                                               not args.no_remap)[0]["fp64_ops_per_amp"]
+                                                               not args.no_remap, bool(args.jit))[0]["fp64_ops_per_amp"]
             if args.mode == "sweep" else None,
         },
         "amp_updates_per_step": credit,
@@ -382,6 +385,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-remap", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--jit", type=int, default=0, choices=[0, 1, 2],
+                    help="1: sweep passes as runtime-compiled kernels (compiled during warmup)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",

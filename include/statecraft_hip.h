@@ -99,6 +99,10 @@ typedef struct sc_op {
 #define SC_OPT_TEST_DRIFT 5     /* test hook: adds value*1e-12 to psi[0] at every adjoint checkpoint, default 0 */
 #define SC_OPT_REMAP 6          /* 1: sweep passes relabel qubits in place to keep coalesced low positions useful, default 1 */
 #define SC_OPT_SWEEP_BLOCKS_PER_CU 7 /* resident sweep workgroups per CU (0: as many as fit), default 0 */
+#define SC_OPT_JIT 8            /* 1: sweep passes run as kernels compiled at run time (hipRTC) with the pass
+                                   constant-folded; compiled once per circuit structure and cached; 2: the same
+                                   with the next chunk's HBM loads issued before the current chunk's stages;
+                                   default 0 */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
@@ -210,12 +214,20 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 /* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks);
 /* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
- * (n >= 12) and reports its passes, stages and fused 16x16 blocks.  When state_inout is non-NULL (2^n
+ * (n >= 12) as a context with the given low-qubit, remap and jit options would, and reports its passes,
+ * stages, fused 16x16 blocks and FP64 operations per amplitude.  When state_inout is non-NULL (2^n
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, uint64_t *passes, uint64_t *stages,
                         uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout);
+/* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
+ * SC_OPT_JIT path) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the
+ * compile wall time and the code object size; when dump_dir is non-NULL each module's source (.hip) and
+ * code object (.co) are written there. */
+int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int threads, uint64_t *kernels,
+                        double *seconds, uint64_t *code_bytes, const char *dump_dir);
 
 #ifdef __cplusplus
 }

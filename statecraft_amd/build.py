@@ -57,14 +57,33 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def _embed_ops_header() -> Path:
+    """The runtime-compiled pass kernels include sc_ops.hpp; its text is compiled into the library."""
+    text = (CSRC / "sc_ops.hpp").read_text()
+    delim = "SCOPS"
+    if f"){delim}\"" in text:
+        raise RuntimeError("sc_ops.hpp contains the raw-string delimiter")
+    src = OBJDIR / "sc_ops_text.cpp"
+    body = f'namespace sc {{\nextern const char kOpsHeaderText[] = R"{delim}({text}){delim}";\n}}\n'
+    if not src.exists() or src.read_text() != body:
+        src.write_text(body)
+    obj = OBJDIR / "sc_ops_text.o"
+    cmd = [os.environ.get("CXX", "g++"), "-O1", "-fPIC", "-c", str(src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
 def build(jobs: int | None = None, verbose: bool = True) -> Path:
     OBJDIR.mkdir(parents=True, exist_ok=True)
     sources = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, sources))
+    objs.append(_embed_ops_header())
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lhiprtc"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "sc_internal.hpp"
+#include "sc_jit.hpp"
 #include "sc_plan.hpp"
 
 namespace sc {
@@ -190,7 +191,8 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 // mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
 int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
     SweepPlan plan;
-    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan, std::min(ctx->max_fused, kRegBits));
+    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan, std::min(ctx->max_fused, kRegBits),
+                         ctx->jit ? kCompiledPassCost : kInterpretedCost);
     if (rc) return rc;
     if (plan.passes.empty()) return SC_OK;
     const size_t pass_bytes = plan.passes.size() * sizeof(SweepPass);
@@ -213,13 +215,16 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     const SweepOp *dops = (const SweepOp *)((char *)d.p + pass_bytes);
     const double *dpool = (const double *)((char *)d.p + pass_bytes + ops_bytes);
     const size_t stride = nelem_pool > 1 ? plan.pool_complex : 0;
+    std::vector<hipFunction_t> fns;
+    if (ctx->jit && (rc = jit_pass_kernels(ctx, plan, fns))) return rc;
     // keep each launch's grid under ~2^20 blocks
     const int log_chunks = st->n - kChunkBits;
     int per_launch = std::max(1, (int)std::min<uint64_t>((uint64_t)st->batch, (1ULL << 20) >> log_chunks));
     for (size_t p = 0; p < plan.passes.size(); p++) {
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             int cnt = std::min(per_launch, st->batch - e0);
-            rc = launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride);
+            rc = fns.empty() ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
+                             : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride);
             if (rc) return rc;
             ctx->last_launches++;
         }
@@ -354,6 +359,10 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value < 0 || value > 8) return fail(SC_E_ARG, "sweep blocks per CU must be 0..8");
         ctx->sweep_blocks_per_cu = (int)value;
         return SC_OK;
+    case SC_OPT_JIT:
+        if (value < 0 || value > 2) return fail(SC_E_ARG, "jit must be 0, 1 or 2");
+        ctx->jit = (int)value;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -369,6 +378,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_TEST_DRIFT: *value = ctx->test_drift; return SC_OK;
     case SC_OPT_REMAP: *value = ctx->remap; return SC_OK;
     case SC_OPT_SWEEP_BLOCKS_PER_CU: *value = ctx->sweep_blocks_per_cu; return SC_OK;
+    case SC_OPT_JIT: *value = ctx->jit; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -682,7 +692,7 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 }
 
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, uint64_t *passes, uint64_t *stages,
                         uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
@@ -693,7 +703,9 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
     if (rc) return rc;
     SweepPlan plan;
-    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan))) return rc;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits,
+                          jit ? kCompiledPassCost : kInterpretedCost)))
+        return rc;
     uint64_t ns = 0;
     for (const auto &p : plan.passes) ns += (uint64_t)p.nstages;
     if (passes) *passes = plan.passes.size();
@@ -706,6 +718,25 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
         execute_plan_host(num_qubits, plan, pool.data(), state_inout);
     }
     return SC_OK;
+}
+
+int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int threads, uint64_t *kernels,
+                        double *seconds, uint64_t *code_bytes, const char *dump_dir) {
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
+    if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+    if (threads < 1 || threads > 64) return fail(SC_E_ARG, "threads must be 1..64");
+    if (jit != 1 && jit != 2) return fail(SC_E_ARG, "jit must be 1 or 2");
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
+    if (rc) return rc;
+    SweepPlan plan;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost))) return rc;
+    if (kernels) *kernels = plan.passes.size();
+    return jit_compile_host(plan, jit, threads, seconds, code_bytes, dump_dir);
 }
 
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks) {

@@ -61,6 +61,7 @@ struct Context {
     int low_qubits = 4;
     int remap = 1;
     int sweep_blocks_per_cu = 0;
+    int jit = 0;
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)

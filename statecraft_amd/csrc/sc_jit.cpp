@@ -1,0 +1,393 @@
+// sc_jit.cpp -- sweep passes compiled at run time (hipRTC).
+//
+// The compiled sweep kernel (sc_sweep.hip) interprets a pass: it reads every op record, branches on its
+// kind and bit positions, and holds the code of every op variant.  At 30 qubits and above that dispatch
+// costs about as much as an op's arithmetic.  Here each pass of a plan becomes its own kernel whose op
+// sequence, bit positions, LDS slots and HBM offsets are compile-time constants: the ops inline to
+// straight-line FP64 code, exchanges (X, CNOT, SWAP) become register renames, and the matrices stay in
+// the pass pool, so one kernel serves every parameter value of a circuit structure.
+//
+// Kernels are cached per device by their source text for the life of the process; a plan whose passes
+// are all cached launches without compiling.  The op library (sc_ops.hpp) is compiled into the library
+// as text (kOpsHeaderText) and handed to hipRTC as a header.
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "sc_jit.hpp"
+#include "sc_plan.hpp"
+
+namespace sc {
+
+extern const char kOpsHeaderText[];
+
+namespace {
+
+std::string hex64(uint64_t v) {
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxULL", (unsigned long long)v);
+    return b;
+}
+std::string hex32(uint32_t v) {
+    char b[24];
+    std::snprintf(b, sizeof b, "0x%xu", v);
+    return b;
+}
+
+// Per-lane value sum_i (bit i of `var` ? tab[i] : 0), combined with `op` ("+" or "^"), as an expression.
+template <typename T>
+std::string lane_expr(const char *var, const T *tab, int bits, const char *op, bool wide) {
+    std::string e;
+    for (int i = 0; i < bits; i++) {
+        if (tab[i] == 0) continue;
+        if (!e.empty()) e += std::string(" ") + op + " ";
+        e += std::string("((") + var + " & " + std::to_string(1u << i) + "u) ? " +
+             (wide ? hex64((uint64_t)tab[i]) : hex32((uint32_t)tab[i])) + " : " + (wide ? "0ULL" : "0u") + ")";
+    }
+    if (e.empty()) e = wide ? "0ULL" : "0u";
+    return e;
+}
+
+// Expression depositing the bits of `var` (a chunk index) at the set positions of `mask`, one shift per
+// run of consecutive positions.
+std::string deposit_expr(const char *var, uint64_t mask) {
+    std::string e;
+    int j = 0;  // next chunk-index bit
+    for (int pos = 0; pos < 64;) {
+        if (!((mask >> pos) & 1ULL)) {
+            pos++;
+            continue;
+        }
+        int len = 0;
+        while (pos + len < 64 && ((mask >> (pos + len)) & 1ULL)) len++;
+        const uint64_t lm = len >= 64 ? ~0ULL : ((1ULL << len) - 1ULL);
+        if (!e.empty()) e += " | ";
+        e += "(((" + std::string(var) + " >> " + std::to_string(j) + ") & " + hex64(lm) + ") << " +
+             std::to_string(pos) + ")";
+        j += len;
+        pos += len;
+    }
+    if (e.empty()) e = "0ULL";
+    return e;
+}
+
+// The call that applies op o from register set S to D: the per-kind entry point of sc_ops.hpp that run_op
+// would dispatch to, with every field a literal.
+std::string op_call(const SweepOp &o, const char *S, const char *D) {
+    std::ostringstream s;
+    const std::string sd = std::string(S) + ", " + D;
+    const std::string m = "emats + " + std::to_string(o.mat);
+    const int k0 = o.k0, k1 = o.k1;
+    switch (o.kind) {
+    case SW_U1: s << "op_u1m<" << k0 << ">(" << sd << ", " << m << ");"; break;
+    case SW_U1R: s << "op_u1r<" << k0 << ">(" << sd << ", " << m << ");"; break;
+    case SW_U1C:
+        s << "op_u1c<" << k0 << ">(" << sd << ", " << m << ", " << (unsigned)o.rctrl << "u, " << (unsigned)o.tctrl
+          << "u, tid);";
+        break;
+    case SW_X: s << "op_x<" << k0 << ", false>(" << sd << ", " << (unsigned)o.rctrl << "u, true);"; break;
+    case SW_XL:
+        s << "op_xl<" << k0 << ">(" << sd << ", " << (unsigned)o.rctrl << "u, " << (unsigned)o.tctrl << "u, tid);";
+        break;
+    case SW_U2: s << "op_u2<" << k0 << ", " << k1 << ", false>(" << sd << ", " << m << ");"; break;
+    case SW_U2R: s << "op_u2<" << k0 << ", " << k1 << ", true>(" << sd << ", " << m << ");"; break;
+    case SW_SWAP: s << "op_swap<" << k0 << ", " << k1 << ">(" << sd << ");"; break;
+    case SW_D4: s << "op_d4<false>(" << sd << ", " << m << ");"; break;
+    case SW_D4R: s << "op_d4<true>(" << sd << ", " << m << ");"; break;
+    case SW_DIAG16: s << "op_diag16(" << sd << ", " << m << ");"; break;
+    case SW_DIAGL:
+        s << "op_diagl_sel<" << (k0 < 0 ? kRegBits : k0) << ">(" << sd << ", " << m << ", " << (unsigned)o.rctrl
+          << "u, " << (unsigned)o.tctrl << "u, " << (unsigned)o.ta << "u, " << (unsigned)o.tb << "u, " << (int)o.oa
+          << ", " << (int)o.ob << ", tid, cb);";
+        break;
+    case SW_CX: s << "op_cx<" << k0 << ", " << k1 << ">(" << sd << ");"; break;
+    default: s << "copy_all(" << sd << ");"; break;
+    }
+    if (o.octrl == 0) return s.str();
+    // outer (chunk-fixed) controls: identity on chunks where they are not all set
+    return "if ((cb & " + hex64(o.octrl) + ") == " + hex64(o.octrl) + ") { " + s.str() + " } else { copy_all(" + sd +
+           "); }";
+}
+
+// Source of one pass kernel; "@K@" stands for the kernel name.  The structure is that of k_sweep
+// (sc_sweep.hip) with every table read replaced by its value.  With `prefetch` the HBM loads of a
+// workgroup's next chunk are issued before the stages of the current one, into a third register set, so
+// that they arrive while the chunk is computed; otherwise they are issued after the stages, as in k_sweep.
+std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch) {
+    std::ostringstream s;
+    s << "extern \"C\" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
+         "@K@(sc::v2d *__restrict__ amp, int n, int log_chunks, uint64_t total_chunks,\n"
+         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin) {\n"
+         "  using namespace sc;\n"
+         "  __shared__ v2d lds[kChunk];\n"
+         "  CV2 *mats = (CV2 *)mats_arg;\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         "  uint64_t c = blockIdx.x;\n"
+         "  if (c >= total_chunks) return;\n"
+         "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
+         "  auto cbase = [](uint64_t x) -> uint64_t { return "
+      << deposit_expr("x", ps.outer_mask) << "; };\n";
+    s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, kThreadBits, "+", true) << ";\n";
+    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, kThreadBits, "^", false) << ";\n";
+    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, kThreadBits, "^", false) << ";\n";
+    auto regs_load = [&](const char *dst, const char *srcp) {
+        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << ps.glob_reg[r] << "];\n";
+    };
+    auto regs_store = [&](const char *dstp, const char *src) {
+        for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << ps.glob_reg[r] << "] = " << src << "[" << r << "];\n";
+    };
+    auto lds_write = [&](const char *base, const uint32_t *off, const char *src) {
+        for (int r = 0; r < kRegs; r++) s << "    lds[" << base << " ^ " << off[r] << "u] = " << src << "[" << r << "];\n";
+    };
+    auto lds_read = [&](const char *dst, const char *base, const uint32_t *off) {
+        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = lds[" << base << " ^ " << off[r] << "u];\n";
+    };
+    s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
+    s << "  uint64_t cb = cbase(c & chunk_mask);\n"
+         "  {\n"
+         "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gthread);\n";
+    regs_load("p", "src");
+    lds_write("sthread", ps.io_reg_slot, "p");
+    s << "  }\n"
+         "  __syncthreads();\n"
+         "  for (;;) {\n"
+         "    const uint64_t elem = (uint64_t)elem_begin + (c >> log_chunks);\n"
+         "    CV2 *emats = mats + elem * mat_stride;\n"
+         "    const uint64_t next = c + gridDim.x;\n"
+         "    const uint64_t ncb = next < total_chunks ? cbase(next & chunk_mask) : 0ULL;\n";
+    auto next_loads = [&](const char *dst) {
+        s << "    if (next < total_chunks) {\n"
+             "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gthread);\n";
+        regs_load(dst, "src");
+        s << "    }\n";
+    };
+    if (prefetch) next_loads("q");
+    for (int st = 0; st < ps.nstages; st++) {
+        const SweepStage &stg = ps.stages[st];
+        s << "    {\n"
+             "    const uint32_t t = opaque(tid);\n"
+             "    const uint32_t ts = "
+          << lane_expr("t", stg.thread_slot, kThreadBits, "^", false) << ";\n";
+        lds_read("a", "ts", stg.lds_off);
+        bool in_a = true;
+        for (int o = stg.op_begin; o < stg.op_end; o++) {
+            s << "    " << op_call(plan.ops[o], in_a ? "a" : "p", in_a ? "p" : "a") << "\n";
+            in_a = !in_a;
+        }
+        s << "    const uint32_t tw = opaque(ts);\n";
+        lds_write("tw", stg.lds_off, in_a ? "a" : "p");
+        s << "    __syncthreads();\n"
+             "    }\n";
+    }
+    s << "    {\n"
+         "    const uint32_t sl = opaque(sout);\n";
+    lds_read("a", "sl", ps.st_reg_slot);
+    s << "    }\n";
+    if (!prefetch) next_loads("p");
+    s << "    {\n"
+         "    v2d *dst = amp + (elem << n) + cb + opaque(gthread);\n";
+    regs_store("dst", "a");
+    s << "    }\n"
+         "    if (next >= total_chunks) break;\n"
+         "    c = next;\n"
+         "    cb = ncb;\n"
+         "    __syncthreads();\n"
+         "    {\n"
+         "    const uint32_t sl = opaque(sthread);\n";
+    lds_write("sl", ps.io_reg_slot, prefetch ? "q" : "p");
+    s << "    }\n"
+         "    __syncthreads();\n"
+         "  }\n"
+         "}\n";
+    return s.str();
+}
+
+std::string replace_name(const std::string &body, const std::string &name) {
+    std::string out = body;
+    const size_t at = out.find("@K@");
+    out.replace(at, 3, name);
+    return out;
+}
+
+const char *const kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                   "-Xclang", "-packed-fp32-ops"};
+
+// Compiles one module; code receives the code object.
+int compile_module(const std::string &src, std::vector<char> &code) {
+    hiprtcProgram prog = nullptr;
+    const char *headers[] = {kOpsHeaderText};
+    const char *names[] = {"sc_ops.hpp"};
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "sc_passes.hip", 1, headers, names);
+    if (r != HIPRTC_SUCCESS) return fail(SC_E_HIP, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
+    r = hiprtcCompileProgram(prog, (int)(sizeof kRtcOptions / sizeof kRtcOptions[0]), kRtcOptions);
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+            log.resize(n);
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+        }
+        (void)hiprtcDestroyProgram(&prog);
+        return fail(SC_E_HIP, std::string("sweep pass compilation failed: ") + hiprtcGetErrorString(r) + "\n" +
+                                  log.substr(0, 4000));
+    }
+    size_t n = 0;
+    r = hiprtcGetCodeSize(prog, &n);
+    if (r == HIPRTC_SUCCESS) {
+        code.resize(n);
+        r = hiprtcGetCode(prog, code.data());
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    if (r != HIPRTC_SUCCESS) return fail(SC_E_HIP, std::string("hiprtcGetCode: ") + hiprtcGetErrorString(r));
+    return SC_OK;
+}
+
+// Compiles the given kernel bodies, split over up to `threads` modules compiled in parallel.  Kernel i of
+// the result is named sc_pass_<i>; modules[j] holds kernels [first[j], first[j+1]).
+int compile_bodies(const std::vector<std::string> &bodies, int threads, std::vector<std::vector<char>> &codes,
+                   std::vector<size_t> &first) {
+    const size_t nb = bodies.size();
+    const size_t groups = std::max<size_t>(1, std::min<size_t>((size_t)std::max(threads, 1), nb));
+    first.assign(groups + 1, 0);
+    for (size_t g = 0; g <= groups; g++) first[g] = nb * g / groups;
+    std::vector<std::string> srcs(groups);
+    for (size_t g = 0; g < groups; g++) {
+        std::string m = "#include \"sc_ops.hpp\"\n";
+        for (size_t i = first[g]; i < first[g + 1]; i++) m += replace_name(bodies[i], "sc_pass_" + std::to_string(i));
+        srcs[g] = std::move(m);
+    }
+    codes.assign(groups, {});
+    std::vector<int> rcs(groups, SC_OK);
+    std::vector<std::string> errs(groups);
+    auto work = [&](size_t g) {
+        rcs[g] = compile_module(srcs[g], codes[g]);
+        if (rcs[g]) errs[g] = sc_last_error();
+    };
+    if (groups == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t g = 0; g < groups; g++) pool.emplace_back(work, g);
+        for (auto &t : pool) t.join();
+    }
+    for (size_t g = 0; g < groups; g++)
+        if (rcs[g]) return fail(rcs[g], errs[g]);
+    return SC_OK;
+}
+
+struct JitCache {
+    std::mutex mu;
+    std::unordered_map<std::string, hipFunction_t> fns;  // key: device id + kernel body
+    std::vector<hipModule_t> modules;                    // loaded for the life of the process
+};
+JitCache &cache() {
+    static JitCache *c = new JitCache();  // never destroyed: kernels may run until the process ends
+    return *c;
+}
+
+int compile_threads() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(8u, hc ? hc : 1u));
+}
+
+}  // namespace
+
+std::string jit_pass_source(const SweepPlan &plan, size_t pass, int variant) {
+    return pass_source(plan, plan.passes[pass], variant == 2);
+}
+
+int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
+    JitCache &jc = cache();
+    std::lock_guard<std::mutex> lock(jc.mu);
+    const std::string dev = std::to_string(ctx->device) + "\n";
+    std::vector<std::string> keys(plan.passes.size());
+    std::vector<std::string> missing;
+    std::unordered_map<std::string, size_t> missing_at;
+    for (size_t p = 0; p < plan.passes.size(); p++) {
+        std::string body = pass_source(plan, plan.passes[p], ctx->jit == 2);
+        keys[p] = dev + body;
+        if (!jc.fns.count(keys[p]) && !missing_at.count(keys[p])) {
+            missing_at[keys[p]] = missing.size();
+            missing.push_back(std::move(body));
+        }
+    }
+    if (!missing.empty()) {
+        if (jc.fns.size() + missing.size() > kJitMaxKernels)
+            return fail(SC_E_UNSUPPORTED, "runtime-compiled sweep kernel cache is full (" +
+                                              std::to_string(kJitMaxKernels) + " kernels)");
+        std::vector<std::vector<char>> codes;
+        std::vector<size_t> first;
+        int rc = compile_bodies(missing, compile_threads(), codes, first);
+        if (rc) return rc;
+        for (size_t g = 0; g + 1 < first.size(); g++) {
+            hipModule_t mod = nullptr;
+            SC_HIP_CHECK(hipModuleLoadData(&mod, codes[g].data()));
+            jc.modules.push_back(mod);
+            for (size_t i = first[g]; i < first[g + 1]; i++) {
+                hipFunction_t f = nullptr;
+                SC_HIP_CHECK(hipModuleGetFunction(&f, mod, ("sc_pass_" + std::to_string(i)).c_str()));
+                jc.fns[dev + missing[i]] = f;
+            }
+        }
+    }
+    out.resize(plan.passes.size());
+    for (size_t p = 0; p < plan.passes.size(); p++) out[p] = jc.fns.at(keys[p]);
+    return SC_OK;
+}
+
+int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *seconds, uint64_t *code_bytes,
+                     const char *dump_dir) {
+    std::vector<std::string> bodies;
+    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(pass_source(plan, plan.passes[p], variant == 2));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::vector<char>> codes;
+    std::vector<size_t> first;
+    int rc = compile_bodies(bodies, threads, codes, first);
+    if (rc) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    uint64_t total = 0;
+    for (auto &c : codes) total += c.size();
+    if (code_bytes) *code_bytes = total;
+    if (dump_dir && *dump_dir) {
+        for (size_t g = 0; g < codes.size(); g++) {
+            const std::string base = std::string(dump_dir) + "/sc_passes_" + std::to_string(g);
+            std::string src = "#include \"sc_ops.hpp\"\n";
+            for (size_t i = first[g]; i < first[g + 1]; i++) src += replace_name(bodies[i], "sc_pass_" + std::to_string(i));
+            FILE *f = std::fopen((base + ".hip").c_str(), "w");
+            if (!f) return fail(SC_E_ARG, "cannot write " + base + ".hip");
+            std::fwrite(src.data(), 1, src.size(), f);
+            std::fclose(f);
+            f = std::fopen((base + ".co").c_str(), "wb");
+            if (!f) return fail(SC_E_ARG, "cannot write " + base + ".co");
+            std::fwrite(codes[g].data(), 1, codes[g].size(), f);
+            std::fclose(f);
+        }
+    }
+    return SC_OK;
+}
+
+int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
+                     int elem_count, size_t mat_stride) {
+    int log_chunks = st->n - kChunkBits;
+    uint64_t total = (uint64_t)elem_count << log_chunks;
+    const uint64_t grid = std::min<uint64_t>(total, (uint64_t)sweep_grid_size(ctx));
+    v2d *amp = reinterpret_cast<v2d *>(st->amp);
+    const v2d *mats = reinterpret_cast<const v2d *>(mats_dev);
+    int n = st->n;
+    uint64_t stride = (uint64_t)mat_stride;
+    void *args[] = {&amp, &n, &log_chunks, &total, &mats, &stride, &elem_begin};
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
+    SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kThreads, 1, 1, 0, ctx->stream, args, nullptr));
+    return SC_OK;
+}
+
+}  // namespace sc

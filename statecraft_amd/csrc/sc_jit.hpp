@@ -1,0 +1,30 @@
+// sc_jit.hpp -- sweep passes compiled at run time (sc_jit.cpp).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sc_internal.hpp"
+
+namespace sc {
+
+struct SweepPlan;
+
+// Most pass kernels kept loaded per process (each holds its pass's straight-line code, ~20-200 KB).
+constexpr size_t kJitMaxKernels = 4096;
+
+// Kernel source of one pass of a plan (the kernel is named "@K@"); variant = the SC_OPT_JIT value.
+std::string jit_pass_source(const SweepPlan &plan, size_t pass, int variant);
+// One kernel per pass of the plan, compiled on first use and cached per device by source text.
+int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out);
+// Compiles every pass of a plan without a device (timing, and the code objects for inspection when
+// dump_dir is set).
+int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *seconds, uint64_t *code_bytes,
+                     const char *dump_dir);
+int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
+                     int elem_count, size_t mat_stride);
+// Workgroups of a sweep launch that are resident at once (occupancy x CUs, capped by the context option).
+int sweep_grid_size(Context *ctx);
+
+}  // namespace sc

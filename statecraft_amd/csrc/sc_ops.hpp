@@ -73,7 +73,9 @@ __device__ __forceinline__ v2d cmul(v2d m, v2d x) {
 }
 __device__ __forceinline__ v2d cmad(v2d m, v2d x, v2d acc) {
     acc.x = fma(m.x, x.x, acc.x);
-    acc.x = fma(-m.y, x.y, acc.x);
+    // contracted to fma(-m.y, x.y, acc.x); an explicit negation of every matrix entry makes instruction
+    // selection preparation quadratic in the size of straight-line pass kernels
+    acc.x = acc.x - m.y * x.y;
     acc.y = fma(m.x, x.y, acc.y);
     acc.y = fma(m.y, x.x, acc.y);
     return acc;
@@ -269,6 +271,44 @@ __device__ __forceinline__ void run_cx(int k0, int k1, const Regs &s, Regs &d) {
     }
 }
 
+// ---- one entry point per op kind with the register bits as template arguments: run_op dispatches to these,
+// and the runtime-compiled pass kernels call them directly with every argument a constant.
+template <int K>
+__device__ __forceinline__ void op_u1m(const Regs &s, Regs &d, CV2 *m) {
+    op_u1<K, false>(s, d, m[0], m[1], m[2], m[3], 0u);
+}
+
+// 2x2 on bit K where the register controls rctrl and the thread controls tctrl are set (other lanes: identity)
+template <int K>
+__device__ __forceinline__ void op_u1c(const Regs &s, Regs &d, CV2 *m, uint32_t rctrl, uint32_t tctrl,
+                                       uint32_t tid) {
+    const bool c = (tid & tctrl) == tctrl;
+    const v2d one = make_v2d(1.0, 0.0), zero = make_v2d(0.0, 0.0);
+    op_u1<K, true>(s, d, sel(c, m[0], one), sel(c, m[1], zero), sel(c, m[2], zero), sel(c, m[3], one), rctrl);
+}
+
+template <int K>
+__device__ __forceinline__ void op_xl(const Regs &s, Regs &d, uint32_t rctrl, uint32_t tctrl, uint32_t tid) {
+    op_x<K, true>(s, d, rctrl, (tid & tctrl) == tctrl);
+}
+
+// SW_DIAGL with A on register bit K (K == kRegBits: A is the thread / outer bit ta / oa).  The thread and
+// chunk-fixed bits are resolved once per lane: lo / hi are the factors for A = 0 / 1.
+template <int K>
+__device__ __forceinline__ void op_diagl_sel(const Regs &s, Regs &d, CV2 *m, uint32_t rctrl, uint32_t tctrl,
+                                             uint32_t ta, uint32_t tb, int oa, int ob, uint32_t tid,
+                                             uint64_t cb) {
+    const bool bb = ((tid & tb) != 0u) || (ob >= 0 && ((cb >> ob) & 1ULL));
+    v2d lo = sel(bb, m[1], m[0]), hi = sel(bb, m[3], m[2]);
+    if (K == kRegBits) {
+        const bool ba = ((tid & ta) != 0u) || (oa >= 0 && ((cb >> oa) & 1ULL));
+        lo = sel(ba, hi, lo);
+    }
+    const bool c = (tid & tctrl) == tctrl;
+    const v2d one = make_v2d(1.0, 0.0);
+    op_diagl<K>(s, d, sel(c, lo, one), sel(c, hi, one), rctrl);
+}
+
 __device__ __forceinline__ void run_op(const Regs &s, Regs &d, const Op &op, CV2 *mats, uint32_t tid, uint64_t cb) {
     if ((cb & op.octrl) != op.octrl) {  // outer controls unset: identity on this chunk
         copy_all(s, d);
@@ -277,16 +317,14 @@ __device__ __forceinline__ void run_op(const Regs &s, Regs &d, const Op &op, CV2
     CV2 *m = mats + op.mat;
     switch (op.kind) {
     case SW_CX: run_cx(op.k0, op.k1, s, d); break;
-    case SW_U1: {
-        const v2d g00 = m[0], g01 = m[1], g10 = m[2], g11 = m[3];
+    case SW_U1:
         switch (op.k0) {
-        case 0: op_u1<0, false>(s, d, g00, g01, g10, g11, 0u); break;
-        case 1: op_u1<1, false>(s, d, g00, g01, g10, g11, 0u); break;
-        case 2: op_u1<2, false>(s, d, g00, g01, g10, g11, 0u); break;
-        default: op_u1<3, false>(s, d, g00, g01, g10, g11, 0u); break;
+        case 0: op_u1m<0>(s, d, m); break;
+        case 1: op_u1m<1>(s, d, m); break;
+        case 2: op_u1m<2>(s, d, m); break;
+        default: op_u1m<3>(s, d, m); break;
         }
         break;
-    }
     case SW_U1R:
         switch (op.k0) {
         case 0: op_u1r<0>(s, d, m); break;

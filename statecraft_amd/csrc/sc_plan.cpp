@@ -433,13 +433,6 @@ void host_apply(const SweepOp &op, const double *m, double *a, uint32_t tid, uin
     std::memcpy(a, out, sizeof(out));
 }
 
-// VALU operations per amplitude of a register op (complex multiply-add = 4).  Ops with a thread-index control
-// or diagonal source pay per-lane selects on top; the rest resolve their conditions per workgroup.
-// Every op also pays a fixed issue cost that fusing ops saves: scalar loads of its record and matrix and
-// about a dozen dispatch branches, each of which stalls the wavefront's instruction fetch.  Measured on
-// gfx950 at about half the cost of a 2x2 complex gate, i.e. ~8 FP64 operations per amplitude.
-constexpr double kOpOverhead = 8.0;
-
 // Qubits an op needs as register bits of its stage (local = the pass's local qubits): its targets and, for
 // a non-diagonal controlled op whose qubits fit in the register bits, its local controls too -- it then
 // needs no per-lane condition and can fuse with its neighbours.  Chunk-fixed controls are uniform per
@@ -451,31 +444,46 @@ uint64_t reg_need(const PlanOp &op, uint64_t local) {
     return popc(both) <= kRegBits ? both : op.targets;
 }
 
-double op_cost(const SweepOp &op) {
-    double c;
+// FP64 operations per amplitude of a register op (complex multiply-add = 4); exchanges do no arithmetic.
+double op_fp64(const SweepOp &op) {
     switch (op.kind) {
-    case SW_U1: c = 8.0; break;
-    case SW_U1R: c = 4.0; break;
-    case SW_U1C: c = op.tctrl ? 9.0 : 8.0; break;
-    case SW_X: c = op.rctrl ? 4.0 : 2.0; break;  // register controls: per-pair selects; else moves
-    case SW_CX: c = 2.0; break;
-    case SW_XL: c = 4.0; break;
-    case SW_U2: c = 16.0; break;
-    case SW_U2R: c = 8.0; break;
-    case SW_SWAP: c = 2.0; break;
-    case SW_D4R: c = 32.0; break;
-    case SW_DIAG16: c = 4.0; break;
-    case SW_DIAGL: c = op.rctrl ? 8.0 : 4.0; break;
-    default: c = 64.0; break;
+    case SW_U1: return 8.0;
+    case SW_U1R: return 4.0;
+    case SW_U1C: return 8.0;
+    case SW_U2: return 16.0;
+    case SW_U2R: return 8.0;
+    case SW_D4: return 64.0;
+    case SW_D4R: return 32.0;
+    case SW_DIAG16: return 4.0;
+    case SW_DIAGL: return 4.0;
+    default: return 0.0;
     }
-    return c + kOpOverhead;
+}
+
+// Planning cost of a register op in VALU operations per amplitude.  In the compiled sweep kernel every op
+// also pays a fixed issue cost that fusing ops saves (scalar loads of its record and matrix and about a
+// dozen uniform dispatch branches), and exchanges with register controls or thread-index conditions are
+// per-lane selects.  In a runtime-compiled pass kernel the op is a constant: exchanges under register
+// controls are register renames and the issue cost is little more than the matrix's scalar loads.
+double op_cost(const SweepOp &op, const SweepCostModel &cm) {
+    double c = op_fp64(op);
+    switch (op.kind) {
+    case SW_U1C: c += op.tctrl ? 1.0 : 0.0; break;
+    case SW_X: c = (cm.const_ops || !op.rctrl) ? 0.0 : 2.0; break;
+    case SW_XL: c = 2.0; break;
+    case SW_CX:
+    case SW_SWAP: c = cm.const_ops ? 0.0 : 1.0; break;
+    case SW_DIAGL: c += op.rctrl ? 2.0 : 0.0; break;
+    default: break;
+    }
+    return c + cm.op_overhead;
 }
 
 // A fused segment: a diagonal over the register index (any number of bits) or a dense block on its bits.
-double fused_cost(int bits, bool real, bool diag) {
-    if (diag) return 4.0 + kOpOverhead;
+double fused_cost(int bits, bool real, bool diag, const SweepCostModel &cm) {
+    if (diag) return 4.0 + cm.op_overhead;
     double c = bits <= 1 ? 8.0 : bits == 2 ? 16.0 : 64.0;
-    return (real ? 0.5 * c : c) + kOpOverhead;
+    return (real ? 0.5 * c : c) + cm.op_overhead;
 }
 
 // Register bits a register-internal op reads or writes.
@@ -496,6 +504,24 @@ uint32_t reg_support(const SweepOp &op) {
 
 bool is_diag_kind(const SweepOp &op) { return op.kind == SW_DIAGL || op.kind == SW_DIAG16; }
 
+// Register bits whose values an op changes (it acts non-diagonally on them): its targets.
+uint32_t nondiag_bits(const SweepOp &op) {
+    switch (op.kind) {
+    case SW_U1:
+    case SW_U1R:
+    case SW_U1C:
+    case SW_X:
+    case SW_XL:
+    case SW_CX: return 1u << op.k0;
+    case SW_U2:
+    case SW_U2R:
+    case SW_SWAP: return (1u << op.k0) | (1u << op.k1);
+    case SW_DIAGL:
+    case SW_DIAG16: return 0u;
+    default: return (1u << kRegBits) - 1u;
+    }
+}
+
 // Ops that only read and write register bits (no thread-index or chunk-fixed dependence): they can be folded
 // into a fused block or diagonal table.
 bool register_internal(const SweepOp &op) {
@@ -508,7 +534,7 @@ bool register_internal(const SweepOp &op) {
 }  // namespace
 
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
-                int max_fuse_bits) {
+                int max_fuse_bits, const SweepCostModel &cm) {
     plan = SweepPlan();
     if (n < kChunkBits) return fail(SC_E_ARG, "sweep plans need at least 12 qubits");
     const uint64_t full = (n == 64) ? ~0ULL : ((1ULL << n) - 1ULL);
@@ -918,6 +944,28 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 sops.push_back(so);
                 sreal.push_back(op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP || op.is_x);
             }
+            // ---- gather diagonals: a register diagonal moves forward to the next one when every op in
+            // between leaves its register bits' values alone (acts non-diagonally only on other bits), so
+            // runs such as Ry Rz Ry Rz .. become Ry Ry .. Rz Rz .., whose diagonals fuse into one table.
+            // Processing from the back lets whole groups form.
+            for (size_t i = sops.size(); i-- > 0;) {
+                if (!register_internal(sops[i]) || !is_diag_kind(sops[i])) continue;
+                const uint32_t dep = reg_support(sops[i]);
+                size_t e = i + 1;
+                while (e < sops.size() && !(register_internal(sops[e]) && is_diag_kind(sops[e])) &&
+                       (nondiag_bits(sops[e]) & dep) == 0u)
+                    e++;
+                if (e == sops.size() || e == i + 1 || !(register_internal(sops[e]) && is_diag_kind(sops[e])))
+                    continue;
+                const SweepOp d = sops[i];
+                const bool dr = sreal[i];
+                for (size_t t = i; t + 1 < e; t++) {
+                    sops[t] = sops[t + 1];
+                    sreal[t] = sreal[t + 1];
+                }
+                sops[e - 1] = d;
+                sreal[e - 1] = dr;
+            }
             // ---- fusion: split the stage's op list into consecutive segments, each either left as is or
             // (when every op in it stays inside the register bits) replaced by one op: a diagonal table over
             // the register index when every op is diagonal (4 VALU ops per amplitude), else a dense block on
@@ -930,7 +978,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             for (size_t e = 1; e <= L; e++) {
                 const SweepOp &last = sops[e - 1];
                 const bool table = last.kind == SW_DIAGL && last.k1 >= 0;
-                best[e] = best[e - 1] + (table ? fused_cost(0, false, true) : op_cost(last));
+                best[e] = best[e - 1] + (table ? fused_cost(0, false, true, cm) : op_cost(last, cm));
                 from[e] = table ? (int)(e - 1) : -1;
                 uint32_t sup = 0;
                 bool real = true, diag = true;
@@ -940,7 +988,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     real = real && sreal[s0];
                     diag = diag && is_diag_kind(sops[s0]);
                     if (e - s0 < 2 || max_fuse_bits <= 0 || (!diag && popc(sup) > max_fuse_bits)) continue;
-                    double c = best[s0] + fused_cost(popc(sup), real, diag);
+                    double c = best[s0] + fused_cost(popc(sup), real, diag, cm);
                     if (c < best[e] - 1e-9) {
                         best[e] = c;
                         from[e] = (int)s0;
@@ -1042,7 +1090,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                                     so.k0 != so.k1;
         if (so.kind == SW_DIAGL) ok = ok && so.k1 < 0 && so.k0 < kRegBits;
         if (!ok) return fail(SC_E_STATE, "internal error: sweep plan produced an invalid register op");
-        plan.fp64_ops += op_cost(so);
+        plan.fp64_ops += op_fp64(so);
     }
     for (auto &so : plan.ops)
         if (so.mat >= 0) so.mat = (int32_t)recipe_off[so.mat];

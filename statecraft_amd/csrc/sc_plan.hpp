@@ -69,13 +69,22 @@ struct SweepPlan {
     uint64_t pool_complex = 0;
     std::vector<int> standalone;   // unused for now (every op kind fits a sweep)
     int fused_blocks = 0;
-    double fp64_ops = 0.0;  // estimated FP64 VALU operations per amplitude over all passes
+    double fp64_ops = 0.0;  // FP64 operations per amplitude of the register ops over all passes
 };
+
+// Cost model of the planner's fusion choices, in FP64 operations per amplitude.
+struct SweepCostModel {
+    double op_overhead;  // fixed cost of every register op
+    bool const_ops;      // ops are compile-time constants (runtime-compiled pass kernels)
+};
+// The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
+constexpr SweepCostModel kInterpretedCost{1.0, false};
+constexpr SweepCostModel kCompiledPassCost{0.5, true};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
-                int max_fuse_bits = kRegBits);
+                int max_fuse_bits = kRegBits, const SweepCostModel &cm = kInterpretedCost);
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS

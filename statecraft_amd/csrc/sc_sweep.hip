@@ -27,6 +27,7 @@
 #include <algorithm>
 
 #include "sc_internal.hpp"
+#include "sc_jit.hpp"
 
 namespace sc {
 
@@ -176,7 +177,10 @@ k_sweep(v2d *__restrict__ amp, int n, int log_chunks, uint64_t total_chunks, con
     }
 }
 
-int sweep_grid_limit(Context *ctx) {
+}  // namespace
+
+// Runtime-compiled pass kernels have the same launch bounds and LDS footprint, hence the same occupancy.
+int sweep_grid_size(Context *ctx) {
     static int per_cu = 0;
     if (per_cu == 0) {
         int nb = 0;
@@ -190,13 +194,11 @@ int sweep_grid_limit(Context *ctx) {
     return want * ctx->cu_count;
 }
 
-}  // namespace
-
 int launch_sweep(Context *ctx, State *st, const SweepPass *pass_dev, const SweepOp *ops_dev, const double *mats_dev,
                  int elem_begin, int elem_count, size_t mat_stride) {
     const int log_chunks = st->n - kChunkBits;
     const uint64_t total = (uint64_t)elem_count << log_chunks;
-    const uint64_t grid = std::min<uint64_t>(total, (uint64_t)sweep_grid_limit(ctx));
+    const uint64_t grid = std::min<uint64_t>(total, (uint64_t)sweep_grid_size(ctx));
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
     hipLaunchKernelGGL(k_sweep, dim3((unsigned)grid), dim3(kThreads), 0, ctx->stream, reinterpret_cast<v2d *>(st->amp),
                        st->n, log_chunks, total, pass_dev, ops_dev, reinterpret_cast<const v2d *>(mats_dev),

@@ -1,0 +1,123 @@
+"""GPU parity of the runtime-compiled sweep passes (SC_OPT_JIT 1 and 2): each pass as its own kernel with the
+pass constant-folded, against the oracle (1e-12 max-abs) and against the compiled sweep kernel."""
+import math
+
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import oracle
+from circuits import random_circuit, random_state
+from statecraft_amd import capi, workloads
+from statecraft_amd.circuit import lower
+from test_plan_host import _diag_real_circuit
+
+pytestmark = pytest.mark.gpu
+EPS = 1e-12
+
+
+@pytest.fixture(params=[1, 2], ids=["jit", "jit-prefetch"])
+def jit_ctx(hip_ctx, request):
+    hip_ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_SWEEP)
+    hip_ctx.set_option(capi.OPT_JIT, request.param)
+    yield hip_ctx
+    hip_ctx.set_option(capi.OPT_JIT, 0)
+    hip_ctx.set_option(capi.OPT_LOW_QUBITS, 4)
+    hip_ctx.set_option(capi.OPT_REMAP, 1)
+
+
+def run(ctx, n, ops, mats, init=None, batch=1, batched=False):
+    st = capi.State(ctx, n, batch)
+    if init is not None:
+        st.upload(init)
+    if batched:
+        st.apply_batched(ops, mats)
+    else:
+        st.apply(ops, mats)
+    out = st.download()
+    st.close()
+    return out
+
+
+@pytest.mark.parametrize("n,low,remap", [(12, 0, 1), (13, 4, 1), (15, 4, 0), (17, 8, 1), (16, 2, 1)])
+def test_jit_random_circuits(jit_ctx, n, low, remap):
+    rng = np.random.default_rng(71 * n + low)
+    ops, mats = random_circuit(n, 300, rng)
+    init = random_state(n, rng)
+    jit_ctx.set_option(capi.OPT_LOW_QUBITS, low)
+    jit_ctx.set_option(capi.OPT_REMAP, remap)
+    got = run(jit_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+    jit_ctx.set_option(capi.OPT_JIT, 0)
+    interp = run(jit_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - interp)) <= EPS
+
+
+def test_jit_dense_fusion(jit_ctx):
+    rng = np.random.default_rng(5)
+    n = 13
+    ops, mats = random_circuit(n, 600, rng, kinds=[capi.OP_SINGLE, capi.OP_TWO])
+    ops["q0"] = ops["q0"] % 4
+    ops["q1"] = np.where(ops["kind"] == capi.OP_TWO, (ops["q0"] + 1 + (ops["q1"] % 3)) % 4, -1)
+    init = random_state(n, rng)
+    got = run(jit_ctx, n, ops, mats, init)
+    assert jit_ctx.plan_stats()["fused_blocks"] > 0
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+
+
+def test_jit_kernels_follow_parameters(jit_ctx):
+    """Circuits of one structure share their pass kernels; every run must read its own matrices."""
+    n = 16
+    for seed in (3, 4, 5):
+        ops, mats = lower(workloads.random_layered(n, 8, seed))
+        got = run(jit_ctx, n, ops, mats)
+        assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS, seed
+
+
+def test_jit_outer_controls(jit_ctx):
+    rng = np.random.default_rng(9)
+    n = 18
+    ops, mats = random_circuit(n, 60, rng, kinds=[capi.OP_MCU, capi.OP_SINGLE, capi.OP_CNOT])
+    init = random_state(n, rng)
+    got = run(jit_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+
+
+@pytest.mark.parametrize("n,low", [(13, 4), (16, 8)])
+def test_jit_diagonal_tables_and_real_blocks(jit_ctx, n, low):
+    rng = np.random.default_rng(17 + n)
+    ops, mats = lower(_diag_real_circuit(n, rng))
+    init = random_state(n, rng)
+    jit_ctx.set_option(capi.OPT_LOW_QUBITS, low)
+    got = run(jit_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+
+
+def test_jit_batched_per_element_matrices(jit_ctx):
+    rng = np.random.default_rng(12)
+    n, B = 13, 3
+    ops, mats0 = random_circuit(n, 50, rng)
+    mats = np.stack([np.asarray(mats0) * np.exp(1j * 0.3 * b) for b in range(B)])
+    init = np.stack([random_state(n, rng) for _ in range(B)])
+    got = run(jit_ctx, n, ops, mats, init, batch=B, batched=True)
+    for b in range(B):
+        assert np.max(np.abs(got[b] - oracle.run_ops(n, ops, mats[b], init[b]))) <= EPS
+
+
+def test_jit_ry_qft_20q(jit_ctx):
+    n = 20
+    circ, thetas = workloads.ry_qft(n, 42)
+    ops, mats = lower(circ)
+    got = run(jit_ctx, n, ops, mats)
+    c = np.ones(1 << n)
+    idx = np.arange(1 << n)
+    for q, t in enumerate(thetas):
+        c *= np.where((idx >> q) & 1, math.sin(t / 2), math.cos(t / 2))
+    assert np.max(np.abs(got - bf.qft_closed_form(c.astype(complex)))) <= EPS
+
+
+def test_jit_random_layered_20q(jit_ctx):
+    n = 20
+    ops, mats = lower(workloads.random_layered(n, 20, 7))
+    got = run(jit_ctx, n, ops, mats)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS

@@ -87,3 +87,17 @@ def test_qft_20_plan_matches_oracle():
     ops, mats = lower(circ)
     stats, out = capi.plan_sweeps_host(16, ops, mats, 4, oracle.zero_state(16))
     assert np.max(np.abs(out - oracle.run_ops(16, ops, mats))) <= 1e-12
+
+
+def test_jit_pass_kernels_compile_without_device(tmp_path):
+    """Every pass of a plan becomes a kernel source that hipRTC compiles for gfx950 (no device needed)."""
+    rng = np.random.default_rng(23)
+    n = 14
+    ops, mats = random_circuit(n, 60, rng)
+    stats, _ = capi.plan_sweeps_host(n, ops, mats, 4)
+    r = capi.jit_compile_host(n, ops, mats, 4, True, 1, str(tmp_path))
+    assert r["kernels"] == stats["passes"] >= 1
+    assert r["code_bytes"] > 0
+    srcs = sorted(tmp_path.glob("sc_passes_*.hip"))
+    assert srcs and all(p.with_suffix(".co").stat().st_size > 0 for p in srcs)
+    assert sum(p.read_text().count("extern \"C\" __global__") for p in srcs) == stats["passes"]
