@@ -190,48 +190,74 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 
 // mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
 int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
-    SweepPlan plan;
-    int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, plan, std::min(ctx->max_fused, kRegBits),
-                         ctx->jit ? kCompiledPassCost : kInterpretedCost);
-    if (rc) return rc;
+    // ---- the plan (and its runtime-compiled kernels) of this circuit structure, cached per context
+    const int max_fuse = std::min(ctx->max_fused, kRegBits);
+    if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
+    const std::string key = plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit);
+    PlanCache::Entry *ent = ctx->plan_cache->find(key);
+    if (!ent) {
+        std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
+        e->key = key;
+        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse,
+                             ctx->jit ? kCompiledPassCost : kInterpretedCost);
+        if (rc) return rc;
+        if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
+        ent = ctx->plan_cache->insert(std::move(e));
+    }
+    ent->used = ++ctx->plan_cache->tick;
+    const SweepPlan &plan = ent->plan;
+    const std::vector<hipFunction_t> &fns = ent->fns;
     if (plan.passes.empty()) return SC_OK;
-    const size_t pass_bytes = plan.passes.size() * sizeof(SweepPass);
-    const size_t ops_bytes = std::max<size_t>(plan.ops.size(), 1) * sizeof(SweepOp);
+    // ---- upload: pass and op records (compiled sweep kernel only) and every element's matrix pool
+    const bool records = fns.empty();
+    const size_t pass_bytes = records ? plan.passes.size() * sizeof(SweepPass) : 0;
+    const size_t ops_bytes = records ? std::max<size_t>(plan.ops.size(), 1) * sizeof(SweepOp) : 0;
     const size_t nelem_pool = mats_elems.size();
     const size_t pool_bytes = std::max<uint64_t>(plan.pool_complex, 1) * 16 * nelem_pool;
-    const size_t total = pass_bytes + ops_bytes + pool_bytes + 1024;
-    rc = ensure_host_scratch(ctx, total);
+    const size_t total = pass_bytes + ops_bytes + pool_bytes;
+    int rc = ensure_host_scratch(ctx, total);
     if (rc) return rc;
     char *h = (char *)ctx->scratch_host;
-    std::memcpy(h, plan.passes.data(), pass_bytes);
-    if (!plan.ops.empty()) std::memcpy(h + pass_bytes, plan.ops.data(), plan.ops.size() * sizeof(SweepOp));
+    if (records) {
+        std::memcpy(h, plan.passes.data(), pass_bytes);
+        if (!plan.ops.empty()) std::memcpy(h + pass_bytes, plan.ops.data(), plan.ops.size() * sizeof(SweepOp));
+    }
     double *hpool = (double *)(h + pass_bytes + ops_bytes);
     for (size_t e = 0; e < nelem_pool; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e);
-    DevBuf d;
-    rc = d.alloc(total);
-    if (rc) return rc;
-    SC_HIP_CHECK(hipMemcpyAsync(d.p, h, pass_bytes + ops_bytes + pool_bytes, hipMemcpyHostToDevice, ctx->stream));
-    const SweepPass *dpass = (const SweepPass *)d.p;
-    const SweepOp *dops = (const SweepOp *)((char *)d.p + pass_bytes);
-    const double *dpool = (const double *)((char *)d.p + pass_bytes + ops_bytes);
+    if (total > ctx->plan_dev_bytes) {
+        if (ctx->plan_dev) SC_HIP_CHECK(hipFree(ctx->plan_dev));
+        ctx->plan_dev = nullptr;
+        ctx->plan_dev_bytes = 0;
+        const size_t want = std::max(total, (size_t)1 << 20);
+        if (hipMalloc(&ctx->plan_dev, want) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->plan_dev = nullptr;
+            return fail(SC_E_OOM, "device allocation of " + std::to_string(want) + " bytes for a sweep plan failed");
+        }
+        ctx->plan_dev_bytes = want;
+    }
+    char *d = (char *)ctx->plan_dev;
+    SC_HIP_CHECK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, ctx->stream));
+    const SweepPass *dpass = (const SweepPass *)d;
+    const SweepOp *dops = (const SweepOp *)(d + pass_bytes);
+    const double *dpool = (const double *)(d + pass_bytes + ops_bytes);
     const size_t stride = nelem_pool > 1 ? plan.pool_complex : 0;
-    std::vector<hipFunction_t> fns;
-    if (ctx->jit && (rc = jit_pass_kernels(ctx, plan, fns))) return rc;
     // keep each launch's grid under ~2^20 blocks
     const int log_chunks = st->n - kChunkBits;
     int per_launch = std::max(1, (int)std::min<uint64_t>((uint64_t)st->batch, (1ULL << 20) >> log_chunks));
     for (size_t p = 0; p < plan.passes.size(); p++) {
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             int cnt = std::min(per_launch, st->batch - e0);
-            rc = fns.empty() ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
-                             : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride);
+            rc = records ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
+                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride);
             if (rc) return rc;
             ctx->last_launches++;
         }
         ctx->last_sweeps++;
     }
     ctx->last_blocks += plan.fused_blocks;
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // plan buffers die with this call
+    // the pinned staging buffer and plan_dev are reused by the next call
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SC_OK;
 }
 
@@ -322,6 +348,8 @@ int sc_ctx_destroy(sc_ctx *ctx) {
     for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->scratch_host) (void)hipHostFree(ctx->scratch_host);
+    if (ctx->plan_dev) (void)hipFree(ctx->plan_dev);
+    delete ctx->plan_cache;
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SC_OK;

@@ -53,6 +53,8 @@ struct ProfScope {
     ~ProfScope();
 };
 
+struct PlanCache;  // sc_plan.hpp
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -69,6 +71,10 @@ struct Context {
     size_t scratch_bytes = 0;
     void *scratch_host = nullptr;  // pinned
     size_t scratch_host_bytes = 0;
+    // device copy of a sweep plan's records and matrix pool (used by sweep execution only)
+    void *plan_dev = nullptr;
+    size_t plan_dev_bytes = 0;
+    PlanCache *plan_cache = nullptr;  // owned
     // plan statistics of the last sc_apply_ops
     uint64_t last_sweeps = 0, last_launches = 0, last_blocks = 0;
 };

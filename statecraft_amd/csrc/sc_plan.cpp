@@ -1174,6 +1174,41 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
     }
 }
 
+PlanCache::Entry *PlanCache::find(const std::string &key) {
+    for (auto &e : entries)
+        if (e->key == key) return e.get();
+    return nullptr;
+}
+
+PlanCache::Entry *PlanCache::insert(std::unique_ptr<Entry> e) {
+    if (entries.size() >= kMaxEntries) {
+        size_t lru = 0;
+        for (size_t i = 1; i < entries.size(); i++)
+            if (entries[i]->used < entries[lru]->used) lru = i;
+        entries.erase(entries.begin() + (long)lru);
+    }
+    entries.push_back(std::move(e));
+    return entries.back().get();
+}
+
+std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
+                           int jit) {
+    std::string k;
+    k.reserve(16 + ops.size() * 48);
+    auto put = [&](const void *p, size_t bytes) { k.append((const char *)p, bytes); };
+    const int32_t head[5] = {n, low_qubits, remap ? 1 : 0, max_fuse_bits, jit};
+    put(head, sizeof head);
+    for (const PlanOp &op : ops) {
+        const int32_t f[4] = {op.kind, op.q0, op.q1, op.mat};
+        const uint64_t m[3] = {op.controls, op.targets, op.diag};
+        const uint8_t c = (uint8_t)((op.is_x ? 1 : 0) | (op.is_diag ? 2 : 0) | (op.is_real ? 4 : 0));
+        put(f, sizeof f);
+        put(m, sizeof m);
+        put(&c, 1);
+    }
+    return k;
+}
+
 void build_pool(const SweepPlan &plan, const double *mats, double *out) {
     // first pass: plain recipes (kinds 0, 1) at their offsets
     std::vector<uint64_t> offs(plan.pool.size());

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,25 @@ constexpr SweepCostModel kCompiledPassCost{0.5, true};
 // register bits needs).
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
                 int max_fuse_bits = kRegBits, const SweepCostModel &cm = kInterpretedCost);
+// Plans of recently executed circuits.  A plan depends on the op structure (kinds, qubits, controls,
+// matrix offsets and the diagonal / real / exchange classification of every matrix) and the options that
+// shape it, not on matrix values, so circuits that differ only in parameters share one entry.
+struct PlanCache {
+    struct Entry {
+        std::string key;
+        SweepPlan plan;
+        std::vector<hipFunction_t> fns;  // runtime-compiled pass kernels (SC_OPT_JIT), else empty
+        uint64_t used = 0;
+    };
+    static constexpr size_t kMaxEntries = 16;
+    std::vector<std::unique_ptr<Entry>> entries;
+    uint64_t tick = 0;
+    Entry *find(const std::string &key);
+    Entry *insert(std::unique_ptr<Entry> e);  // evicts the least recently used entry when full
+};
+std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
+                           int jit);
+
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS

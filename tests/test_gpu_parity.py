@@ -227,3 +227,26 @@ def test_determinism(hip_ctx):
     a = run_gpu(hip_ctx, n, ops, mats, None, capi.EXEC_SWEEP)
     b = run_gpu(hip_ctx, n, ops, mats, None, capi.EXEC_SWEEP)
     assert np.array_equal(a, b)
+
+
+def test_plan_cache_reuse_and_classification(hip_ctx):
+    """Plans are cached by circuit structure: circuits that differ only in parameters reuse a plan and must
+    read their own matrices; a matrix that changes class (diagonal / real / exchange) at the same op must not
+    reuse the plan made for the other class."""
+    hip_ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_SWEEP)
+    n = 16
+    for seed in (3, 4, 5):
+        ops, mats = lower(workloads.random_layered(n, 6, seed))
+        got = run_gpu(hip_ctx, n, ops, mats, None, capi.EXEC_SWEEP)
+        assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS, seed
+    rng = np.random.default_rng(3)
+    ops, mats = random_circuit(n, 120, rng, kinds=[capi.OP_SINGLE, capi.OP_CNOT])
+    init = random_state(n, rng)
+    singles = np.flatnonzero(ops["kind"] == capi.OP_SINGLE)
+    for variant in range(3):
+        m = np.array(mats, copy=True)
+        for i in singles[::3]:
+            o = int(ops[i]["mat"])
+            m[o:o + 4] = [np.array([1, 0, 0, 1j]), np.array([0, 1, 1, 0]), np.array([1, 1, 1, -1]) / np.sqrt(2)][variant]
+        got = run_gpu(hip_ctx, n, ops, m, init, capi.EXEC_SWEEP)
+        assert np.max(np.abs(got - oracle.run_ops(n, ops, m, init))) <= EPS, variant
