@@ -42,9 +42,10 @@ def log(msg: str) -> None:
 def make_context(args) -> capi.Context:
     ctx = capi.Context(0, MODES[args.mode])
     ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
-    ctx.set_option(capi.OPT_REMAP, 0 if args.no_remap else 1)
+    ctx.set_option(capi.OPT_REMAP, 1 if args.remap else 0)
     ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
     ctx.set_option(capi.OPT_JIT, args.jit)
+    ctx.set_option(capi.OPT_SWEEP_PASS_CAP, args.pass_cap)
     return ctx
 
 
@@ -142,8 +143,8 @@ def bench_c2(args):
             "qubits": n, "depth": depth, "ops": int(len(ops)), "shots": args.shots, "exec_mode": args.mode,
             "sweeps_per_step": sweeps, "kernel_launches_per_step": plan["launches"],
             "fused_16x16_blocks": plan["fused_blocks"], "parallelism": "1 GPU",
-            "low_qubits": args.low_qubits, "remap": not args.no_remap, "blocks_per_cu": args.blocks_per_cu,
-            "jit": bool(args.jit),
+            "low_qubits": args.low_qubits, "remap": bool(args.remap), "blocks_per_cu": args.blocks_per_cu,
+            "jit": args.jit, "pass_cap": args.pass_cap,
         },
         "roofline": {
             "bound": "hbm",
@@ -160,7 +161,7 @@ def bench_c2(args):
             "copy_kernel_gbs": copy_gbs,
             "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None,
             "fp64_ops_per_amp_per_step": capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None,
-                                                               not args.no_remap, bool(args.jit))[0]["fp64_ops_per_amp"]
+                                                               bool(args.remap), bool(args.jit))[0]["fp64_ops_per_amp"]
             if args.mode == "sweep" else None,
         },
         "amp_updates_per_step": credit,
@@ -258,7 +259,7 @@ def bench_c3(args):
                  "plan": ctx.plan_stats()},
                 marked_probability=p, closed_form=want, rel_error=abs(p - want) / want,
                 roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "kernel": "k_sweep"},
+                          "frac": achieved / HBM_PEAK_GBS, "kernel": "sc_pass_*" if args.jit else "k_sweep"},
                 shots_1024_distinct=int(len(np.unique(out))))
     st.close()
     ctx.close()
@@ -271,8 +272,7 @@ def bench_c4(args):
     n_inst, p = args.instances, 4
     sizes = (rng.next_longs(n_inst) % np.uint64(11)).astype(np.int64) + 12
     ctx = make_context(args)
-    t0 = time.perf_counter()
-    mean_c = []
+    batches = []
     for n in range(12, 23):
         idx = np.nonzero(sizes == n)[0]
         if len(idx) == 0:
@@ -283,28 +283,48 @@ def bench_c4(args):
         w = 0.5 + rng.next_doubles(B * n).reshape(B, n)
         ops, _ = lower(workloads.qaoa_ring(n, p, g[0], b[0], w[0]))
         mats = np.stack([lower(workloads.qaoa_ring(n, p, g[k], b[k], w[k]))[1] for k in range(B)])
-        st = capi.State(ctx, n, B)
-        st.apply_batched(ops, mats)
         zm = np.array([(1 << i) | (1 << ((i + 1) % n)) for i in range(n)], dtype=np.uint64)
-        # <C> = sum_e w_e (1 - <Z_i Z_j>_e) / 2 per instance
-        zz = np.stack([st.expectation(np.zeros(1, np.uint64), zm[e:e + 1], [1.0]) for e in range(n)])  # [n, B]
-        mean_c.extend(list(0.5 * np.sum(w.T * (1.0 - zz), axis=0)))
-        st.close()
-    ctx.synchronize()
-    t_qaoa = time.perf_counter() - t0
+        batches.append((n, B, ops, mats, w, zm))
+
+    def qaoa():
+        mean_c = []
+        for n, B, ops, mats, w, zm in batches:
+            st = capi.State(ctx, n, B)
+            st.apply_batched(ops, mats)
+            # <C> = sum_e w_e (1 - <Z_i Z_j>_e) / 2 per instance
+            zz = np.stack([st.expectation(np.zeros(1, np.uint64), zm[e:e + 1], [1.0]) for e in range(n)])  # [n, B]
+            mean_c.extend(list(0.5 * np.sum(w.T * (1.0 - zz), axis=0)))
+            st.close()
+        ctx.synchronize()
+        return mean_c
+
     # GHZ-20 trajectories with depolarising p = 0.01 on both operands after every CNOT
     nt, nq = args.trajectories, 20
     gops, gmats = lower(workloads.ghz(nq))
     noise_after = (gops["kind"] == capi.OP_CNOT).astype(np.uint8)
     draws = 2 * int(noise_after.sum())
     u = np.stack([SplittableRandom(1000 + t).next_doubles(draws) for t in range(nt)])
+
+    def trajectories():
+        st = capi.State(ctx, nq, nt)
+        st.run_depolarizing_trajectories(gops, gmats, noise_after, 0.01, u)
+        parity = st.expectation([0], [(1 << nq) - 1], [1.0])
+        ctx.synchronize()
+        st.close()
+        return parity
+
+    for i in range(args.warmup):  # plans and pass kernels of every instance size and noise segment
+        qaoa()
+        trajectories()
+        log(f"c4 warmup {i + 1}/{args.warmup} done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mean_c = qaoa()
+    t_qaoa = (time.perf_counter() - t0) / args.steps
     t1 = time.perf_counter()
-    st = capi.State(ctx, nq, nt)
-    st.run_depolarizing_trajectories(gops, gmats, noise_after, 0.01, u)
-    parity = st.expectation([0], [(1 << nq) - 1], [1.0])
-    ctx.synchronize()
-    t_traj = time.perf_counter() - t1
-    st.close()
+    for _ in range(args.steps):
+        parity = trajectories()
+    t_traj = (time.perf_counter() - t1) / args.steps
     return _line("batched QAOA instances/s (12-22 q, p=4) + GHZ-20 depolarising trajectories/s",
                  n_inst / t_qaoa, "instances/s", args, 1e3 * (t_qaoa + t_traj),
                  {"workload": "qaoa-ring-batched+ghz20-trajectories", "instances": n_inst,
@@ -326,7 +346,8 @@ def bench_c5(args):
     mats = np.stack([lower(workloads.hea(n, layers, thetas[k]))[1] for k in range(batch)])
     ctx = make_context(args)
     st = capi.State(ctx, n, batch)
-    st.adjoint_gradient(ops, mats[:1].repeat(batch, 0), P, *obs, checkpoint_every=0, per_element=True)  # warm
+    # warm-up with the timed call's segmentation: plans and pass kernels of every checkpoint segment
+    st.adjoint_gradient(ops, mats[:1].repeat(batch, 0), P, *obs, checkpoint_every=args.checkpoint, per_element=True)
     st.set_basis(0)
     t0 = time.perf_counter()
     grads, energies = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=args.checkpoint, per_element=True)
@@ -336,14 +357,20 @@ def bench_c5(args):
     # parameter shift on the plain engine: E(theta +- pi/2) per parameter, one state per evaluation
     one = capi.State(ctx, n, 1)
     sample = list(range(0, P, max(1, P // args.shift_sample)))[:args.shift_sample]
-    ps = []
-    t1 = time.perf_counter()
+    # the shifted circuits are lowered before the timed region: only engine work is timed
+    shifted = []
     for p in sample:
-        vals = []
         for sgn in (1, -1):
             th = thetas[0].copy()
             th[p] += sgn * math.pi / 2
-            o, m = lower(workloads.hea(n, layers, th))
+            shifted.append(lower(workloads.hea(n, layers, th)))
+    one.set_basis(0)
+    one.apply(*shifted[0])  # plan (and kernels) of the structure, as for the adjoint warm-up
+    ps = []
+    t1 = time.perf_counter()
+    for k in range(len(sample)):
+        vals = []
+        for o, m in shifted[2 * k:2 * k + 2]:
             one.set_basis(0)
             one.apply(o, m)
             vals.append(one.expectation(*obs)[0])
@@ -382,11 +409,14 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true")
     ap.add_argument("--low-qubits", type=int, default=4)
-    ap.add_argument("--no-remap", action="store_true")
+    ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--jit", type=int, default=0, choices=[0, 1, 2],
-                    help="1: sweep passes as runtime-compiled kernels (compiled during warmup)")
+    ap.add_argument("--pass-cap", type=int, default=-1,
+                    help="SC_OPT_SWEEP_PASS_CAP: FP64 operations per amplitude one pass takes on (-1 default, 0 none)")
+    ap.add_argument("--jit", type=int, default=2, choices=[0, 1, 2],
+                    help="SC_OPT_JIT: 0 compiled sweep kernel, 1 runtime-compiled pass kernels, 2 the same with "
+                         "next-chunk prefetch (kernels are compiled during warmup)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",

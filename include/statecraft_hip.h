@@ -103,6 +103,8 @@ typedef struct sc_op {
                                    constant-folded; compiled once per circuit structure and cached; 2: the same
                                    with the next chunk's HBM loads issued before the current chunk's stages;
                                    default 0 */
+#define SC_OPT_SWEEP_PASS_CAP 9 /* most estimated FP64 operations per amplitude one sweep pass takes on (0: no
+                                   limit; -1: the default of the sweep kernel kind), default -1 */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
@@ -214,12 +216,13 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 /* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks);
 /* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
- * (n >= 12) as a context with the given low-qubit, remap and jit options would, and reports its passes,
+ * (n >= 12) as a context with the given low-qubit, remap, jit and pass-cap options would, and reports its passes,
  * stages, fused 16x16 blocks and FP64 operations per amplitude.  When state_inout is non-NULL (2^n
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, uint64_t *passes,
+                        uint64_t *stages,
                         uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout);
 /* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
  * SC_OPT_JIT path) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the

@@ -193,13 +193,15 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     // ---- the plan (and its runtime-compiled kernels) of this circuit structure, cached per context
     const int max_fuse = std::min(ctx->max_fused, kRegBits);
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
-    const std::string key = plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit);
+    SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
+    if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
+    const std::string key =
+        plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit, cm.pass_cap);
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
         e->key = key;
-        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse,
-                             ctx->jit ? kCompiledPassCost : kInterpretedCost);
+        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm);
         if (rc) return rc;
         if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
         ent = ctx->plan_cache->insert(std::move(e));
@@ -391,6 +393,10 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value < 0 || value > 2) return fail(SC_E_ARG, "jit must be 0, 1 or 2");
         ctx->jit = (int)value;
         return SC_OK;
+    case SC_OPT_SWEEP_PASS_CAP:
+        if (value < -1 || value > 100000) return fail(SC_E_ARG, "sweep pass cap must be -1..100000");
+        ctx->pass_cap = (int)value;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -407,6 +413,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_REMAP: *value = ctx->remap; return SC_OK;
     case SC_OPT_SWEEP_BLOCKS_PER_CU: *value = ctx->sweep_blocks_per_cu; return SC_OK;
     case SC_OPT_JIT: *value = ctx->jit; return SC_OK;
+    case SC_OPT_SWEEP_PASS_CAP: *value = ctx->pass_cap; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -720,7 +727,8 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 }
 
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, uint64_t *passes, uint64_t *stages,
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, uint64_t *passes,
+                        uint64_t *stages,
                         uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
@@ -730,10 +738,10 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     std::vector<PlanOp> pops;
     int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
     if (rc) return rc;
+    SweepCostModel cm = jit ? kCompiledPassCost : kInterpretedCost;
+    if (pass_cap >= 0) cm.pass_cap = (double)pass_cap;
     SweepPlan plan;
-    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits,
-                          jit ? kCompiledPassCost : kInterpretedCost)))
-        return rc;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, cm))) return rc;
     uint64_t ns = 0;
     for (const auto &p : plan.passes) ns += (uint64_t)p.nstages;
     if (passes) *passes = plan.passes.size();

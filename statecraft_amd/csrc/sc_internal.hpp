@@ -64,6 +64,7 @@ struct Context {
     int remap = 1;
     int sweep_blocks_per_cu = 0;
     int jit = 0;
+    int pass_cap = -1;  // SC_OPT_SWEEP_PASS_CAP
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)

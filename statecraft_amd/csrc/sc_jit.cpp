@@ -42,15 +42,17 @@ std::string hex32(uint32_t v) {
     return b;
 }
 
-// Per-lane value sum_i (bit i of `var` ? tab[i] : 0), combined with `op` ("+" or "^"), as an expression.
+// Per-lane value sum_i (bit i of `var` ? scale * tab[i] : 0), combined with `op` ("+" or "^"), as an
+// expression.
 template <typename T>
-std::string lane_expr(const char *var, const T *tab, int bits, const char *op, bool wide) {
+std::string lane_expr(const char *var, const T *tab, int bits, const char *op, bool wide, uint32_t scale = 1) {
     std::string e;
     for (int i = 0; i < bits; i++) {
         if (tab[i] == 0) continue;
         if (!e.empty()) e += std::string(" ") + op + " ";
         e += std::string("((") + var + " & " + std::to_string(1u << i) + "u) ? " +
-             (wide ? hex64((uint64_t)tab[i]) : hex32((uint32_t)tab[i])) + " : " + (wide ? "0ULL" : "0u") + ")";
+             (wide ? hex64((uint64_t)tab[i] * scale) : hex32((uint32_t)tab[i] * scale)) + " : " +
+             (wide ? "0ULL" : "0u") + ")";
     }
     if (e.empty()) e = wide ? "0ULL" : "0u";
     return e;
@@ -136,8 +138,10 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "  auto cbase = [](uint64_t x) -> uint64_t { return "
       << deposit_expr("x", ps.outer_mask) << "; };\n";
     s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, kThreadBits, "+", true) << ";\n";
-    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, kThreadBits, "^", false) << ";\n";
-    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, kThreadBits, "^", false) << ";\n";
+    // LDS addresses are byte offsets (slot * 16), so that each access costs one XOR
+    s << "  char *const ldsb = (char *)lds;\n";
+    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, kThreadBits, "^", false, 16) << ";\n";
+    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, kThreadBits, "^", false, 16) << ";\n";
     auto regs_load = [&](const char *dst, const char *srcp) {
         for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << ps.glob_reg[r] << "];\n";
     };
@@ -145,10 +149,12 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << ps.glob_reg[r] << "] = " << src << "[" << r << "];\n";
     };
     auto lds_write = [&](const char *base, const uint32_t *off, const char *src) {
-        for (int r = 0; r < kRegs; r++) s << "    lds[" << base << " ^ " << off[r] << "u] = " << src << "[" << r << "];\n";
+        for (int r = 0; r < kRegs; r++)
+            s << "    *(v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u << "u)) = " << src << "[" << r << "];\n";
     };
     auto lds_read = [&](const char *dst, const char *base, const uint32_t *off) {
-        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = lds[" << base << " ^ " << off[r] << "u];\n";
+        for (int r = 0; r < kRegs; r++)
+            s << "    " << dst << "[" << r << "] = *(const v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u << "u));\n";
     };
     s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
     s << "  uint64_t cb = cbase(c & chunk_mask);\n"
@@ -175,7 +181,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         s << "    {\n"
              "    const uint32_t t = opaque(tid);\n"
              "    const uint32_t ts = "
-          << lane_expr("t", stg.thread_slot, kThreadBits, "^", false) << ";\n";
+          << lane_expr("t", stg.thread_slot, kThreadBits, "^", false, 16) << ";\n";
         lds_read("a", "ts", stg.lds_off);
         bool in_a = true;
         for (int o = stg.op_begin; o < stg.op_end; o++) {

@@ -486,6 +486,15 @@ double fused_cost(int bits, bool real, bool diag, const SweepCostModel &cm) {
     return (real ? 0.5 * c : c) + cm.op_overhead;
 }
 
+// FP64 operations per amplitude an op is expected to cost after fusion: diagonals share tables (a
+// quarter of one each), exchanges are free, real matrices cost half.
+double plan_op_fp64(const PlanOp &op) {
+    if (op.is_x || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP) return 0.0;
+    if (op.is_diag || op.kind == SC_OP_DIAG2) return 1.0;
+    if (op.kind == SC_OP_TWO) return op.is_real ? 8.0 : 16.0;
+    return op.is_real ? 4.0 : 8.0;
+}
+
 // Register bits a register-internal op reads or writes.
 uint32_t reg_support(const SweepOp &op) {
     switch (op.kind) {
@@ -577,21 +586,68 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // ---- choose the pass's op set and local (logical) qubits; the low physical positions are forced
         uint64_t L = 0;
         for (int p = 0; p < lb; p++) L |= 1ULL << logi[p];
-        Blocker blk;
-        std::vector<int> sched, next;
-        for (int i : remaining) {
-            const PlanOp &op = ops[i];
-            if (blk.blocked(op) || popc(L | op.targets) > kChunkBits || popc(op.targets) > kRegBits) {
-                blk.mark(op);
-                next.push_back(i);
-                continue;
+        const uint64_t fixed = L;
+        // A pass stops taking on arithmetic at the cost model's cap: the HBM sweep a pass costs hides only
+        // so much FP64 work, and ops left over on qubits that stay local (the low positions) fill the
+        // arithmetic headroom of later passes instead.
+        // Ops a pass with local qubits LL takes on (program order; an op left behind blocks its qubits).
+        auto take = [&](uint64_t LL, bool grow, uint64_t *L_out, std::vector<int> *s_out,
+                        std::vector<int> *n_out) -> int {
+            Blocker b;
+            double wk = 0.0;
+            int cnt = 0;
+            for (int i : remaining) {
+                const PlanOp &op = ops[i];
+                const double w = plan_op_fp64(op);
+                const bool fits = grow ? popc(LL | op.targets) <= kChunkBits : (op.targets & ~LL) == 0;
+                if (b.blocked(op) || !fits || popc(op.targets) > kRegBits ||
+                    (cm.pass_cap > 0.0 && w > 0.0 && wk + w > cm.pass_cap)) {
+                    b.mark(op);
+                    if (n_out) n_out->push_back(i);
+                    continue;
+                }
+                wk += w;
+                cnt++;
+                if (grow) LL |= op.targets;
+                if (s_out) s_out->push_back(i);
             }
-            L |= op.targets;
-            sched.push_back(i);
+            if (L_out) *L_out = LL;
+            return cnt;
+        };
+        // first guess: the targets of the earliest takeable ops, filled with the lowest positions
+        take(L, true, &L, nullptr, nullptr);
+        if (!fixup) {
+            for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
+            // hill climbing: exchange one free local qubit for an outer target qubit while that lets the
+            // pass take on more ops (on layered circuits this halves the number of passes)
+            uint64_t cand = 0;
+            for (int i : remaining) cand |= ops[i].targets;
+            int best = take(L, false, nullptr, nullptr, nullptr);
+            for (int iter = 0; iter < 4 * kChunkBits; iter++) {
+                bool improved = false;
+                for (uint64_t fa = L & ~fixed; fa && !improved; fa &= fa - 1) {
+                    const uint64_t a = fa & (~fa + 1ULL);
+                    for (uint64_t fb = cand & ~L; fb; fb &= fb - 1) {
+                        const uint64_t L2 = (L & ~a) | (fb & (~fb + 1ULL));
+                        const int c = take(L2, false, nullptr, nullptr, nullptr);
+                        if (c > best) {
+                            best = c;
+                            L = L2;
+                            improved = true;
+                            break;
+                        }
+                    }
+                }
+                if (!improved) break;
+            }
         }
+        std::vector<int> sched, next;
+        take(L, false, nullptr, &sched, &next);
+        if (!fixup && sched.empty()) return fail(SC_E_STATE, "internal error: a sweep pass took on no op");
         if (fixup) {
             // bring logical 0..lb-1 home, then misplaced qubits together with the occupant of their home
-            for (int q = 0; q < lb; q++) L |= 1ULL << q;
+            for (int q = 0; q < lb; q++)
+                if (popc(L | (1ULL << q)) <= kChunkBits) L |= 1ULL << q;
             for (int q = 0; q < n; q++) {
                 if (phys[q] == q) continue;
                 uint64_t add = (1ULL << q) | (1ULL << logi[q]);
@@ -628,6 +684,8 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         }
         // ---- local qubits: fill to 12 with the qubits at the lowest unused physical positions
         for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
+        if (popc(L) != kChunkBits) return fail(SC_E_STATE, "internal error: a sweep pass needs 12 local qubits");
+        if (plan.passes.size() > 4 * ops.size() + 64) return fail(SC_E_STATE, "internal error: sweep planning stalled");
         int lq[kChunkBits];  // local position j -> physical position (ascending)
         int lp[64];          // logical qubit -> local position, -1 when outer
         uint64_t Lphys = 0;
@@ -1192,12 +1250,13 @@ PlanCache::Entry *PlanCache::insert(std::unique_ptr<Entry> e) {
 }
 
 std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
-                           int jit) {
+                           int jit, double pass_cap) {
     std::string k;
-    k.reserve(16 + ops.size() * 48);
+    k.reserve(32 + ops.size() * 48);
     auto put = [&](const void *p, size_t bytes) { k.append((const char *)p, bytes); };
     const int32_t head[5] = {n, low_qubits, remap ? 1 : 0, max_fuse_bits, jit};
     put(head, sizeof head);
+    put(&pass_cap, sizeof pass_cap);
     for (const PlanOp &op : ops) {
         const int32_t f[4] = {op.kind, op.q0, op.q1, op.mat};
         const uint64_t m[3] = {op.controls, op.targets, op.diag};

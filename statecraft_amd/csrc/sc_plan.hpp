@@ -77,10 +77,11 @@ struct SweepPlan {
 struct SweepCostModel {
     double op_overhead;  // fixed cost of every register op
     bool const_ops;      // ops are compile-time constants (runtime-compiled pass kernels)
+    double pass_cap;     // most estimated FP64 operations per amplitude a pass takes on (0: no limit)
 };
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
-constexpr SweepCostModel kInterpretedCost{1.0, false};
-constexpr SweepCostModel kCompiledPassCost{0.5, true};
+constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0};
+constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).
@@ -103,7 +104,7 @@ struct PlanCache {
     Entry *insert(std::unique_ptr<Entry> e);  // evicts the least recently used entry when full
 };
 std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
-                           int jit);
+                           int jit, double pass_cap);
 
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);

@@ -114,3 +114,24 @@ def test_layered_diagonals_gather_into_tables(n, depth):
         stats, out = capi.plan_sweeps_host(n, ops, mats, 4, init, False, jit)
         assert stats["fp64_ops_per_amp"] <= 5.2 * n * depth
         assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+
+
+@pytest.mark.parametrize("seed", [2, 7])
+def test_restore_passes_with_eight_low_qubits(seed):
+    """Qubit relabelling with all eight low positions forced: the passes that bring qubits home after the
+    last op must stay within 12 local qubits (regression: they once asked for up to 16)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(12, 27))
+    ops, mats = random_circuit(n, int(rng.integers(50, 600)), rng)
+    stats, _ = capi.plan_sweeps_host(n, ops, mats, 8, None, True)
+    assert stats["passes"] >= 1
+
+
+@pytest.mark.parametrize("n,low,remap", [(13, 8, True), (15, 8, True), (16, 3, True), (14, 0, False)])
+def test_local_set_search_matches_oracle(n, low, remap):
+    rng = np.random.default_rng(500 + n + low)
+    ops, mats = lower(workloads.random_layered(n, 8, 11))
+    init = random_state(n, rng)
+    for jit in (False, True):
+        stats, out = capi.plan_sweeps_host(n, ops, mats, low, init, remap, jit)
+        assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
