@@ -70,6 +70,33 @@ def cpu_baseline(n, ops, mats, seconds: float):
     return credit / dt, done, dt
 
 
+def pmc_traffic(fetch_csv: str, write_csv: str, copy_bytes: float):
+    """HBM bytes per sweep launch from two rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE;
+    KiB per dispatch), each scaled by what the same counter reads for k_copy, whose traffic is known
+    (copy_bytes / 2 read and copy_bytes / 2 written per launch)."""
+    import csv
+    import re
+
+    def per_dispatch(path, counter):
+        acc = {}
+        for row in csv.DictReader(open(path)):
+            if row["Counter_Name"] != counter:
+                continue
+            k = row["Kernel_Name"].replace("sc::(anonymous namespace)::", "").replace("void ", "")
+            k = re.sub(r"sc_pass_\d+", "sc_pass_*", re.sub(r"\(.*", "", k))
+            s, d = acc.get(k, (0.0, set()))
+            acc[k] = (s + float(row["Counter_Value"]), d | {row["Dispatch_Id"]})
+        return {k: s / len(d) for k, (s, d) in acc.items()}
+
+    f, w = per_dispatch(fetch_csv, "FETCH_SIZE"), per_dispatch(write_csv, "WRITE_SIZE")
+    kern = "sc_pass_*" if "sc_pass_*" in f else "k_sweep"
+    if kern not in f or kern not in w or "k_copy" not in f or "k_copy" not in w:
+        return None
+    f_scale = (copy_bytes / 2) / f["k_copy"]
+    w_scale = (copy_bytes / 2) / w["k_copy"]
+    return f[kern] * f_scale + w[kern] * w_scale
+
+
 def bench_c2(args):
     n, depth = args.qubits, args.depth
     circ = workloads.random_layered(n, depth, args.seed)
@@ -118,6 +145,9 @@ def bench_c2(args):
     copy_gbs = None
     if not args.no_copy_reference:
         dst = capi.State(ctx, n)
+        dst.copy_from(st)  # first touch of the new allocation, untimed
+        ctx.synchronize()
+        ctx.profile_reset()
         ctx.set_option(capi.OPT_PROFILE, 1)
         for _ in range(3):
             dst.copy_from(st)
@@ -125,6 +155,11 @@ def bench_c2(args):
         c_n, c_ms, c_bytes = ctx.profile_read(capi.KFAM_COPY)
         copy_gbs = (c_bytes / c_n) / (c_ms / c_n * 1e-3) / 1e9 if c_n else None
         dst.close()
+    fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None, bool(args.remap), bool(args.jit),
+                                           args.pass_cap)[0]["fp64_ops_per_amp"] if args.mode == "sweep" else None)
+    traffic = None
+    if args.traffic_csv:
+        traffic = pmc_traffic(args.traffic_csv[0], args.traffic_csv[1], 32.0 * (1 << n))
     result = {
         "metric": METRIC,
         "value": value,
@@ -152,7 +187,7 @@ def bench_c2(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": {"sweep": "sc_pass_*" if args.jit else "k_sweep", "fused": "k_dense",
                        "direct": "k_single"}[args.mode],
             "bytes_per_launch": bytes_per_launch,
@@ -160,9 +195,11 @@ def bench_c2(args):
             "launches": launches,
             "copy_kernel_gbs": copy_gbs,
             "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None,
-            "fp64_ops_per_amp_per_step": capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None,
-                                                               bool(args.remap), bool(args.jit))[0]["fp64_ops_per_amp"]
-            if args.mode == "sweep" else None,
+            # the same launches against the FP64 vector peak (an FMA counts 2 flops)
+            "fp64_tflops": (2.0 * fp64_per_step * (1 << n) / (circuit_ms * 1e-3) / 1e12) if fp64_per_step else None,
+            "fp64_frac": (2.0 * fp64_per_step * (1 << n) / (circuit_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS)
+            if fp64_per_step else None,
+            "fp64_ops_per_amp_per_step": fp64_per_step,
         },
         "amp_updates_per_step": credit,
         "gate_ms_per_step": circuit_ms,
@@ -412,6 +449,9 @@ def main(argv=None) -> int:
     ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--traffic-csv", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="rocprofv3 --pmc counter_collection.csv files of FETCH_SIZE and WRITE_SIZE runs of this "
+                         "workload: fills roofline.traffic")
     ap.add_argument("--pass-cap", type=int, default=-1,
                     help="SC_OPT_SWEEP_PASS_CAP: FP64 operations per amplitude one pass takes on (-1 default, 0 none)")
     ap.add_argument("--jit", type=int, default=2, choices=[0, 1, 2],
