@@ -127,6 +127,10 @@ static_assert(sizeof(SweepOp) == 32, "sweep op records are 32 bytes");
 // Slot tables are 32-bit so that the kernel reads them with scalar dword loads.
 struct SweepStage {
     int32_t op_begin, op_end;
+    // 1: the exchange into this stage stays inside each wavefront (the wave-index bits, thread bits 6 and 7,
+    // sit on the same local positions before and after), so it needs no workgroup barrier
+    int32_t wave_local;
+    int32_t pad;
     uint32_t lds_off[kRegs];            // swizzled LDS slot offset per register index
     uint32_t thread_slot[kThreadBits];  // swizzled slot contribution of each thread-index bit
 };
@@ -134,7 +138,9 @@ struct SweepStage {
 struct SweepPass {
     uint64_t outer_mask;            // global qubits outside the chunk
     int32_t nstages;
-    int32_t pad;
+    // bit 0: the last stage -> store phase exchange stays inside each wavefront; bit 1: so does the store
+    // phase -> next chunk's load phase (the workgroup may overwrite its LDS without a barrier)
+    int32_t wave_local;
     uint64_t glob_thread[kThreadBits];  // load/store phase: global offset of thread-index bit i
     uint64_t glob_reg[kRegs];       // load/store phase: global offset of register index r
     uint32_t io_thread_slot[kThreadBits];  // load phase: swizzled slot of thread-index bit i

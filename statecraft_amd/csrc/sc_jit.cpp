@@ -156,15 +156,19 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         for (int r = 0; r < kRegs; r++)
             s << "    " << dst << "[" << r << "] = *(const v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u << "u));\n";
     };
+    // Exchanges in which every wavefront reads back only the LDS slots it wrote need no workgroup barrier.
+    const int S = ps.nstages;
+    auto sync = [&](bool wave_local) { s << (wave_local ? "    wave_sync();\n" : "    __syncthreads();\n"); };
+    const bool into_first = S > 0 ? ps.stages[0].wave_local != 0 : (ps.wave_local & 1) != 0;
     s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
     s << "  uint64_t cb = cbase(c & chunk_mask);\n"
          "  {\n"
          "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gthread);\n";
     regs_load("p", "src");
     lds_write("sthread", ps.io_reg_slot, "p");
-    s << "  }\n"
-         "  __syncthreads();\n"
-         "  for (;;) {\n"
+    s << "  }\n";
+    sync(into_first);
+    s << "  for (;;) {\n"
          "    const uint64_t elem = (uint64_t)elem_begin + (c >> log_chunks);\n"
          "    CV2 *emats = mats + elem * mat_stride;\n"
          "    const uint64_t next = c + gridDim.x;\n"
@@ -190,8 +194,8 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         }
         s << "    const uint32_t tw = opaque(ts);\n";
         lds_write("tw", stg.lds_off, in_a ? "a" : "p");
-        s << "    __syncthreads();\n"
-             "    }\n";
+        sync(st + 1 < S ? ps.stages[st + 1].wave_local != 0 : (ps.wave_local & 1) != 0);
+        s << "    }\n";
     }
     s << "    {\n"
          "    const uint32_t sl = opaque(sout);\n";
@@ -204,14 +208,14 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     s << "    }\n"
          "    if (next >= total_chunks) break;\n"
          "    c = next;\n"
-         "    cb = ncb;\n"
-         "    __syncthreads();\n"
-         "    {\n"
+         "    cb = ncb;\n";
+    sync((ps.wave_local & 2) != 0);  // every wavefront has read its store-phase slots
+    s << "    {\n"
          "    const uint32_t sl = opaque(sthread);\n";
     lds_write("sl", ps.io_reg_slot, prefetch ? "q" : "p");
-    s << "    }\n"
-         "    __syncthreads();\n"
-         "  }\n"
+    s << "    }\n";
+    sync(into_first);
+    s << "  }\n"
          "}\n";
     return s.str();
 }

@@ -93,6 +93,15 @@ __device__ __forceinline__ T opaque(T v) {
     return v;
 }
 
+// Orders this wavefront's LDS accesses before and after it: enough for an exchange in which every wavefront
+// reads back only slots it wrote (a wavefront's LDS operations execute in order), where other exchanges
+// need a workgroup barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void copy_all(const Regs &s, Regs &d) {
 #pragma unroll
     for (int r = 0; r < kRegs; r++) d[r] = s[r];

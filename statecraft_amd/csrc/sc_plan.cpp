@@ -659,22 +659,67 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // ---- split into stages
         std::vector<StageBuild> stages;
         std::vector<int> rest = sched;
-        while (!rest.empty() && (int)stages.size() < kMaxStages) {
-            StageBuild sb;
+        // Ops a stage with register qubits RR takes on (program order, blocking as for passes); with grow the
+        // register set grows with the ops' needs up to 4 qubits.
+        auto stage_take = [&](uint64_t RR, bool grow, StageBuild *sb, std::vector<int> *rest_out) -> int {
             Blocker sblk;
-            std::vector<int> rest2;
+            int cnt = 0;
             for (int i : rest) {
                 const PlanOp &op = ops[i];
                 const uint64_t need = reg_need(op, L);
-                if (sblk.blocked(op) || popc(sb.R | need) > kRegBits) {
+                const bool fits = grow ? popc(RR | need) <= kRegBits : (need & ~RR) == 0;
+                if (sblk.blocked(op) || !fits) {
                     sblk.mark(op);
-                    rest2.push_back(i);
+                    if (rest_out) rest_out->push_back(i);
                     continue;
                 }
-                sb.R |= need;
-                sb.ops.push_back(i);
+                if (grow) RR |= need;
+                cnt++;
+                if (sb) sb->ops.push_back(i);
             }
-            stages.push_back(sb);
+            if (sb) sb->R = RR;
+            return cnt;
+        };
+        while (!rest.empty() && (int)stages.size() < kMaxStages) {
+            StageBuild sb;
+            stage_take(0, true, &sb, nullptr);
+            // hill climbing on the register qubits, as for the pass's local qubits: every exchange through
+            // LDS between stages costs a read and a write of the whole chunk
+            uint64_t R = sb.R, cand = 0;
+            for (int i : rest) cand |= reg_need(ops[i], L);
+            int best = stage_take(R, false, nullptr, nullptr);
+            for (int iter = 0; iter < 16; iter++) {
+                bool improved = false;
+                for (uint64_t fq = cand & ~R; fq && !improved; fq &= fq - 1) {
+                    const uint64_t q = fq & (~fq + 1ULL);
+                    if (popc(R) < kRegBits) {
+                        const int c = stage_take(R | q, false, nullptr, nullptr);
+                        if (c > best) {
+                            best = c;
+                            R |= q;
+                            improved = true;
+                        }
+                        continue;
+                    }
+                    for (uint64_t fr = R; fr; fr &= fr - 1) {
+                        const uint64_t R2 = (R & ~(fr & (~fr + 1ULL))) | q;
+                        const int c = stage_take(R2, false, nullptr, nullptr);
+                        if (c > best) {
+                            best = c;
+                            R = R2;
+                            improved = true;
+                            break;
+                        }
+                    }
+                }
+                if (!improved) break;
+            }
+            StageBuild chosen;
+            std::vector<int> rest2;
+            stage_take(R, false, &chosen, &rest2);
+            if (chosen.ops.empty()) return fail(SC_E_STATE, "internal error: a sweep stage took on no op");
+            chosen.R = R;
+            stages.push_back(chosen);
             rest.swap(rest2);
         }
         if (!rest.empty()) {  // truncated: return to the pool in program order
@@ -773,8 +818,11 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         struct StageLayout {
             int rpos[kRegBits];
             int tpos[kThreadBits];
+            bool wave_local;
         };
         std::vector<StageLayout> layouts(stages.size());
+        std::vector<uint32_t> stage_rp(stages.size(), 0u);
+        int pass_wave_local = 0;
         for (size_t s = 0; s < stages.size(); s++) {
             uint32_t rp = 0;
             for (int q = 0; q < n; q++)
@@ -796,11 +844,66 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             }
             for (int p = kChunkBits - 1; p >= 0 && popc(rp) < kRegBits; p--)
                 if (!((rp >> p) & 1u)) rp |= 1u << p;
-            int rj = 0, tj = 0;
-            for (int p = 0; p < kChunkBits; p++) {
-                if ((rp >> p) & 1u) layouts[s].rpos[rj++] = p;
-                else layouts[s].tpos[tj++] = p;
+            stage_rp[s] = rp;
+        }
+        // ---- wave-index bits: thread bits 6 and 7 select the wavefront.  When they sit on the same two
+        // local positions before and after an exchange, every wavefront reads back only what it wrote and
+        // the exchange needs no workgroup barrier.  A dynamic programme over the stages picks the ordered
+        // pair (a, b) per stage (both outside the stage's register positions) with the fewest changes,
+        // starting from the load phase's pair (6, 7) and ending at the store phase's (pi[6], pi[7]).
+        {
+            const int S = (int)stages.size();
+            const int kPairs = kChunkBits * kChunkBits;
+            const int INF = 1 << 28;
+            std::vector<std::vector<int>> cost(S, std::vector<int>(kPairs, INF)), from(S, std::vector<int>(kPairs, -1));
+            auto ok = [&](int s, int w) {
+                const int a = w / kChunkBits, b = w % kChunkBits;
+                return a != b && !((stage_rp[s] >> a) & 1u) && !((stage_rp[s] >> b) & 1u);
+            };
+            const int w_load = 6 * kChunkBits + 7, w_store = pi[6] * kChunkBits + pi[7];
+            for (int w = 0; w < kPairs; w++)
+                if (S > 0 && ok(0, w)) cost[0][w] = (w != w_load);
+            for (int s = 1; s < S; s++)
+                for (int w = 0; w < kPairs; w++) {
+                    if (!ok(s, w)) continue;
+                    for (int v = 0; v < kPairs; v++) {
+                        if (cost[s - 1][v] >= INF) continue;
+                        const int c = cost[s - 1][v] + (v != w);
+                        if (c < cost[s][w] || (c == cost[s][w] && v == w)) {
+                            cost[s][w] = c;
+                            from[s][w] = v;
+                        }
+                    }
+                }
+            std::vector<int> wsel(S, -1);
+            if (S > 0) {
+                int best = -1;
+                for (int w = 0; w < kPairs; w++) {
+                    if (cost[S - 1][w] >= INF) continue;
+                    const int c = cost[S - 1][w] + (w != w_store);
+                    if (best < 0 || c < cost[S - 1][best] + (best != w_store)) best = w;
+                }
+                for (int s = S - 1; s >= 0; s--) {
+                    wsel[s] = best;
+                    best = from[s][best];
+                }
             }
+            for (int s = 0; s < S; s++) {
+                const uint32_t rp = stage_rp[s];
+                const int a = wsel[s] / kChunkBits, b = wsel[s] % kChunkBits;
+                int rj = 0, tj = 0;
+                for (int p = 0; p < kChunkBits; p++) {
+                    if ((rp >> p) & 1u) layouts[s].rpos[rj++] = p;
+                    else if (p != a && p != b) layouts[s].tpos[tj++] = p;
+                }
+                layouts[s].tpos[6] = a;
+                layouts[s].tpos[7] = b;
+                layouts[s].wave_local = wsel[s] == (s == 0 ? w_load : wsel[s - 1]);
+            }
+            pass_wave_local = 0;
+            if (S > 0 ? wsel[S - 1] == w_store : w_load == w_store) pass_wave_local |= 1;
+            if (w_store == w_load) pass_wave_local |= 2;
+            pass.wave_local = pass_wave_local;
         }
         // ---- swizzle: hv[p] for local bits p >= kSwzBits, chosen so every stage's lanes 0..3 (and 0..2)
         // land on distinct low slot bits.
@@ -871,6 +974,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         for (size_t s = 0; s < stages.size(); s++) {
             SweepStage &stg = pass.stages[s];
             const StageLayout &lay = layouts[s];
+            stg.wave_local = lay.wave_local ? 1 : 0;
             for (int i = 0; i < kThreadBits; i++) stg.thread_slot[i] = slot_of(1u << lay.tpos[i]);
             for (int r = 0; r < kRegs; r++) {
                 uint32_t l = 0;

@@ -208,6 +208,10 @@ class HipStatevectorEngine:
     def id(self) -> str:
         return self.ID
 
+    def close(self) -> None:
+        """Releases the device context (its stream, scratch and cached plans)."""
+        self.ctx.close()
+
     def simulate(self, request: SimulationRequest) -> SimulationResult:
         if request is None:
             raise NullPointerException("request")
