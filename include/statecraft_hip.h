@@ -224,8 +224,15 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
                         uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, uint64_t *passes,
                         uint64_t *stages,
                         uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout);
+/* Host-only (no device needed): the reverse sweep of an adjoint gradient as sweep passes (the plan the GPU's
+ * runtime-compiled adjoint passes run), executed on the host: psi_inout holds the circuit's output state,
+ * lam_inout H applied to it (2^n interleaved complex values each, one batch element); on return they hold
+ * the states uncomputed to the circuit's input and grads[n_params] the gradient. */
+int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                         int n_params, int low_qubits, double *psi_inout, double *lam_inout, double *grads);
 /* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
- * SC_OPT_JIT path) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the
+ * SC_OPT_JIT path; jit = 3: the adjoint passes of the circuit's reverse sweep) and compiles them with hipRTC
+ * on `threads` threads.  Reports the kernel count, the
  * compile wall time and the code object size; when dump_dir is non-NULL each module's source (.hip) and
  * code object (.co) are written there. */
 int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,

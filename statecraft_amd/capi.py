@@ -79,7 +79,7 @@ EXPORTED = [
     "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
     "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_expectation_pauli",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
-    "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host",
+    "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
 ]
 
 _lib = None
@@ -130,6 +130,7 @@ def _sig(lib):
         "sc_last_plan_stats": (i32, [P, U64P, U64P, U64P]),
         "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, U64P, U64P, U64P, DP, DP]),
         "sc_jit_compile_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, U64P, DP, U64P, ctypes.c_char_p]),
+        "sc_adjoint_plan_host": (i32, [i32, P, u64, DP, u64, i32, i32, DP, DP, DP]),
     }
     for name, (res, args) in s.items():
         f = getattr(lib, name)
@@ -187,6 +188,18 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
                                     1 if remap else 0, 1 if jit else 0, int(pass_cap),
                                     ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(f), ptr))
     return {"passes": a.value, "stages": b.value, "fused_blocks": c.value, "fp64_ops_per_amp": f.value}, out
+
+
+def adjoint_plan_host(n: int, ops, mats, n_params: int, psi, lam, low_qubits: int = 4):
+    """Host-only reverse sweep of an adjoint gradient as sweep passes: returns (grads, psi, lambda) after it."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+    p = np.array(psi, dtype=np.complex128, copy=True)
+    lm = np.array(lam, dtype=np.complex128, copy=True)
+    g = np.zeros(max(n_params, 1), dtype=np.float64)
+    check(lib().sc_adjoint_plan_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(n_params),
+                                     int(low_qubits), dptr(p.view(np.float64)), dptr(lm.view(np.float64)), dptr(g)))
+    return g[:n_params], p, lm
 
 
 def jit_compile_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = True, threads: int = 1,

@@ -11,9 +11,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <vector>
 
 #include "sc_internal.hpp"
+#include "sc_jit.hpp"
 #include "sc_plan.hpp"
 
 namespace sc {
@@ -99,23 +102,29 @@ __global__ void __launch_bounds__(kBlock) k_dot_partial(const double2 *__restric
     block_reduce2(re, im, out + (elem * kParts + blockIdx.x) * 2);
 }
 
-// lam (+)= coeff * i^ny * (-1)^{|(i^x)&z|} psi_{i^x}
-__global__ void __launch_bounds__(kBlock) k_pauli_apply(const double2 *__restrict__ psi, double2 *__restrict__ lam,
-                                                        int n, uint64_t total, uint64_t x, uint64_t z, double cr,
-                                                        double ci, int first) {
+// lam = sum_t c_t i^ny_t (-1)^{|(i^x_t)&z_t|} psi_{i^x_t} in one pass: the terms are grouped by x (goff[k] ..
+// goff[k+1] in tz / tc), so psi is gathered once per distinct x and lam written once.
+__global__ void __launch_bounds__(kBlock) k_pauli_sum(const double2 *__restrict__ psi, double2 *__restrict__ lam,
+                                                      int n, uint64_t total, const uint64_t *__restrict__ gx,
+                                                      const int *__restrict__ goff, int ngroups,
+                                                      const uint64_t *__restrict__ tz, const double *__restrict__ tc) {
     const uint64_t dmask = (1ULL << n) - 1ULL;
     for (uint64_t g = blockIdx.x * (uint64_t)kBlock + threadIdx.x; g < total; g += (uint64_t)gridDim.x * kBlock) {
-        uint64_t i = g & dmask;
-        uint64_t j = i ^ x;
-        double2 v = psi[(g - i) + j];
-        if (__popcll(j & z) & 1) v = make_double2(-v.x, -v.y);
-        double2 t = cmul(cr, ci, v);
-        if (first) {
-            lam[g] = t;
-        } else {
-            double2 l = lam[g];
-            lam[g] = make_double2(l.x + t.x, l.y + t.y);
+        const uint64_t i = g & dmask;
+        double ar = 0.0, ai = 0.0;
+        for (int k = 0; k < ngroups; k++) {
+            const uint64_t j = i ^ gx[k];
+            const double2 v = psi[(g - i) + j];
+            double wr = 0.0, wi = 0.0;
+            for (int t = goff[k]; t < goff[k + 1]; t++) {
+                const double sg = (__popcll(j & tz[t]) & 1) ? -1.0 : 1.0;
+                wr = fma(sg, tc[2 * t], wr);
+                wi = fma(sg, tc[2 * t + 1], wi);
+            }
+            ar += wr * v.x - wi * v.y;
+            ai += wr * v.y + wi * v.x;
         }
+        lam[g] = make_double2(ar, ai);
     }
 }
 
@@ -331,7 +340,161 @@ void inverse_matrix(int kind, const double *m, double *out) {
         }
 }
 
+// The reverse sweep over ops [begin, end) as one op list: the ops in reverse order with their inverse matrices
+// (inv_elems, same offsets as the forward matrices), and before the inverse of every parameterized op a probe
+// of its generator (gradient slot slot[g]).
+int adjoint_sequence(int n, const sc_op *ops, uint64_t begin, uint64_t end, const std::vector<int> &slot,
+                     const std::vector<const double *> &inv_elems, uint64_t n_mat_complex, std::vector<PlanOp> &seq) {
+    std::vector<sc_op> rev(ops + begin, ops + end);
+    std::reverse(rev.begin(), rev.end());
+    std::vector<PlanOp> inv_ops;
+    int rc = analyze_ops(n, rev.data(), rev.size(), inv_elems, n_mat_complex, inv_ops);
+    if (rc) return rc;
+    seq.clear();
+    seq.reserve(2 * inv_ops.size());
+    for (size_t j = 0; j < inv_ops.size(); j++) {
+        const uint64_t g = end - 1 - j;
+        if (slot[g] >= 0) {
+            const sc_op &o = ops[g];
+            PlanOp pr;
+            pr.kind = kPlanGrad;
+            pr.q0 = o.q0;
+            pr.q1 = o.gen == SC_GEN_ZZ ? o.q1 : -1;
+            pr.mat = slot[g];
+            pr.gen = o.gen;
+            pr.is_real = true;
+            if (o.gen == SC_GEN_X || o.gen == SC_GEN_Y) {
+                pr.targets = 1ULL << o.q0;
+            } else {
+                pr.is_diag = true;
+                pr.diag = (1ULL << o.q0) | (pr.q1 >= 0 ? 1ULL << pr.q1 : 0ULL);
+            }
+            seq.push_back(pr);
+        }
+        seq.push_back(inv_ops[j]);
+    }
+    return SC_OK;
+}
+
+// Sum of gsub partials per (slot, element), in order: out[k] = sum_s part[k * gsub + s].
+__global__ void k_sub_final(const double *__restrict__ part, uint64_t count, int gsub, double *__restrict__ out) {
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < gsub; s++) {
+        re += part[(k * gsub + s) * 2];
+        im += part[(k * gsub + s) * 2 + 1];
+    }
+    out[2 * k] = re;
+    out[2 * k + 1] = im;
+}
+
+// The reverse sweep as runtime-compiled adjoint passes.  Per checkpoint segment (last first) the reversed ops
+// with inverse matrices, and a gradient probe before the inverse of every parameterized op, are planned like a
+// forward circuit; each pass then reads and writes psi and lambda once for all the ops and probes it holds,
+// instead of once per op.  res receives <lambda| G |psi> per (slot, element).
+int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint64_t n_ops, const std::vector<int> &slot,
+                   int n_slots, const std::vector<const double *> &inv_elems, uint64_t n_mat_complex, uint64_t C,
+                   uint64_t n_ckpt, const std::function<int(uint64_t)> &check_drift, double *res) {
+    const int n = st->n;
+    const uint64_t B = (uint64_t)st->batch;
+    const int log_chunks = n - kChunkBits;
+    const uint64_t nchunks = 1ULL << log_chunks;
+    const uint64_t glimit = (uint64_t)std::max(1, sweep_grid_size(ctx));
+    const uint64_t Bl = std::min<uint64_t>(B, glimit);
+    const int gsub = (int)std::max<uint64_t>(1, std::min<uint64_t>(nchunks, glimit / Bl));
+    const uint64_t count = (uint64_t)n_slots * B;
+    DevBuf dpart;
+    int rc = dpart.alloc(std::max<uint64_t>(count * (uint64_t)(gsub + 1), 1) * 2 * sizeof(double));
+    if (rc) return rc;
+    if (count > 0) SC_HIP_CHECK(hipMemsetAsync(dpart.p, 0, count * (uint64_t)gsub * 2 * sizeof(double), ctx->stream));
+    double *part = (double *)dpart.p;
+    if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
+    for (int64_t k = (int64_t)n_ckpt; k >= 0; k--) {
+        const uint64_t begin = C > 0 ? (uint64_t)k * C : 0;
+        const uint64_t end = (C > 0 && (uint64_t)k < n_ckpt) ? ((uint64_t)k + 1) * C : n_ops;
+        if ((uint64_t)k < n_ckpt && (rc = check_drift((uint64_t)k))) return rc;
+        if (end <= begin) continue;
+        std::vector<PlanOp> seq;
+        if ((rc = adjoint_sequence(n, ops, begin, end, slot, inv_elems, n_mat_complex, seq))) return rc;
+        // plan and kernels of this segment's structure, cached like forward plans
+        const int max_fuse = std::min(ctx->max_fused, 2);  // both states' registers stay within the budget
+        const std::string key = plan_cache_key(n, seq, ctx->low_qubits, false, max_fuse, ctx->jit, 0.0) + "adjoint";
+        PlanCache::Entry *ent = ctx->plan_cache->find(key);
+        if (!ent) {
+            std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
+            e->key = key;
+            if ((rc = plan_sweeps(n, seq, ctx->low_qubits, false, e->plan, max_fuse, kCompiledPassCost))) return rc;
+            if (!e->plan.passes.empty() && (rc = jit_adjoint_kernels(ctx, e->plan, e->fns))) return rc;
+            ent = ctx->plan_cache->insert(std::move(e));
+        }
+        ent->used = ++ctx->plan_cache->tick;
+        const SweepPlan &plan = ent->plan;
+        if (plan.passes.empty()) continue;
+        // matrix pools of the inverse matrices, one per element when they differ
+        const uint64_t E = inv_elems.size();
+        std::vector<double> hpool(2 * std::max<uint64_t>(plan.pool_complex, 1) * E);
+        for (uint64_t e = 0; e < E; e++) build_pool(plan, inv_elems[e], hpool.data() + 2 * plan.pool_complex * e);
+        DevBuf dpool;
+        if ((rc = dpool.alloc(hpool.size() * sizeof(double)))) return rc;
+        SC_HIP_CHECK(hipMemcpyAsync(dpool.p, hpool.data(), hpool.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        const v2d *mats = (const v2d *)dpool.p;
+        uint64_t stride = E > 1 ? plan.pool_complex : 0;
+        v2d *psi = (v2d *)st->amp, *lamv = (v2d *)lam;
+        int nn = n, lc = log_chunks, Bi = (int)B, gs = gsub;
+        for (size_t p = 0; p < plan.passes.size(); p++) {
+            for (uint64_t e0 = 0; e0 < B; e0 += Bl) {
+                int e0i = (int)e0, nb = (int)std::min<uint64_t>(Bl, B - e0);
+                void *args[] = {&psi, &lamv, &nn, &lc, &mats, &stride, &e0i, &nb, &gs, &part, &Bi};
+                ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 64.0 * (double)((uint64_t)nb << n));
+                SC_HIP_CHECK(hipModuleLaunchKernel(ent->fns[p], (unsigned)((uint64_t)nb * (uint64_t)gsub), 1, 1,
+                                                   kThreads, 1, 1, 0, ctx->stream, args, nullptr));
+            }
+        }
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the pool buffer is released with this segment
+    }
+    if (count > 0) {
+        double *out = part + count * (uint64_t)gsub * 2;
+        hipLaunchKernelGGL(k_sub_final, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, part, count,
+                           gsub, out);
+        SC_HIP_CHECK(hipGetLastError());
+        SC_HIP_CHECK(hipMemcpyAsync(res, out, count * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return SC_OK;
+}
+
 }  // namespace
+
+int adjoint_plan_host(int n, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                      int n_params, int low_qubits, SweepPlan &plan, std::vector<double> &inv, std::vector<int> &slot) {
+    if (n_ops > 0 && !ops) return fail(SC_E_NULL, "ops is null");
+    if (n_mat_complex > 0 && !mats) return fail(SC_E_NULL, "mats is null");
+    if (n < kChunkBits || n > 30) return fail(SC_E_ARG, "adjoint sweep plans need 12..30 qubits");
+    if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+    if (n_params < 0) return fail(SC_E_ARG, "n_params must be non-negative");
+    slot.assign(n_ops, -1);
+    int n_slots = 0;
+    for (uint64_t i = 0; i < n_ops; i++) {
+        const sc_op &o = ops[i];
+        if (o.param < 0) continue;
+        if (o.param >= n_params) return fail(SC_E_ARG, "op " + std::to_string(i) + " parameter index out of range");
+        const bool ok = (o.kind == SC_OP_SINGLE && (o.gen == SC_GEN_X || o.gen == SC_GEN_Y || o.gen == SC_GEN_Z)) ||
+                        (o.kind == SC_OP_DIAG2 && o.gen == SC_GEN_ZZ);
+        if (!ok) return fail(SC_E_UNSUPPORTED, "op " + std::to_string(i) + ": unsupported generator for adjoint passes");
+        slot[i] = n_slots++;
+    }
+    inv.assign(2 * std::max<uint64_t>(n_mat_complex, 1), 0.0);
+    for (uint64_t i = 0; i < n_ops; i++)
+        if (ops[i].mat >= 0 && ops[i].kind != SC_OP_CNOT && ops[i].kind != SC_OP_SWAP)
+            inverse_matrix(ops[i].kind, mats + 2 * (int64_t)ops[i].mat, inv.data() + 2 * ops[i].mat);
+    std::vector<const double *> inv_elems{inv.data()};
+    std::vector<PlanOp> seq;
+    int rc = adjoint_sequence(n, ops, 0, n_ops, slot, inv_elems, n_mat_complex, seq);
+    if (rc) return rc;
+    return plan_sweeps(n, seq, low_qubits, false, plan, 2, kCompiledPassCost);
+}
 
 }  // namespace sc
 
@@ -347,6 +510,26 @@ int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zm
     if (rc) return rc;
     if ((rc = set_device(st->ctx))) return rc;
     return expectation(st->ctx, st, st->amp, xmask, zmask, coeff, n_terms, out);
+}
+
+int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                         int n_params, int low_qubits, double *psi_inout, double *lam_inout, double *grads) {
+    if (!psi_inout || !lam_inout || !grads) return fail(SC_E_NULL, "state, lambda and grads are required");
+    SweepPlan plan;
+    std::vector<double> inv;
+    std::vector<int> slot;
+    int rc = adjoint_plan_host(num_qubits, ops, n_ops, mats, n_mat_complex, n_params, low_qubits, plan, inv, slot);
+    if (rc) return rc;
+    int n_slots = 0;
+    for (int v : slot) n_slots = std::max(n_slots, v + 1);
+    std::vector<double> pool(2 * std::max<uint64_t>(plan.pool_complex, 1));
+    build_pool(plan, inv.data(), pool.data());
+    std::vector<double> sums(2 * (size_t)std::max(n_slots, 1), 0.0);
+    execute_plan_host(num_qubits, plan, pool.data(), psi_inout, lam_inout, sums.data());
+    std::fill(grads, grads + n_params, 0.0);
+    for (uint64_t i = 0; i < n_ops; i++)
+        if (slot[i] >= 0) grads[ops[i].param] += 2.0 * ops[i].coeff * sums[2 * slot[i] + 1];
+    return SC_OK;
 }
 
 int sc_adjoint_stats(sc_ctx *ctx, uint64_t *reanchors, double *max_drift) {
@@ -429,13 +612,45 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     if ((rc = dlam.alloc(state_bytes))) return rc;
     double2 *lam = (double2 *)dlam.p;
     const uint64_t total = B << n;
-    for (int t = 0; t < n_terms; t++) {
-        int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
-        double pr = ny == 0 ? 1.0 : ny == 2 ? -1.0 : 0.0, pi = ny == 1 ? 1.0 : ny == 3 ? -1.0 : 0.0;
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 48.0 * (double)total);
-        hipLaunchKernelGGL(k_pauli_apply, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                           total, xmask[t], zmask[t], coeff[t] * pr, coeff[t] * pi, t == 0 ? 1 : 0);
+    {
+        // terms grouped by their X mask, coefficients with i^ny folded in
+        std::vector<uint64_t> gx, tz;
+        std::vector<int> goff;
+        std::vector<double> tc;
+        std::vector<int> order(n_terms);
+        for (int t = 0; t < n_terms; t++) order[t] = t;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return xmask[a] < xmask[b]; });
+        for (int t : order) {
+            if (gx.empty() || gx.back() != xmask[t]) {
+                gx.push_back(xmask[t]);
+                goff.push_back((int)tz.size());
+            }
+            const int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+            const double pr = ny == 0 ? 1.0 : ny == 2 ? -1.0 : 0.0, pi = ny == 1 ? 1.0 : ny == 3 ? -1.0 : 0.0;
+            tz.push_back(zmask[t]);
+            tc.push_back(coeff[t] * pr);
+            tc.push_back(coeff[t] * pi);
+        }
+        goff.push_back((int)tz.size());
+        const size_t bytes = (gx.size() + tz.size()) * 8 + goff.size() * 4 + tc.size() * 8 + 64;
+        DevBuf dterms;
+        if ((rc = dterms.alloc(bytes))) return rc;
+        char *d = (char *)dterms.p;
+        const uint64_t *dgx = (const uint64_t *)d;
+        const uint64_t *dtz = dgx + gx.size();
+        const double *dtc = (const double *)(dtz + tz.size());
+        const int *dgoff = (const int *)(dtc + tc.size());
+        if (!gx.empty()) {
+            SC_HIP_CHECK(hipMemcpyAsync((void *)dgx, gx.data(), gx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            SC_HIP_CHECK(hipMemcpyAsync((void *)dtz, tz.data(), tz.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            SC_HIP_CHECK(hipMemcpyAsync((void *)dtc, tc.data(), tc.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        }
+        SC_HIP_CHECK(hipMemcpyAsync((void *)dgoff, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, (16.0 * (double)gx.size() + 16.0) * (double)total);
+        hipLaunchKernelGGL(k_pauli_sum, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
+                           total, dgx, dgoff, (int)gx.size(), dtz, dtc);
         SC_HIP_CHECK(hipGetLastError());
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // dterms is released at the end of this block
     }
     // ---- energies E_b = Re <psi|lambda>
     const uint64_t part_count = (uint64_t)std::max(n_slots, 1) * B;
@@ -451,83 +666,105 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         if ((rc = finish_parts(ctx, epart, B, e.data()))) return rc;
         for (uint64_t b = 0; b < B; b++) energies[b] = e[2 * b];
     }
+    // ---- keep the forward output: it is what psi must hold when this call returns, and a copy is far cheaper
+    // than running the circuit again (which remains the fallback when the copy does not fit)
+    DevBuf dsave;
+    const bool saved = dsave.alloc(state_bytes) == SC_OK;
+    if (saved) SC_HIP_CHECK(hipMemcpyAsync(dsave.p, st->amp, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
     // ---- reverse sweep
     ctx->adj_reanchors = 0;
     ctx->adj_max_drift = 0.0;
     std::vector<double> mx;
     const unsigned dgrid = grid_for(ctx, total);
-    for (int64_t g = (int64_t)n_ops - 1; g >= 0; g--) {
-        // psi should now equal the state after op g: compare with the checkpoint taken there
-        if (C > 0 && (uint64_t)(g + 1) % C == 0 && (uint64_t)(g + 1) / C >= 1 && (uint64_t)(g + 1) / C <= n_ckpt) {
-            uint64_t k = (uint64_t)(g + 1) / C - 1;
-            double2 *ck = (double2 *)((char *)dckpt.p + state_bytes * k);
-            if (ctx->test_drift > 0) {
-                hipLaunchKernelGGL(k_add_to_first, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   st->amp, n, (int)B, (double)ctx->test_drift * 1e-12);
-                SC_HIP_CHECK(hipGetLastError());
-            }
-            if ((rc = ensure_scratch(ctx, (size_t)dgrid * sizeof(double)))) return rc;
-            hipLaunchKernelGGL(k_maxdiff, dim3(dgrid), dim3(kBlock), 0, ctx->stream, st->amp, ck, total,
-                               (double *)ctx->scratch);
+    // psi should equal the state the forward run checkpointed after its segment k: re-anchor when it drifted
+    auto check_drift = [&](uint64_t k) -> int {
+        double2 *ck = (double2 *)((char *)dckpt.p + state_bytes * k);
+        if (ctx->test_drift > 0) {
+            hipLaunchKernelGGL(k_add_to_first, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream,
+                               st->amp, n, (int)B, (double)ctx->test_drift * 1e-12);
             SC_HIP_CHECK(hipGetLastError());
-            mx.resize(dgrid);
-            SC_HIP_CHECK(hipMemcpyAsync(mx.data(), ctx->scratch, dgrid * sizeof(double), hipMemcpyDeviceToHost,
-                                        ctx->stream));
-            SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-            double drift = *std::max_element(mx.begin(), mx.end());
-            ctx->adj_max_drift = std::max(ctx->adj_max_drift, drift);
-            if (!(drift <= 1e-10)) {
-                SC_HIP_CHECK(hipMemcpyAsync(st->amp, ck, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
-                ctx->adj_reanchors++;
-            }
         }
-        const PlanOp &op = pops[g];
-        const sc_op &o = ops[g];
-        double *pslot = slot[g] >= 0 ? part + (uint64_t)slot[g] * B * kParts * 2 : nullptr;
-        const double *dinvp = (const double *)dinv.p + 2 * (o.mat >= 0 ? o.mat : 0);
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 64.0 * (double)total);
-        switch (op.kind) {
-        case SC_OP_SINGLE:
-        case SC_OP_MCU:
-        case SC_OP_CNOT: {
-            int target = op.kind == SC_OP_CNOT ? op.q1 : op.q0;
-            uint64_t controls = op.controls;
-            int lg = n - 1 - __builtin_popcountll(controls);
-            hipLaunchKernelGGL(k_adj_pair, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                               target, controls | (1ULL << target), controls, lg, dinvp, inv_stride,
-                               op.is_x ? 1 : 0, slot[g] >= 0 ? o.gen : 0, pslot);
-            break;
-        }
-        case SC_OP_TWO:
-            hipLaunchKernelGGL(k_adj_two, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                               op.q0, op.q1, dinvp, inv_stride);
-            break;
-        case SC_OP_DIAG2:
-            hipLaunchKernelGGL(k_adj_diag2, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                               op.q0, op.q1, dinvp, inv_stride, slot[g] >= 0 ? o.gen : 0, pslot);
-            break;
-        case SC_OP_SWAP:
-            hipLaunchKernelGGL(k_adj_swap, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                               op.q0, op.q1);
-            break;
-        }
+        int rc2 = ensure_scratch(ctx, (size_t)dgrid * sizeof(double));
+        if (rc2) return rc2;
+        hipLaunchKernelGGL(k_maxdiff, dim3(dgrid), dim3(kBlock), 0, ctx->stream, st->amp, ck, total,
+                           (double *)ctx->scratch);
         SC_HIP_CHECK(hipGetLastError());
+        mx.resize(dgrid);
+        SC_HIP_CHECK(hipMemcpyAsync(mx.data(), ctx->scratch, dgrid * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        double drift = *std::max_element(mx.begin(), mx.end());
+        ctx->adj_max_drift = std::max(ctx->adj_max_drift, drift);
+        if (!(drift <= 1e-10)) {
+            SC_HIP_CHECK(hipMemcpyAsync(st->amp, ck, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            ctx->adj_reanchors++;
+        }
+        return SC_OK;
+    };
+    std::vector<double> res(2 * (uint64_t)n_slots * B, 0.0);
+    bool fused = ctx->jit != 0 && ctx->exec_mode == SC_EXEC_SWEEP && n >= kChunkBits;
+    for (uint64_t i = 0; i < n_ops && fused; i++)
+        if (slot[i] >= 0 && ops[i].kind == SC_OP_MCU) fused = false;  // controlled generators: per-op path
+    if (fused) {
+        std::vector<const double *> inv_elems;
+        for (uint64_t e = 0; e < E; e++) inv_elems.push_back(inv.data() + 2 * e * n_mat_complex);
+        rc = fused_backward(ctx, st, lam, ops, n_ops, slot, n_slots, inv_elems, n_mat_complex, C, n_ckpt,
+                            check_drift, res.data());
+        if (rc) return rc;
+    } else {
+        for (int64_t g = (int64_t)n_ops - 1; g >= 0; g--) {
+            if (C > 0 && (uint64_t)(g + 1) % C == 0 && (uint64_t)(g + 1) / C >= 1 && (uint64_t)(g + 1) / C <= n_ckpt)
+                if ((rc = check_drift((uint64_t)(g + 1) / C - 1))) return rc;
+            const PlanOp &op = pops[g];
+            const sc_op &o = ops[g];
+            double *pslot = slot[g] >= 0 ? part + (uint64_t)slot[g] * B * kParts * 2 : nullptr;
+            const double *dinvp = (const double *)dinv.p + 2 * (o.mat >= 0 ? o.mat : 0);
+            ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 64.0 * (double)total);
+            switch (op.kind) {
+            case SC_OP_SINGLE:
+            case SC_OP_MCU:
+            case SC_OP_CNOT: {
+                int target = op.kind == SC_OP_CNOT ? op.q1 : op.q0;
+                uint64_t controls = op.controls;
+                int lg = n - 1 - __builtin_popcountll(controls);
+                hipLaunchKernelGGL(k_adj_pair, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                                   n, target, controls | (1ULL << target), controls, lg, dinvp, inv_stride,
+                                   op.is_x ? 1 : 0, slot[g] >= 0 ? o.gen : 0, pslot);
+                break;
+            }
+            case SC_OP_TWO:
+                hipLaunchKernelGGL(k_adj_two, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                                   n, op.q0, op.q1, dinvp, inv_stride);
+                break;
+            case SC_OP_DIAG2:
+                hipLaunchKernelGGL(k_adj_diag2, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                                   n, op.q0, op.q1, dinvp, inv_stride, slot[g] >= 0 ? o.gen : 0, pslot);
+                break;
+            case SC_OP_SWAP:
+                hipLaunchKernelGGL(k_adj_swap, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                                   n, op.q0, op.q1);
+                break;
+            }
+            SC_HIP_CHECK(hipGetLastError());
+        }
+        if (n_slots > 0 && (rc = finish_parts(ctx, part, (uint64_t)n_slots * B, res.data()))) return rc;
     }
     // ---- gradients
     std::fill(grads, grads + B * (uint64_t)n_params, 0.0);
-    if (n_slots > 0) {
-        std::vector<double> res(2 * (uint64_t)n_slots * B);
-        if ((rc = finish_parts(ctx, part, (uint64_t)n_slots * B, res.data()))) return rc;
-        for (uint64_t i = 0; i < n_ops; i++) {
-            if (slot[i] < 0) continue;
-            for (uint64_t b = 0; b < B; b++)
-                grads[b * n_params + ops[i].param] += 2.0 * ops[i].coeff * res[2 * ((uint64_t)slot[i] * B + b) + 1];
-        }
+    for (uint64_t i = 0; i < n_ops; i++) {
+        if (slot[i] < 0) continue;
+        for (uint64_t b = 0; b < B; b++)
+            grads[b * n_params + ops[i].param] += 2.0 * ops[i].coeff * res[2 * ((uint64_t)slot[i] * B + b) + 1];
     }
     // restore psi to the forward output (the reverse sweep uncomputed it)
-    rc = apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, mats_per_element ? n_mat_complex : 0,
-                        mats_per_element != 0);
-    if (rc) return rc;
+    if (saved) {
+        SC_HIP_CHECK(hipMemcpyAsync(st->amp, dsave.p, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        set_error("");
+        rc = apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, mats_per_element ? n_mat_complex : 0,
+                            mats_per_element != 0);
+        if (rc) return rc;
+    }
     SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SC_OK;
 }

@@ -764,13 +764,23 @@ int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
     if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
     if (threads < 1 || threads > 64) return fail(SC_E_ARG, "threads must be 1..64");
-    if (jit != 1 && jit != 2) return fail(SC_E_ARG, "jit must be 1 or 2");
-    std::vector<const double *> elems{mats};
-    std::vector<PlanOp> pops;
-    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
-    if (rc) return rc;
+    if (jit < 1 || jit > 3) return fail(SC_E_ARG, "jit must be 1, 2 or 3");
     SweepPlan plan;
-    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost))) return rc;
+    int rc;
+    if (jit == 3) {  // the circuit's adjoint reverse sweep (parameterized ops carry param / gen)
+        std::vector<double> inv;
+        std::vector<int> slot;
+        int n_params = 0;
+        for (uint64_t i = 0; i < n_ops; i++) n_params = std::max(n_params, ops[i].param + 1);
+        if ((rc = adjoint_plan_host(num_qubits, ops, n_ops, mats, n_mat_complex, n_params, low_qubits, plan, inv,
+                                    slot)))
+            return rc;
+    } else {
+        std::vector<const double *> elems{mats};
+        std::vector<PlanOp> pops;
+        if ((rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops))) return rc;
+        if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost))) return rc;
+    }
     if (kernels) *kernels = plan.passes.size();
     return jit_compile_host(plan, jit, threads, seconds, code_bytes, dump_dir);
 }

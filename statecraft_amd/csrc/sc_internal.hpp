@@ -120,7 +120,7 @@ struct SweepOp {
     uint8_t ta, tb;  // SW_DIAGL: thread-index masks of the diagonal's bit sources
     int8_t oa, ob;   // SW_DIAGL: outer (global) qubit positions of the diagonal's bit sources, -1: none
     uint64_t octrl;  // outer (chunk-fixed) control mask, global qubit positions
-    uint64_t pad1;
+    uint64_t slot;   // SW_GRAD: gradient slot the inner product accumulates into (ta = generator)
 };
 static_assert(sizeof(SweepOp) == 32, "sweep op records are 32 bytes");
 

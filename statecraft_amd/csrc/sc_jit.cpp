@@ -220,6 +220,137 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     return s.str();
 }
 
+// Source of one adjoint pass: the pass applied to two states, psi and lambda, at once (every op on both, the
+// matrices shared), with each gradient probe's <lambda| G |psi> reduced per wavefront and accumulated per
+// workgroup in LDS.  Chunks move through one 64 KiB LDS buffer, psi then lambda.  Workgroup w serves batch
+// element e0 + w % nb and that element's chunks sub, sub + gsub, ... (sub = w / nb); at the end it writes
+// its sum for probe k to part[(slot_k * B + e) * gsub + sub].
+std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps) {
+    std::ostringstream s;
+    std::vector<uint64_t> probes;
+    for (int st = 0; st < ps.nstages; st++)
+        for (int o = ps.stages[st].op_begin; o < ps.stages[st].op_end; o++)
+            if (plan.ops[o].kind == SW_GRAD) probes.push_back(plan.ops[o].slot);
+    const int K = (int)probes.size();
+    s << "extern \"C\" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
+         "@K@(sc::v2d *__restrict__ psi, sc::v2d *__restrict__ lam, int n, int log_chunks,\n"
+         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int e0, int nb, int gsub,\n"
+         "    double *__restrict__ part, int B) {\n"
+         "  using namespace sc;\n"
+         "  __shared__ v2d lds[kChunk];\n"
+      << "  __shared__ double gacc[4][" << std::max(K, 1) << "][2];\n"
+      << "  char *const ldsb = (char *)lds;\n"
+         "  CV2 *mats = (CV2 *)mats_arg;\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         "  const int e = e0 + (int)(blockIdx.x % (uint32_t)nb);\n"
+         "  const uint64_t sub = blockIdx.x / (uint32_t)nb;\n"
+         "  const uint64_t nchunks = 1ULL << log_chunks;\n"
+         "  CV2 *emats = mats + (uint64_t)e * mat_stride;\n"
+         "  v2d *P = psi + ((uint64_t)e << n);\n"
+         "  v2d *L = lam + ((uint64_t)e << n);\n"
+      << "  for (int i = tid; i < " << 8 * std::max(K, 1) << "; i += kThreads) (&gacc[0][0][0])[i] = 0.0;\n"
+      << "  auto cbase = [](uint64_t x) -> uint64_t { return " << deposit_expr("x", ps.outer_mask) << "; };\n";
+    s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, kThreadBits, "+", true) << ";\n";
+    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, kThreadBits, "^", false, 16) << ";\n";
+    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, kThreadBits, "^", false, 16) << ";\n";
+    auto emit_regs = [&](const char *fmt_l, const uint32_t *off, const char *base, const char *arr, bool write) {
+        (void)fmt_l;
+        for (int r = 0; r < kRegs; r++) {
+            if (write)
+                s << "    *(v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u << "u)) = " << arr << "[" << r << "];\n";
+            else
+                s << "    " << arr << "[" << r << "] = *(const v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u
+                  << "u));\n";
+        }
+    };
+    s << "  Regs a, p, b, q;\n"
+         "  __syncthreads();\n"
+         "  for (uint64_t cl = sub; cl < nchunks; cl += (uint64_t)gsub) {\n"
+         "    const uint64_t cb = cbase(cl);\n";
+    // load phase of one state into stage 0's layout (or straight to the store layout when there are no stages)
+    auto load_state = [&](const char *base_ptr, const char *arr) {
+        s << "    {\n    const v2d *src = " << base_ptr << " + cb + opaque(gthread);\n";
+        for (int r = 0; r < kRegs; r++) s << "    " << arr << "[" << r << "] = src[" << ps.glob_reg[r] << "];\n";
+        emit_regs("", ps.io_reg_slot, "sthread", arr, true);
+        s << "    }\n    __syncthreads();\n";
+        if (ps.nstages > 0) {
+            const SweepStage &stg = ps.stages[0];
+            s << "    {\n    const uint32_t t = opaque(tid);\n    const uint32_t ts = "
+              << lane_expr("t", stg.thread_slot, kThreadBits, "^", false, 16) << ";\n";
+            emit_regs("", stg.lds_off, "ts", arr, false);
+            s << "    }\n    __syncthreads();\n";
+        }
+    };
+    load_state("P", "a");
+    load_state("L", "b");
+    bool in_a = true;  // current registers: a/b (else p/q)
+    int k = 0;
+    for (int st = 0; st < ps.nstages; st++) {
+        const SweepStage &stg = ps.stages[st];
+        if (st > 0) {
+            // exchange both states from stage st-1's layout to stage st's
+            const SweepStage &prev = ps.stages[st - 1];
+            for (int which = 0; which < 2; which++) {
+                const char *arr = which == 0 ? (in_a ? "a" : "p") : (in_a ? "b" : "q");
+                s << "    {\n    const uint32_t t = opaque(tid);\n    const uint32_t tw = "
+                  << lane_expr("t", prev.thread_slot, kThreadBits, "^", false, 16) << ";\n";
+                emit_regs("", prev.lds_off, "tw", arr, true);
+                s << "    __syncthreads();\n    const uint32_t t2 = opaque(tid);\n    const uint32_t ts = "
+                  << lane_expr("t2", stg.thread_slot, kThreadBits, "^", false, 16) << ";\n";
+                emit_regs("", stg.lds_off, "ts", arr, false);
+                s << "    __syncthreads();\n    }\n";
+            }
+        }
+        for (int o = stg.op_begin; o < stg.op_end; o++) {
+            const SweepOp &op = plan.ops[o];
+            const char *P0 = in_a ? "a" : "p", *P1 = in_a ? "p" : "a";
+            const char *L0 = in_a ? "b" : "q", *L1 = in_a ? "q" : "b";
+            if (op.kind == SW_GRAD) {
+                s << "    {\n    double re = 0.0, im = 0.0;\n    grad_acc<" << (int)op.ta << ", " << (int)op.k0 << ", "
+                  << (op.k1 >= 0 ? (int)op.k1 : 0) << ">(" << P0 << ", " << L0 << ", re, im);\n"
+                  << "    for (int m = 32; m >= 1; m >>= 1) { re += __shfl_xor(re, m); im += __shfl_xor(im, m); }\n"
+                  << "    if ((tid & 63u) == 0u) { gacc[tid >> 6][" << k << "][0] += re; gacc[tid >> 6][" << k
+                  << "][1] += im; }\n    }\n";
+                k++;
+                continue;
+            }
+            s << "    " << op_call(op, P0, P1) << "\n";
+            s << "    " << op_call(op, L0, L1) << "\n";
+            in_a = !in_a;
+        }
+    }
+    // store phase: each state through LDS into the store layout, then to HBM
+    const uint32_t *last_off = ps.nstages > 0 ? ps.stages[ps.nstages - 1].lds_off : ps.io_reg_slot;
+    for (int which = 0; which < 2; which++) {
+        const char *arr = which == 0 ? (in_a ? "a" : "p") : (in_a ? "b" : "q");
+        s << "    {\n";
+        if (ps.nstages > 0) {
+            const SweepStage &lst = ps.stages[ps.nstages - 1];
+            s << "    const uint32_t t = opaque(tid);\n    const uint32_t tw = "
+              << lane_expr("t", lst.thread_slot, kThreadBits, "^", false, 16) << ";\n";
+            emit_regs("", last_off, "tw", arr, true);
+        } else {
+            emit_regs("", ps.io_reg_slot, "sthread", arr, true);  // the registers still hold the load layout
+        }
+        s << "    __syncthreads();\n";
+        s << "    const uint32_t sl = opaque(sout);\n";
+        emit_regs("", ps.st_reg_slot, "sl", arr, false);
+        s << "    v2d *dst = " << (which == 0 ? "P" : "L") << " + cb + opaque(gthread);\n";
+        for (int r = 0; r < kRegs; r++) s << "    dst[" << ps.glob_reg[r] << "] = " << arr << "[" << r << "];\n";
+        s << "    __syncthreads();\n    }\n";
+    }
+    s << "  }\n";
+    // per-workgroup probe sums, in a fixed order over the wavefronts
+    s << "  __syncthreads();\n";
+    for (int j = 0; j < K; j++) {
+        s << "  if (tid < 2u) part[((uint64_t)" << probes[j] << "ULL * (uint64_t)B + (uint64_t)e) * (uint64_t)gsub * 2ULL"
+          << " + sub * 2ULL + tid] = ((gacc[0][" << j << "][tid] + gacc[1][" << j << "][tid]) + gacc[2][" << j
+          << "][tid]) + gacc[3][" << j << "][tid];\n";
+    }
+    s << "}\n";
+    return s.str();
+}
+
 std::string replace_name(const std::string &body, const std::string &name) {
     std::string out = body;
     const size_t at = out.find("@K@");
@@ -314,19 +445,21 @@ std::string jit_pass_source(const SweepPlan &plan, size_t pass, int variant) {
     return pass_source(plan, plan.passes[pass], variant == 2);
 }
 
-int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
+namespace {
+
+// Kernels for the given bodies ("@K@" names the kernel), compiled on first use and cached per device.
+int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vector<hipFunction_t> &out) {
     JitCache &jc = cache();
     std::lock_guard<std::mutex> lock(jc.mu);
     const std::string dev = std::to_string(ctx->device) + "\n";
-    std::vector<std::string> keys(plan.passes.size());
+    std::vector<std::string> keys(bodies.size());
     std::vector<std::string> missing;
     std::unordered_map<std::string, size_t> missing_at;
-    for (size_t p = 0; p < plan.passes.size(); p++) {
-        std::string body = pass_source(plan, plan.passes[p], ctx->jit == 2);
-        keys[p] = dev + body;
+    for (size_t p = 0; p < bodies.size(); p++) {
+        keys[p] = dev + bodies[p];
         if (!jc.fns.count(keys[p]) && !missing_at.count(keys[p])) {
             missing_at[keys[p]] = missing.size();
-            missing.push_back(std::move(body));
+            missing.push_back(bodies[p]);
         }
     }
     if (!missing.empty()) {
@@ -348,15 +481,40 @@ int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunctio
             }
         }
     }
-    out.resize(plan.passes.size());
-    for (size_t p = 0; p < plan.passes.size(); p++) out[p] = jc.fns.at(keys[p]);
+    out.resize(bodies.size());
+    for (size_t p = 0; p < bodies.size(); p++) out[p] = jc.fns.at(keys[p]);
     return SC_OK;
+}
+
+}  // namespace
+
+int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
+    std::vector<std::string> bodies;
+    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(pass_source(plan, plan.passes[p], ctx->jit == 2));
+    return jit_kernels(ctx, bodies, out);
+}
+
+std::vector<uint64_t> jit_pass_probes(const SweepPlan &plan, size_t pass) {
+    std::vector<uint64_t> slots;
+    const SweepPass &ps = plan.passes[pass];
+    for (int st = 0; st < ps.nstages; st++)
+        for (int o = ps.stages[st].op_begin; o < ps.stages[st].op_end; o++)
+            if (plan.ops[o].kind == SW_GRAD) slots.push_back(plan.ops[o].slot);
+    return slots;
+}
+
+int jit_adjoint_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
+    std::vector<std::string> bodies;
+    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(adjoint_pass_source(plan, plan.passes[p]));
+    return jit_kernels(ctx, bodies, out);
 }
 
 int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *seconds, uint64_t *code_bytes,
                      const char *dump_dir) {
     std::vector<std::string> bodies;
-    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(pass_source(plan, plan.passes[p], variant == 2));
+    for (size_t p = 0; p < plan.passes.size(); p++)
+        bodies.push_back(variant == 3 ? adjoint_pass_source(plan, plan.passes[p])
+                                      : pass_source(plan, plan.passes[p], variant == 2));
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::vector<char>> codes;
     std::vector<size_t> first;

@@ -20,8 +20,18 @@ std::string jit_pass_source(const SweepPlan &plan, size_t pass, int variant);
 int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out);
 // Compiles every pass of a plan without a device (timing, and the code objects for inspection when
 // dump_dir is set).
+// variant 3: the plan is an adjoint plan (gradient probes) and compiles as adjoint passes.
 int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *seconds, uint64_t *code_bytes,
                      const char *dump_dir);
+// The reverse sweep of an adjoint gradient over ops[0, n_ops) for one batch element, planned as sweep passes
+// (sc_adjoint.hip); inv receives the inverse matrices the plan's pool is built from, slot each op's
+// gradient slot (-1 for none).
+int adjoint_plan_host(int n, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                      int n_params, int low_qubits, SweepPlan &plan, std::vector<double> &inv, std::vector<int> &slot);
+// Adjoint passes (plans with gradient probes): one kernel per pass over psi and lambda; probe slots of a pass
+// in the order its kernel writes them.
+int jit_adjoint_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out);
+std::vector<uint64_t> jit_pass_probes(const SweepPlan &plan, size_t pass);
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
                      int elem_count, size_t mat_stride);
 // Workgroups of a sweep launch that are resident at once (occupancy x CUs, capped by the context option).

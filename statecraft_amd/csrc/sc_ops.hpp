@@ -50,6 +50,8 @@ enum SweepOpKind : int32_t {
     SW_DIAGL = 12,   // 4-entry diagonal m[2A + B]; A is register bit k0 (k0 >= 0) or the thread / outer bit
                      // (ta, oa); B is the thread / outer bit (tb, ob) or 0; applied where rctrl and tctrl are set
     SW_CX = 13,      // exchange on bit k0 where register bit k1 is set (CNOT inside the registers): moves only
+    SW_GRAD = 14,    // adjoint passes only: accumulate <lambda| G |psi> for the Pauli generator G (ta = SC_GEN_*)
+                     // on register bit k0 (and k1 for ZZ) into gradient slot pad1; the states are unchanged
 };
 
 // Amplitudes as a native two-element vector, so that register arrays of them stay in VGPRs.
@@ -411,6 +413,35 @@ __device__ __forceinline__ void run_op(const Regs &s, Regs &d, const Op &op, CV2
         break;
     }
     default: copy_all(s, d); break;
+    }
+}
+
+// ---- adjoint passes: <lambda| G |psi> over one thread's 16 register amplitudes, G a Pauli generator on
+// register bit K0 (GEN 1 = X, 2 = Y, 3 = Z) or Z on K0 and K1 (GEN 4 = ZZ).  re/im accumulate conj(lambda) * (G psi).
+__device__ __forceinline__ void cdot_acc(v2d a, v2d b, double &re, double &im) {
+    re = fma(a.x, b.x, re);
+    re = fma(a.y, b.y, re);
+    im = fma(a.x, b.y, im);
+    im = im - a.y * b.x;
+}
+
+template <int GEN, int K0, int K1>
+__device__ __forceinline__ void grad_acc(const Regs &psi, const Regs &lam, double &re, double &im) {
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) {
+        const int b0 = (r >> K0) & 1;
+        if (GEN == 1) {
+            cdot_acc(lam[r], psi[r ^ (1 << K0)], re, im);
+        } else if (GEN == 2) {  // Y psi: (-i psi1, i psi0)
+            const v2d p = psi[r ^ (1 << K0)];
+            cdot_acc(lam[r], b0 ? make_v2d(-p.y, p.x) : make_v2d(p.y, -p.x), re, im);
+        } else if (GEN == 3) {
+            const v2d p = psi[r];
+            cdot_acc(lam[r], b0 ? make_v2d(-p.x, -p.y) : p, re, im);
+        } else {
+            const v2d p = psi[r];
+            cdot_acc(lam[r], (b0 ^ ((r >> K1) & 1)) ? make_v2d(-p.x, -p.y) : p, re, im);
+        }
     }
 }
 

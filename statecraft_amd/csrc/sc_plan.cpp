@@ -439,6 +439,7 @@ void host_apply(const SweepOp &op, const double *m, double *a, uint32_t tid, uin
 // workgroup and stay outside; diagonal factors and controls of diagonal ops may come from thread-index bits
 // at little cost.
 uint64_t reg_need(const PlanOp &op, uint64_t local) {
+    if (op.kind == kPlanGrad) return op.targets | op.diag;  // probes read their generator's qubits per register
     if (op.is_diag || op.controls == 0) return op.targets;
     const uint64_t both = op.targets | (op.controls & local);
     return popc(both) <= kRegBits ? both : op.targets;
@@ -486,9 +487,13 @@ double fused_cost(int bits, bool real, bool diag, const SweepCostModel &cm) {
     return (real ? 0.5 * c : c) + cm.op_overhead;
 }
 
+// Qubits an op needs among a pass's local qubits: its targets (and a probe's generator qubits).
+uint64_t local_need(const PlanOp &op) { return op.kind == kPlanGrad ? (op.targets | op.diag) : op.targets; }
+
 // FP64 operations per amplitude an op is expected to cost after fusion: diagonals share tables (a
 // quarter of one each), exchanges are free, real matrices cost half.
 double plan_op_fp64(const PlanOp &op) {
+    if (op.kind == kPlanGrad) return 4.0;
     if (op.is_x || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP) return 0.0;
     if (op.is_diag || op.kind == SC_OP_DIAG2) return 1.0;
     if (op.kind == SC_OP_TWO) return op.is_real ? 8.0 : 16.0;
@@ -599,7 +604,8 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             for (int i : remaining) {
                 const PlanOp &op = ops[i];
                 const double w = plan_op_fp64(op);
-                const bool fits = grow ? popc(LL | op.targets) <= kChunkBits : (op.targets & ~LL) == 0;
+                const uint64_t need = local_need(op);
+                const bool fits = grow ? popc(LL | need) <= kChunkBits : (need & ~LL) == 0;
                 if (b.blocked(op) || !fits || popc(op.targets) > kRegBits ||
                     (cm.pass_cap > 0.0 && w > 0.0 && wk + w > cm.pass_cap)) {
                     b.mark(op);
@@ -608,7 +614,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 }
                 wk += w;
                 cnt++;
-                if (grow) LL |= op.targets;
+                if (grow) LL |= need;
                 if (s_out) s_out->push_back(i);
             }
             if (L_out) *L_out = LL;
@@ -621,7 +627,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             // hill climbing: exchange one free local qubit for an outer target qubit while that lets the
             // pass take on more ops (on layered circuits this halves the number of passes)
             uint64_t cand = 0;
-            for (int i : remaining) cand |= ops[i].targets;
+            for (int i : remaining) cand |= local_need(ops[i]);
             int best = take(L, false, nullptr, nullptr, nullptr);
             for (int iter = 0; iter < 4 * kChunkBits; iter++) {
                 bool improved = false;
@@ -1043,6 +1049,13 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 const PlanOp &op = ops[oi];
                 SweepOp so = new_op();
                 switch (op.kind) {
+                case kPlanGrad:
+                    so.kind = SW_GRAD;
+                    so.k0 = (int8_t)reg_of(op.q0);
+                    so.k1 = (int8_t)(op.q1 >= 0 ? reg_of(op.q1) : -1);
+                    so.ta = (uint8_t)op.gen;
+                    so.slot = (uint64_t)op.mat;
+                    break;
                 case SC_OP_SINGLE:
                 case SC_OP_MCU:
                     add_controls(so, op.controls);
@@ -1245,7 +1258,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         const bool two = so.kind == SW_U2 || so.kind == SW_U2R || so.kind == SW_SWAP;
         const bool one = so.kind == SW_U1 || so.kind == SW_U1R || so.kind == SW_U1C || so.kind == SW_X ||
                          so.kind == SW_XL;
-        bool ok = so.kind >= SW_U1 && so.kind <= SW_CX;
+        bool ok = so.kind >= SW_U1 && so.kind <= SW_GRAD;
+        if (so.kind == SW_GRAD)
+            ok = so.k0 >= 0 && so.k0 < kRegBits && so.k1 < kRegBits && so.k1 != so.k0 && so.mat < 0 &&
+                 so.ta >= SC_GEN_X && so.ta <= SC_GEN_ZZ && ((so.ta == SC_GEN_ZZ) == (so.k1 >= 0));
         if (one) ok = ok && so.k0 >= 0 && so.k0 < kRegBits;
         if (two) ok = ok && so.k1 >= 0 && so.k0 > so.k1 && so.k0 < kRegBits;
         if (so.kind == SW_CX) ok = ok && so.k0 >= 0 && so.k0 < kRegBits && so.k1 >= 0 && so.k1 < kRegBits &&
@@ -1276,12 +1292,40 @@ void host_run_op(const SweepOp &op, const double *pool, double *a, uint32_t tid,
     host_apply(op, m, a, tid, chunk_base);
 }
 
+// <lambda| G |psi> of a probe over one thread's register amplitudes (conj(lambda) * (G psi)).
+void host_grad(const SweepOp &op, const double *psi, const double *lam, double *acc) {
+    for (int r = 0; r < kRegs; r++) {
+        const int b0 = (r >> op.k0) & 1;
+        double pr, pi;
+        switch (op.ta) {
+        case SC_GEN_X: pr = psi[2 * (r ^ (1 << op.k0))], pi = psi[2 * (r ^ (1 << op.k0)) + 1]; break;
+        case SC_GEN_Y: {  // Y psi: (-i psi1, i psi0)
+            const double xr = psi[2 * (r ^ (1 << op.k0))], xi = psi[2 * (r ^ (1 << op.k0)) + 1];
+            pr = b0 ? -xi : xi;
+            pi = b0 ? xr : -xr;
+            break;
+        }
+        case SC_GEN_Z: pr = b0 ? -psi[2 * r] : psi[2 * r], pi = b0 ? -psi[2 * r + 1] : psi[2 * r + 1]; break;
+        default: {
+            const bool neg = (b0 ^ ((r >> op.k1) & 1)) != 0;
+            pr = neg ? -psi[2 * r] : psi[2 * r];
+            pi = neg ? -psi[2 * r + 1] : psi[2 * r + 1];
+        }
+        }
+        const double lr = lam[2 * r], li = lam[2 * r + 1];
+        acc[0] += lr * pr + li * pi;
+        acc[1] += lr * pi - li * pr;
+    }
+}
+
 }  // namespace
 
-void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state) {
+void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state, double *lam, double *grads) {
     const int log_chunks = n - kChunkBits;
-    std::vector<double> lds(2 * kChunk);
-    double a[2 * kRegs];
+    const int ns = lam ? 2 : 1;
+    double *states[2] = {state, lam};
+    std::vector<double> lds[2] = {std::vector<double>(2 * kChunk), std::vector<double>(2 * kChunk)};
+    double a[2][2 * kRegs];
     for (const SweepPass &pass : plan.passes) {
         for (uint64_t chunk = 0; chunk < (1ULL << log_chunks); chunk++) {
             uint64_t chunk_base = 0, om = pass.outer_mask, c = chunk;
@@ -1291,47 +1335,58 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 c >>= 1;
                 om &= om - 1ULL;
             }
-            for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
-                uint64_t g = 0;
-                uint32_t s = 0;
-                for (int i = 0; i < kThreadBits; i++)
-                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
-                for (int r = 0; r < kRegs; r++) {
-                    uint64_t gi = chunk_base + g + pass.glob_reg[r];
-                    uint32_t sl = s ^ pass.io_reg_slot[r];
-                    lds[2 * sl] = state[2 * gi];
-                    lds[2 * sl + 1] = state[2 * gi + 1];
+            for (int k = 0; k < ns; k++)
+                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                    uint64_t g = 0;
+                    uint32_t s = 0;
+                    for (int i = 0; i < kThreadBits; i++)
+                        if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
+                    for (int r = 0; r < kRegs; r++) {
+                        uint64_t gi = chunk_base + g + pass.glob_reg[r];
+                        uint32_t sl = s ^ pass.io_reg_slot[r];
+                        lds[k][2 * sl] = states[k][2 * gi];
+                        lds[k][2 * sl + 1] = states[k][2 * gi + 1];
+                    }
                 }
-            }
             for (int st = 0; st < pass.nstages; st++) {
                 const SweepStage &stg = pass.stages[st];
                 for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
                     uint32_t ts = 0;
                     for (int i = 0; i < kThreadBits; i++)
                         if ((tid >> i) & 1u) ts ^= stg.thread_slot[i];
-                    for (int r = 0; r < kRegs; r++) {
-                        uint32_t sl = ts ^ stg.lds_off[r];
-                        a[2 * r] = lds[2 * sl], a[2 * r + 1] = lds[2 * sl + 1];
+                    for (int k = 0; k < ns; k++)
+                        for (int r = 0; r < kRegs; r++) {
+                            uint32_t sl = ts ^ stg.lds_off[r];
+                            a[k][2 * r] = lds[k][2 * sl], a[k][2 * r + 1] = lds[k][2 * sl + 1];
+                        }
+                    for (int o = stg.op_begin; o < stg.op_end; o++) {
+                        const SweepOp &op = plan.ops[o];
+                        if (op.kind == SW_GRAD) {
+                            if (lam && grads) host_grad(op, a[0], a[1], grads + 2 * op.slot);
+                            continue;
+                        }
+                        for (int k = 0; k < ns; k++) host_run_op(op, pool, a[k], tid, chunk_base);
                     }
-                    for (int o = stg.op_begin; o < stg.op_end; o++) host_run_op(plan.ops[o], pool, a, tid, chunk_base);
-                    for (int r = 0; r < kRegs; r++) {
-                        uint32_t sl = ts ^ stg.lds_off[r];
-                        lds[2 * sl] = a[2 * r], lds[2 * sl + 1] = a[2 * r + 1];
-                    }
+                    for (int k = 0; k < ns; k++)
+                        for (int r = 0; r < kRegs; r++) {
+                            uint32_t sl = ts ^ stg.lds_off[r];
+                            lds[k][2 * sl] = a[k][2 * r], lds[k][2 * sl + 1] = a[k][2 * r + 1];
+                        }
                 }
             }
-            for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
-                uint64_t g = 0;
-                uint32_t s = 0;
-                for (int i = 0; i < kThreadBits; i++)
-                    if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.st_thread_slot[i];
-                for (int r = 0; r < kRegs; r++) {
-                    uint64_t gi = chunk_base + g + pass.glob_reg[r];
-                    uint32_t sl = s ^ pass.st_reg_slot[r];
-                    state[2 * gi] = lds[2 * sl];
-                    state[2 * gi + 1] = lds[2 * sl + 1];
+            for (int k = 0; k < ns; k++)
+                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                    uint64_t g = 0;
+                    uint32_t s = 0;
+                    for (int i = 0; i < kThreadBits; i++)
+                        if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.st_thread_slot[i];
+                    for (int r = 0; r < kRegs; r++) {
+                        uint64_t gi = chunk_base + g + pass.glob_reg[r];
+                        uint32_t sl = s ^ pass.st_reg_slot[r];
+                        states[k][2 * gi] = lds[k][2 * sl];
+                        states[k][2 * gi + 1] = lds[k][2 * sl + 1];
+                    }
                 }
-            }
         }
     }
 }
@@ -1364,7 +1419,7 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
     for (const PlanOp &op : ops) {
         const int32_t f[4] = {op.kind, op.q0, op.q1, op.mat};
         const uint64_t m[3] = {op.controls, op.targets, op.diag};
-        const uint8_t c = (uint8_t)((op.is_x ? 1 : 0) | (op.is_diag ? 2 : 0) | (op.is_real ? 4 : 0));
+        const uint8_t c = (uint8_t)((op.is_x ? 1 : 0) | (op.is_diag ? 2 : 0) | (op.is_real ? 4 : 0) | (op.gen << 3));
         put(f, sizeof f);
         put(m, sizeof m);
         put(&c, 1);

@@ -22,7 +22,12 @@ struct PlanOp {
     bool is_diag = false;   // whole op is diagonal
     bool is_real = false;   // every matrix entry has a zero imaginary part (all batch elements)
     int src = -1;           // index into the caller's op array
+    int gen = 0;            // kPlanGrad: SC_GEN_* of the probe; mat holds its gradient slot
 };
+
+// Adjoint passes: a probe that accumulates <lambda| G |psi> for the Pauli generator G on q0 (and q1 for ZZ).
+// It leaves both states unchanged; its qubits must be register bits of its stage.
+constexpr int kPlanGrad = 100;
 
 // Validate ops against n qubits (QuantumCircuit.Operation.validateTargets, QuantumCircuit.java:362-574) and
 // classify them.  `mats_all` lists every batch element's matrix array (one entry when shared) so diagonal /
@@ -109,7 +114,9 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS
-// slots, per-thread register sets, stage ops, store phase).  Used to validate plans on the CPU.
-void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state);
+// slots, per-thread register sets, stage ops, store phase).  Used to validate plans on the CPU.  With lam, an
+// adjoint plan runs every op on both states and adds each probe's <lambda| G |psi> to grads[2 * slot].
+void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state, double *lam = nullptr,
+                       double *grads = nullptr);
 
 }  // namespace sc

@@ -120,13 +120,23 @@ def test_pauli_expectations(hip_ctx):
     st.close()
 
 
+@pytest.fixture(params=[0, 2], ids=["per-op", "adjoint-passes"])
+def adj_ctx(hip_ctx, request):
+    """The reverse sweep per op (SC_OPT_JIT 0), or as runtime-compiled adjoint passes (n >= 12)."""
+    hip_ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_SWEEP)
+    hip_ctx.set_option(capi.OPT_JIT, request.param)
+    yield hip_ctx
+    hip_ctx.set_option(capi.OPT_JIT, 0)
+
+
 def hea_energy(n, layers, thetas, obs):
     ops, mats = lower(workloads.hea(n, layers, thetas))
     return oracle.expect_pauli(oracle.run_ops(n, ops, mats), *obs)
 
 
 @pytest.mark.parametrize("n,layers,ckpt", [(6, 2, 0), (12, 2, 10), (13, 3, 25)])
-def test_adjoint_hea_vs_parameter_shift(hip_ctx, n, layers, ckpt):
+def test_adjoint_hea_vs_parameter_shift(adj_ctx, n, layers, ckpt):
+    hip_ctx = adj_ctx
     rng = np.random.default_rng(n)
     P = 2 * n * layers
     B = 2
@@ -154,7 +164,8 @@ def test_adjoint_hea_vs_parameter_shift(hip_ctx, n, layers, ckpt):
             assert abs(grads[b, p] - fd) <= 1e-7, (b, p)
 
 
-def test_adjoint_checkpoint_reanchor(hip_ctx):
+def test_adjoint_checkpoint_reanchor(adj_ctx):
+    hip_ctx = adj_ctx
     n, layers = 12, 2
     rng = np.random.default_rng(3)
     P = 2 * n * layers
@@ -176,7 +187,8 @@ def test_adjoint_checkpoint_reanchor(hip_ctx):
     st.close()
 
 
-def test_adjoint_qaoa_vs_finite_differences(hip_ctx):
+def test_adjoint_qaoa_vs_finite_differences(adj_ctx):
+    hip_ctx = adj_ctx
     n, p = 12, 2
     rng = np.random.default_rng(44)
     gam, bet = rng.uniform(0, math.pi, p), rng.uniform(0, math.pi, p)
@@ -205,3 +217,31 @@ def test_adjoint_qaoa_vs_finite_differences(hip_ctx):
                 bm[l] -= h
             fd = (energy(gp, bp) - energy(gm, bm)) / (2 * h)
             assert abs(grads[0, 2 * l + which] - fd) <= 1e-7
+
+
+def test_adjoint_passes_equal_per_op_reverse_sweep(hip_ctx):
+    """Runtime-compiled adjoint passes and the per-op reverse sweep give the same gradients and energies
+    (batch of 3 parameter sets, checkpoints every 40 ops, 14 qubits)."""
+    n, layers, B = 14, 3, 3
+    rng = np.random.default_rng(99)
+    P = 2 * n * layers
+    thetas = rng.uniform(0, 2 * math.pi, size=(B, P))
+    obs = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
+    ops, _ = lower(workloads.hea(n, layers, thetas[0]))
+    mats = np.stack([lower(workloads.hea(n, layers, thetas[b]))[1] for b in range(B)])
+    out = []
+    for jit in (0, 2):
+        hip_ctx.set_option(capi.OPT_JIT, jit)
+        try:
+            st = capi.State(hip_ctx, n, B)
+            out.append(st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=40, per_element=True))
+            final = st.download()
+            st.close()
+        finally:
+            hip_ctx.set_option(capi.OPT_JIT, 0)
+        for b in range(B):  # the state is left at the circuit's output
+            o, m = lower(workloads.hea(n, layers, thetas[b]))
+            assert np.max(np.abs(final[b] - oracle.run_ops(n, o, m))) <= 1e-12
+    (g0, e0), (g1, e1) = out
+    assert np.max(np.abs(g1 - g0)) <= 1e-12
+    assert np.max(np.abs(e1 - e0)) <= 1e-12

@@ -136,3 +136,64 @@ def test_local_set_search_matches_oracle(n, low, remap):
     for jit in (False, True):
         stats, out = capi.plan_sweeps_host(n, ops, mats, low, init, remap, jit)
         assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+
+
+def _pauli_apply(psi, n, xm, zm, coeff):
+    """sum_t c_t X^x_t Z^z_t psi for X/Z-only terms (Z applied first)."""
+    idx = np.arange(1 << n)
+    out = np.zeros_like(psi)
+    for x, z, c in zip(xm, zm, coeff):
+        x, z = int(x), int(z)
+        assert x & z == 0
+        zp = np.where(np.array([bin(int(i) & z).count("1") & 1 for i in idx]) == 1, -1.0, 1.0) * psi
+        out += c * zp[idx ^ x]
+    return out
+
+
+@pytest.mark.parametrize("kind,n", [("hea", 12), ("hea", 13), ("qaoa", 12)])
+def test_adjoint_plan_matches_parameter_shift(kind, n):
+    """The reverse sweep of an adjoint gradient as sweep passes (ops reversed with inverse matrices, a probe
+    before each parameterized op), run on the host with the adjoint kernels' data movement: every gradient
+    equals the parameter-shift rule on the oracle, and psi is uncomputed back to |0>."""
+    rng = np.random.default_rng(n + len(kind))
+    if kind == "hea":
+        layers = 2
+        P = 2 * n * layers
+        th = rng.uniform(0, 2 * np.pi, P)
+        build = lambda t: workloads.hea(n, layers, t)  # noqa: E731
+        xm, zm, coeff = workloads.hea_observable(n, rng.normal(size=n), rng.normal(size=n))
+        shift = np.pi / 2
+    else:
+        p, w = 2, rng.uniform(0.5, 1.5, n)
+        P = 2 * p
+        th = rng.uniform(0, np.pi, P)
+        build = lambda t: workloads.qaoa_ring(n, p, t[0::2], t[1::2], w)  # noqa: E731
+        _, xm, zm, coeff = workloads.maxcut_observable(n, w)
+        shift = None
+    ops, mats = lower(build(th))
+    psi = oracle.run_ops(n, ops, mats)
+    lam = _pauli_apply(psi, n, xm, zm, coeff)
+    g, psi0, _ = capi.adjoint_plan_host(n, ops, mats, P, psi, lam)
+    zero = np.zeros(1 << n, complex)
+    zero[0] = 1.0
+    assert np.max(np.abs(psi0 - zero)) <= 1e-12
+
+    def energy(t):
+        o, m = lower(build(t))
+        s = oracle.run_ops(n, o, m)
+        return np.vdot(s, _pauli_apply(s, n, xm, zm, coeff)).real
+
+    for k in range(0, P, max(1, P // 6)):
+        if shift is not None:
+            tp, tm = th.copy(), th.copy()
+            tp[k] += shift
+            tm[k] -= shift
+            want = 0.5 * (energy(tp) - energy(tm))
+        else:  # several gates share a QAOA parameter with weights: central difference
+            h = 1e-6
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            want = (energy(tp) - energy(tm)) / (2 * h)
+        tol = 1e-10 if shift is not None else 1e-6
+        assert abs(g[k] - want) <= tol, (k, g[k], want)
