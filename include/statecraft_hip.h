@@ -186,6 +186,11 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
  * on the same qubit, with the i factor accounted).  out[b] = <psi_b|H|psi_b> for every batch element. */
 int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
                          int n_terms, double *out);
+/* Every term on its own: out[b * n_terms + t] = <psi_b|P_t|psi_b> (real: each P_t is the Hermitian Pauli
+ * string of its masks).  Terms sharing an x-mask are evaluated together in one read of the state (up to
+ * 32 per read), so a diagonal observable of many terms costs one pass. */
+int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
+                               double *out);
 /* Adjoint gradient of E(theta) = <psi(theta)|H|psi(theta)>.  On entry `st` holds the initial states; the
  * call runs the circuit forward (leaving psi_b = U(theta_b)|init_b> in `st`), then sweeps the bra
  * lambda = H psi and the ket psi backwards through U^dagger with one inner-product reduction per

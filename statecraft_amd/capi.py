@@ -78,6 +78,7 @@ EXPORTED = [
     "sc_state_set_basis", "sc_state_upload", "sc_state_download", "sc_state_copy", "sc_apply_ops",
     "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
     "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_expectation_pauli",
+    "sc_expectation_pauli_terms",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
 ]
@@ -123,6 +124,7 @@ def _sig(lib):
         "sc_apply_kraus": (i32, [P, i32, i32, DP, DP]),
         "sc_run_depolarizing_trajectories": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64]),
         "sc_expectation_pauli": (i32, [P, U64P, U64P, DP, i32, DP]),
+        "sc_expectation_pauli_terms": (i32, [P, U64P, U64P, i32, DP]),
         "sc_adjoint_gradient": (i32, [P, P, u64, DP, u64, i32, U64P, U64P, DP, i32, i32, i32, DP, DP]),
         "sc_adjoint_stats": (i32, [P, U64P, DP]),
         "sc_profile_reset": (i32, [P]),
@@ -379,6 +381,16 @@ class State:
         out = np.empty(self.batch)
         check(lib().sc_expectation_pauli(self.h, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P), dptr(c), len(c),
                                          dptr(out)))
+        return out
+
+    def expectation_terms(self, xmask, zmask) -> np.ndarray:
+        """Per-term expectations <P_t> of every batch element: array [batch, n_terms]."""
+        xm = np.ascontiguousarray(xmask, np.uint64)
+        zm = np.ascontiguousarray(zmask, np.uint64)
+        assert xm.shape == zm.shape
+        out = np.empty((self.batch, len(zm)))
+        check(lib().sc_expectation_pauli_terms(self.h, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P), len(zm),
+                                               dptr(out)))
         return out
 
     def adjoint_gradient(self, ops, mats, n_params: int, xmask, zmask, coeff, checkpoint_every: int = 0,

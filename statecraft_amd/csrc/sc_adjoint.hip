@@ -75,19 +75,81 @@ __device__ __forceinline__ uint64_t expand_mask(uint64_t x, uint64_t mask) {
     return x;
 }
 
-// sum_i conj(psi_{i^x}) (-1)^{|i&z|} psi_i per element: grid (kParts, batch), out[(elem*kParts+blk)*2]
-__global__ void __launch_bounds__(kBlock) k_pauli_partial(const double2 *__restrict__ amp, int n, uint64_t x,
-                                                          uint64_t z, double *__restrict__ out) {
-    const uint64_t elem = blockIdx.y;
+// Up to kTermsPerLaunch Pauli terms that share one x mask, evaluated in one read of the state.  With
+// q = conj(psi_{i^x}) psi_i, the index pair {i, i^x} contributes s(i) q + s(i^x) conj(q) to term t's
+// sum_i conj(psi_{i^x}) s(i) psi_i, s(i) = (-1)^{|i & z_t|}, and s(i^x) = s(i) (-1)^{|x & z_t|}: 2 s(i) Re q
+// when |x & z_t| is even, 2i s(i) Im q when odd.  The kernel therefore visits each pair once (index bit
+// ctz(x) clear) and accumulates s(i) Re q or s(i) Im q per term (bit t of `imag`); for x = 0 it visits every
+// index with q = |psi_i|^2.  The host applies the factor 2 and the sign of i^{|x & z_t|}.
+constexpr int kTermsPerLaunch = 32;
+struct TermGroup {
+    uint64_t x;
+    uint64_t z[kTermsPerLaunch];
+    int32_t row[kTermsPerLaunch];  // output row of term t
+    uint32_t imag;
+    int32_t nt;
+};
+
+// grid (parts, elements): part[((e0 + elem) * rows + row_t) * parts + block]
+__global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restrict__ amp, int n, uint64_t e0,
+                                                        TermGroup g, int rows, int parts, double *__restrict__ part) {
+    const uint64_t elem = e0 + blockIdx.y;
     const double2 *base = amp + (elem << n);
-    const uint64_t dim = 1ULL << n;
-    double re = 0.0, im = 0.0;
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < dim; i += (uint64_t)kParts * kBlock) {
-        double2 a = base[i ^ x], b = base[i];
-        if (__popcll(i & z) & 1) b = make_double2(-b.x, -b.y);
-        cdot_acc(a, b, re, im);
+    const bool pairs = g.x != 0;
+    const uint64_t count = pairs ? (1ULL << (n - 1)) : (1ULL << n);
+    const int pbit = pairs ? __builtin_ctzll(g.x) : 0;
+    double acc[kTermsPerLaunch];
+#pragma unroll
+    for (int t = 0; t < kTermsPerLaunch; t++) acc[t] = 0.0;
+    for (uint64_t h = blockIdx.x * (uint64_t)kBlock + threadIdx.x; h < count; h += (uint64_t)parts * kBlock) {
+        uint64_t i = h;
+        double qr, qi;
+        if (pairs) {
+            const uint64_t lo = h & ((1ULL << pbit) - 1ULL);
+            i = ((h ^ lo) << 1) | lo;
+            const double2 a = base[i], b = base[i ^ g.x];
+            qr = fma(b.x, a.x, b.y * a.y);
+            qi = fma(b.x, a.y, -b.y * a.x);
+        } else {
+            const double2 a = base[i];
+            qr = fma(a.x, a.x, a.y * a.y);
+            qi = 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < kTermsPerLaunch; t++) {
+            if (t < g.nt) {
+                const double v = ((g.imag >> t) & 1u) ? qi : qr;
+                acc[t] += (__popcll(i & g.z[t]) & 1) ? -v : v;
+            }
+        }
     }
-    block_reduce2(re, im, out + (elem * kParts + blockIdx.x) * 2);
+    // wavefront sums, then the workgroup's four wavefronts through LDS
+    __shared__ double sh[kBlock / 64][kTermsPerLaunch];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < kTermsPerLaunch; t++) {
+        if (t < g.nt) {
+            double v = acc[t];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sh[wave][t] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < g.nt) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / 64; w++) v += sh[w][threadIdx.x];
+        part[((elem * (uint64_t)rows) + (uint64_t)g.row[threadIdx.x]) * (uint64_t)parts + blockIdx.x] = v;
+    }
+}
+
+// out[r] = sum of the parts partial sums of row r
+__global__ void k_rows_final(const double *__restrict__ part, uint64_t rows, int parts, double *__restrict__ out) {
+    const uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    double v = 0.0;
+    for (int k = 0; k < parts; k++) v += part[r * (uint64_t)parts + k];
+    out[r] = v;
 }
 
 // sum_i conj(a_i) b_i per element
@@ -290,36 +352,91 @@ int finish_parts(Context *ctx, double *part, uint64_t count, double *host) {
     return SC_OK;
 }
 
-// <psi_b|H|psi_b> for all b: out[b]
-int expectation(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, const double *c,
-                int n_terms, double *out) {
+// <P_t>_b for every term t and element b: vals[b * n_terms + t], each term a Hermitian Pauli string
+// i^{|x & z|} X^x Z^z.  Terms are grouped by x mask; each launch evaluates up to kTermsPerLaunch terms of one
+// group in one read of the state, and one synchronisation ends the call.
+int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
+                double *vals) {
     const uint64_t B = (uint64_t)st->batch;
-    int rc = ensure_scratch(ctx, (size_t)B * (kParts + 1) * 2 * sizeof(double));
+    const int n = st->n;
+    const uint64_t rows = (uint64_t)n_terms;
+    // partial sums per row: enough workgroups to fill the GPU, at least 8 amplitudes per thread
+    const uint64_t half = n > 0 ? (1ULL << (n - 1)) : 1ULL;
+    uint64_t parts = std::max<uint64_t>(1, (4ULL * ctx->cu_count + B - 1) / B);
+    parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, half / (8ULL * kBlock)));
+    parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, (32ULL << 20) / (rows * B)));
+    parts = std::min<uint64_t>(parts, 4096);
+    const size_t part_bytes = (size_t)(B * rows * parts) * sizeof(double);
+    int rc = ensure_scratch(ctx, part_bytes + (size_t)(B * rows) * sizeof(double));
     if (rc) return rc;
     double *part = (double *)ctx->scratch;
-    std::vector<double> res(2 * B);
-    for (uint64_t b = 0; b < B; b++) out[b] = 0.0;
+    double *out = (double *)((char *)ctx->scratch + part_bytes);
+    // groups in order of first appearance
+    std::vector<uint64_t> gx;
+    std::vector<std::vector<int>> gterms;
     for (int t = 0; t < n_terms; t++) {
-        {
-            ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 16.0 * (double)(B << st->n));
-            hipLaunchKernelGGL(k_pauli_partial, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, amp, st->n,
-                               xm[t], zm[t], part);
-            SC_HIP_CHECK(hipGetLastError());
+        size_t k = 0;
+        while (k < gx.size() && gx[k] != xm[t]) k++;
+        if (k == gx.size()) {
+            gx.push_back(xm[t]);
+            gterms.emplace_back();
         }
-        if ((rc = finish_parts(ctx, part, B, res.data()))) return rc;
-        int ny = __builtin_popcountll(xm[t] & zm[t]) & 3;
-        for (uint64_t b = 0; b < B; b++) {
-            double re = res[2 * b], im = res[2 * b + 1];
-            double v = ny == 0 ? re : ny == 1 ? -im : ny == 2 ? -re : im;
-            out[b] += c[t] * v;
+        gterms[k].push_back(t);
+    }
+    for (size_t k = 0; k < gx.size(); k++) {
+        for (size_t c0 = 0; c0 < gterms[k].size(); c0 += kTermsPerLaunch) {
+            TermGroup g;
+            std::memset(&g, 0, sizeof(g));
+            g.x = gx[k];
+            g.nt = (int)std::min<size_t>(kTermsPerLaunch, gterms[k].size() - c0);
+            for (int j = 0; j < g.nt; j++) {
+                const int t = gterms[k][c0 + j];
+                g.z[j] = zm[t];
+                g.row[j] = t;
+                if (__builtin_popcountll(xm[t] & zm[t]) & 1) g.imag |= 1u << j;
+            }
+            for (uint64_t e0 = 0; e0 < B; e0 += 65535) {
+                const unsigned cnt = (unsigned)std::min<uint64_t>(65535, B - e0);
+                ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 16.0 * (double)((uint64_t)cnt << n));
+                hipLaunchKernelGGL(k_pauli_terms, dim3((unsigned)parts, cnt), dim3(kBlock), 0, ctx->stream, amp, n,
+                                   e0, g, (int)rows, (int)parts, part);
+                SC_HIP_CHECK(hipGetLastError());
+            }
         }
+    }
+    const uint64_t total_rows = B * rows;
+    hipLaunchKernelGGL(k_rows_final, dim3((unsigned)((total_rows + 255) / 256)), dim3(256), 0, ctx->stream, part,
+                       total_rows, (int)parts, out);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipMemcpyAsync(vals, out, total_rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int t = 0; t < n_terms; t++) {
+        const int ny = __builtin_popcountll(xm[t] & zm[t]) & 3;
+        const double f = (xm[t] != 0 ? 2.0 : 1.0) * ((ny == 1 || ny == 2) ? -1.0 : 1.0);
+        for (uint64_t b = 0; b < B; b++) vals[b * rows + t] *= f;
     }
     return SC_OK;
 }
 
-int check_terms(const State *st, const uint64_t *xm, const uint64_t *zm, const double *c, int n_terms) {
+// <psi_b|H|psi_b> for all b: out[b]
+int expectation(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, const double *c,
+                int n_terms, double *out) {
+    const uint64_t B = (uint64_t)st->batch;
+    std::vector<double> vals(B * (uint64_t)n_terms);
+    int rc = pauli_terms(ctx, st, amp, xm, zm, n_terms, vals.data());
+    if (rc) return rc;
+    for (uint64_t b = 0; b < B; b++) {
+        double e = 0.0;
+        for (int t = 0; t < n_terms; t++) e += c[t] * vals[b * (uint64_t)n_terms + t];
+        out[b] = e;
+    }
+    return SC_OK;
+}
+
+int check_terms(const State *st, const uint64_t *xm, const uint64_t *zm, const double *c, int n_terms,
+                bool need_coeff = true) {
     if (n_terms <= 0) return fail(SC_E_ARG, "observable must have at least one term");
-    if (!xm || !zm || !c) return fail(SC_E_NULL, "observable arrays are null");
+    if (!xm || !zm || (need_coeff && !c)) return fail(SC_E_NULL, "observable arrays are null");
     for (int t = 0; t < n_terms; t++)
         if (st->n < 64 && (((xm[t] | zm[t]) >> st->n) != 0))
             return fail(SC_E_ARG, "observable term " + std::to_string(t) + " acts outside the register");
@@ -424,7 +541,7 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
         if (!ent) {
             std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
             e->key = key;
-            if ((rc = plan_sweeps(n, seq, ctx->low_qubits, false, e->plan, max_fuse, kCompiledPassCost))) return rc;
+            if ((rc = plan_sweeps(n, seq, ctx->low_qubits, false, e->plan, max_fuse, kAdjointPassCost))) return rc;
             if (!e->plan.passes.empty() && (rc = jit_adjoint_kernels(ctx, e->plan, e->fns))) return rc;
             ent = ctx->plan_cache->insert(std::move(e));
         }
@@ -493,7 +610,7 @@ int adjoint_plan_host(int n, const sc_op *ops, uint64_t n_ops, const double *mat
     std::vector<PlanOp> seq;
     int rc = adjoint_sequence(n, ops, 0, n_ops, slot, inv_elems, n_mat_complex, seq);
     if (rc) return rc;
-    return plan_sweeps(n, seq, low_qubits, false, plan, 2, kCompiledPassCost);
+    return plan_sweeps(n, seq, low_qubits, false, plan, 2, kAdjointPassCost);
 }
 
 }  // namespace sc
@@ -510,6 +627,16 @@ int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zm
     if (rc) return rc;
     if ((rc = set_device(st->ctx))) return rc;
     return expectation(st->ctx, st, st->amp, xmask, zmask, coeff, n_terms, out);
+}
+
+int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
+                               double *out) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (!out) return fail(SC_E_NULL, "out is null");
+    int rc = check_terms(st, xmask, zmask, nullptr, n_terms, false);
+    if (rc) return rc;
+    if ((rc = set_device(st->ctx))) return rc;
+    return pauli_terms(st->ctx, st, st->amp, xmask, zmask, n_terms, out);
 }
 
 int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,

@@ -111,6 +111,11 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
           << ", " << (int)o.ob << ", tid, cb);";
         break;
     case SW_CX: s << "op_cx<" << k0 << ", " << k1 << ">(" << sd << ");"; break;
+    case SW_U1U:
+        s << "op_u1u<" << k0 << ", " << (o.ta & 1) << ", " << ((o.ta >> 1) & 1) << ", " << (int)o.tb << ">(" << sd
+          << ", " << m << ");";
+        break;
+    case SW_SCALE: s << "op_scale(" << sd << ", " << m << ");"; break;
     default: s << "copy_all(" << sd << ");"; break;
     }
     if (o.octrl == 0) return s.str();

@@ -51,6 +51,9 @@ enum SweepOpKind : int32_t {
                      // (ta, oa); B is the thread / outer bit (tb, ob) or 0; applied where rctrl and tctrl are set
     SW_CX = 13,      // exchange on bit k0 where register bit k1 is set (CNOT inside the registers): moves only
     SW_GRAD = 14,    // adjoint passes only: accumulate <lambda| G |psi> for the Pauli generator G (ta = SC_GEN_*)
+    SW_U1U = 15,     // runtime-compiled passes only: 2x2 with unit pivots on bit k0 (row r reads its pivot from
+                     // column bit r of ta; tb = 0 complex, 1 real, 2 imaginary off-pivot ratios m[0], m[1])
+    SW_SCALE = 16,   // runtime-compiled passes only: every register times the real factor m[0].x
                      // on register bit k0 (and k1 for ZZ) into gradient slot pad1; the states are unchanged
 };
 
@@ -140,6 +143,39 @@ __device__ __forceinline__ void op_u1r(const Regs &s, Regs &d, CV2 *m) {
         d[r0] = make_v2d(fma(g01, y.x, g00 * x.x), fma(g01, y.y, g00 * x.y));
         d[r1] = make_v2d(fma(g11, y.x, g10 * x.x), fma(g11, y.y, g10 * x.y));
     }
+}
+
+// 2x2 with unit pivots on register bit K: output row r is s[pivot] + x_r * s[other], where the pivot is
+// column C0 (row 0) or C1 (row 1) of the pair and x_r = m[r].  The pivot entries of the original matrix are
+// a common real factor that the pass applies elsewhere, once per amplitude.  FORM 1: real x_r (one FMA per
+// component), FORM 2: purely imaginary x_r (one FMA per component), FORM 0: complex (a complex multiply-add).
+template <int K, int C0, int C1, int FORM>
+__device__ __forceinline__ void op_u1u(const Regs &s, Regs &d, CV2 *m) {
+    const v2d x0 = m[0], x1 = m[1];
+#pragma unroll
+    for (int r0 = 0; r0 < kRegs; r0++) {
+        if (r0 & (1 << K)) continue;
+        const int r1 = r0 | (1 << K);
+        const v2d p0 = C0 ? s[r1] : s[r0], o0 = C0 ? s[r0] : s[r1];
+        const v2d p1 = C1 ? s[r1] : s[r0], o1 = C1 ? s[r0] : s[r1];
+        if (FORM == 1) {
+            d[r0] = make_v2d(fma(x0.x, o0.x, p0.x), fma(x0.x, o0.y, p0.y));
+            d[r1] = make_v2d(fma(x1.x, o1.x, p1.x), fma(x1.x, o1.y, p1.y));
+        } else if (FORM == 2) {
+            d[r0] = make_v2d(p0.x - x0.y * o0.y, fma(x0.y, o0.x, p0.y));
+            d[r1] = make_v2d(p1.x - x1.y * o1.y, fma(x1.y, o1.x, p1.y));
+        } else {
+            d[r0] = cmad(x0, o0, p0);
+            d[r1] = cmad(x1, o1, p1);
+        }
+    }
+}
+
+// Every register times the real factor m[0].x.
+__device__ __forceinline__ void op_scale(const Regs &s, Regs &d, CV2 *m) {
+    const double f = m[0].x;
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) d[r] = make_v2d(f * s[r].x, f * s[r].y);
 }
 
 // Exchange on register bit K in the pairs whose register index has every rctrl bit set; with LANE the exchange

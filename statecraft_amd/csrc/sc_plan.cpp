@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 
 namespace sc {
 
@@ -124,6 +125,32 @@ int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const
                 return fail(SC_E_ARG, "matrix offset out of range for op " + std::to_string(i));
             for (const double *m : mats_all)
                 for (int e = 0; e < cnt; e++) p.is_real = p.is_real && m[2 * ((int64_t)o.mat + e) + 1] == 0.0;
+        }
+        // unit-pivot form of an uncontrolled 2x2 (pivot column pairs in order of preference)
+        if (o.kind == SC_OP_SINGLE && cnt > 0) {
+            for (int cc : {2, 0, 3, 1}) {
+                const int c0 = cc & 1, c1 = (cc >> 1) & 1;
+                bool ok = true, real = true, imag = true;
+                for (const double *m : mats_all) {
+                    const double *g = m + 2 * (int64_t)o.mat;
+                    const double *p0 = g + 2 * c0, *o0 = g + 2 * (1 - c0);
+                    const double *p1 = g + 4 + 2 * c1, *o1 = g + 4 + 2 * (1 - c1);
+                    auto bounded = [](const double *ov, const double *pv) {
+                        const double a = std::hypot(ov[0], ov[1]);
+                        return std::isfinite(a) && a <= std::ldexp(std::fabs(pv[0]), 40);
+                    };
+                    ok = p0[1] == 0.0 && p1[1] == 0.0 && p0[0] == p1[0] && p0[0] != 0.0 && std::isfinite(p0[0]) &&
+                         bounded(o0, p0) && bounded(o1, p1);
+                    if (!ok) break;
+                    real = real && o0[1] == 0.0 && o1[1] == 0.0;
+                    imag = imag && o0[0] == 0.0 && o1[0] == 0.0;
+                }
+                if (ok) {
+                    p.unit = cc;
+                    p.unit_form = real ? 1 : imag ? 2 : 0;
+                    break;
+                }
+            }
         }
         // roles
         switch (o.kind) {
@@ -378,6 +405,23 @@ void host_apply(const SweepOp &op, const double *m, double *a, uint32_t tid, uin
         }
         break;
     }
+    case SW_U1U:
+        for (int r0 = 0; r0 < kRegs; r0++) {
+            if (r0 & (1 << op.k0)) continue;
+            const int r1 = r0 | (1 << op.k0);
+            for (int row = 0; row < 2; row++) {
+                const int c = (op.ta >> row) & 1;
+                const double *pv = a + 2 * (c ? r1 : r0), *ov = a + 2 * (c ? r0 : r1);
+                double t[2];
+                cm(m + 2 * row, ov, t);
+                const int dst = row ? r1 : r0;
+                out[2 * dst] = pv[0] + t[0], out[2 * dst + 1] = pv[1] + t[1];
+            }
+        }
+        break;
+    case SW_SCALE:
+        for (int r = 0; r < kRegs; r++) cm(m, a + 2 * r, out + 2 * r);
+        break;
     case SW_U2:
     case SW_U2R:
         for (int b = 0; b < kRegs; b++) {
@@ -457,6 +501,8 @@ double op_fp64(const SweepOp &op) {
     case SW_D4R: return 32.0;
     case SW_DIAG16: return 4.0;
     case SW_DIAGL: return 4.0;
+    case SW_U1U: return op.tb == 0 ? 4.0 : 2.0;
+    case SW_SCALE: return 2.0;
     default: return 0.0;
     }
 }
@@ -497,6 +543,7 @@ double plan_op_fp64(const PlanOp &op) {
     if (op.is_x || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP) return 0.0;
     if (op.is_diag || op.kind == SC_OP_DIAG2) return 1.0;
     if (op.kind == SC_OP_TWO) return op.is_real ? 8.0 : 16.0;
+    if (op.unit >= 0 && op.kind == SC_OP_SINGLE && !op.is_x && !op.is_diag) return op.unit_form == 0 ? 4.0 : 2.0;
     return op.is_real ? 4.0 : 8.0;
 }
 
@@ -507,6 +554,7 @@ uint32_t reg_support(const SweepOp &op) {
     case SW_U1R:
     case SW_U1C:
     case SW_X: return (1u << op.k0) | op.rctrl;
+    case SW_U1U: return 1u << op.k0;
     case SW_CX:
     case SW_U2:
     case SW_U2R:
@@ -526,6 +574,7 @@ uint32_t nondiag_bits(const SweepOp &op) {
     case SW_U1C:
     case SW_X:
     case SW_XL:
+    case SW_U1U:
     case SW_CX: return 1u << op.k0;
     case SW_U2:
     case SW_U2R:
@@ -542,7 +591,7 @@ bool register_internal(const SweepOp &op) {
     if (op.tctrl != 0u || op.octrl != 0ULL) return false;
     if (op.kind == SW_DIAGL) return op.ta == 0u && op.tb == 0u && op.oa < 0 && op.ob < 0 && op.k0 >= 0;
     return op.kind == SW_U1 || op.kind == SW_U1R || op.kind == SW_U1C || op.kind == SW_X || op.kind == SW_CX ||
-           op.kind == SW_U2 || op.kind == SW_U2R || op.kind == SW_SWAP;
+           op.kind == SW_U2 || op.kind == SW_U2R || op.kind == SW_SWAP || op.kind == SW_U1U;
 }
 
 }  // namespace
@@ -555,15 +604,16 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
     const int lb = std::max(0, std::min(low_qubits, kChunkBits));
     const uint64_t lowmask = (1ULL << lb) - 1ULL;
 
-    std::map<std::pair<int, int>, int> recipe_index;  // (kind, mat) -> recipe
-    auto recipe_for = [&](int kind, int mat, int count) -> int {
-        auto key = std::make_pair(kind, mat);
+    std::map<std::tuple<int, int, int>, int> recipe_index;  // (kind, mat, aux) -> recipe
+    auto recipe_for = [&](int kind, int mat, int count, int aux = 0) -> int {
+        auto key = std::make_tuple(kind, mat, aux);
         auto it = recipe_index.find(key);
         if (it != recipe_index.end()) return it->second;
         PoolRecipe r;
         r.kind = kind;
         r.mat = mat;
         r.count = count;
+        r.aux = aux;
         plan.pool.push_back(r);
         int idx = (int)plan.pool.size() - 1;
         recipe_index[key] = idx;
@@ -977,6 +1027,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             pass.st_reg_slot[r] = slot_of(l);
         }
         // ---- stage ops
+        std::vector<std::pair<int, int>> pass_scale;  // pivots of the pass's unit-pivot ops: (mat, column)
         for (size_t s = 0; s < stages.size(); s++) {
             SweepStage &stg = pass.stages[s];
             const StageLayout &lay = layouts[s];
@@ -1069,6 +1120,13 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                         so.mat = recipe_for(RC_DIAG1, op.mat, 4);
                     } else if (op.is_x) {
                         set_exchange(so, op.q0);
+                    } else if (cm.unit_1q && op.unit >= 0 && op.kind == SC_OP_SINGLE) {
+                        so.kind = SW_U1U;
+                        so.k0 = (int8_t)reg_of(op.q0);
+                        so.ta = (uint8_t)op.unit;
+                        so.tb = (uint8_t)op.unit_form;
+                        so.mat = recipe_for(RC_UNIT, op.mat, 2, op.unit);
+                        pass_scale.push_back({op.mat, op.unit & 1});
                     } else {
                         so.kind = (so.rctrl | so.tctrl) ? SW_U1C : (op.is_real ? SW_U1R : SW_U1);
                         so.k0 = (int8_t)reg_of(op.q0);
@@ -1117,7 +1175,9 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 }
                 }
                 sops.push_back(so);
-                sreal.push_back(op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP || op.is_x);
+                sreal.push_back(so.kind == SW_U1U ? op.unit_form == 1
+                                                  : (op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP ||
+                                                     op.is_x));
             }
             // ---- gather diagonals: a register diagonal moves forward to the next one when every op in
             // between leaves its register bits' values alone (acts non-diagonally only on other bits), so
@@ -1238,6 +1298,42 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             for (auto &so : fused) plan.ops.push_back(so);
             stg.op_end = (int32_t)plan.ops.size();
         }
+        // ---- the unit-pivot ops' common factor: every amplitude of the chunk is multiplied by it once, in
+        // the table of the first uncontrolled matrix op of the pass (a copy of its recipe, scaled), else by a
+        // scaling op at the end of the last stage.  The factor is real and commutes with every op.
+        if (!pass_scale.empty()) {
+            int target = -1;
+            for (int s = 0; s < pass.nstages && target < 0; s++)
+                for (int i = pass.stages[s].op_begin; i < pass.stages[s].op_end; i++) {
+                    const SweepOp &so = plan.ops[i];
+                    const bool full_matrix = so.kind == SW_U1 || so.kind == SW_U1R || so.kind == SW_U2 ||
+                                             so.kind == SW_U2R || so.kind == SW_D4 || so.kind == SW_D4R ||
+                                             so.kind == SW_DIAG16 || so.kind == SW_DIAGL;
+                    if (full_matrix && so.mat >= 0 && so.rctrl == 0 && so.tctrl == 0 && so.octrl == 0) {
+                        target = i;
+                        break;
+                    }
+                }
+            PoolRecipe r;
+            if (target >= 0) {
+                r = plan.pool[plan.ops[target].mat];
+            } else {
+                r.kind = RC_SCALE;
+                r.mat = -1;
+                r.count = 1;
+                SweepOp so;
+                std::memset(&so, 0, sizeof(so));
+                so.kind = SW_SCALE;
+                so.k0 = so.k1 = -1;
+                so.oa = so.ob = -1;
+                plan.ops.push_back(so);
+                pass.stages[pass.nstages - 1].op_end = (int32_t)plan.ops.size();
+                target = (int)plan.ops.size() - 1;
+            }
+            r.scale_srcs.insert(r.scale_srcs.end(), pass_scale.begin(), pass_scale.end());
+            plan.pool.push_back(r);
+            plan.ops[target].mat = (int)plan.pool.size() - 1;
+        }
         plan.passes.push_back(pass);
         for (int k = 0; k < kChunkBits; k++) {
             phys[out_q[k]] = lq[k];
@@ -1257,8 +1353,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // kernel contract: register bit positions in range, two-bit kinds ordered, one register diagonal source
         const bool two = so.kind == SW_U2 || so.kind == SW_U2R || so.kind == SW_SWAP;
         const bool one = so.kind == SW_U1 || so.kind == SW_U1R || so.kind == SW_U1C || so.kind == SW_X ||
-                         so.kind == SW_XL;
-        bool ok = so.kind >= SW_U1 && so.kind <= SW_GRAD;
+                         so.kind == SW_XL || so.kind == SW_U1U;
+        bool ok = so.kind >= SW_U1 && so.kind <= SW_SCALE;
+        if (so.kind == SW_U1U) ok = ok && so.ta < 4 && so.tb <= 2 && so.mat >= 0;
+        if (so.kind == SW_SCALE) ok = ok && so.mat >= 0;
         if (so.kind == SW_GRAD)
             ok = so.k0 >= 0 && so.k0 < kRegBits && so.k1 < kRegBits && so.k1 != so.k0 && so.mat < 0 &&
                  so.ta >= SC_GEN_X && so.ta <= SC_GEN_ZZ && ((so.ta == SC_GEN_ZZ) == (so.k1 >= 0));
@@ -1420,9 +1518,11 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
         const int32_t f[4] = {op.kind, op.q0, op.q1, op.mat};
         const uint64_t m[3] = {op.controls, op.targets, op.diag};
         const uint8_t c = (uint8_t)((op.is_x ? 1 : 0) | (op.is_diag ? 2 : 0) | (op.is_real ? 4 : 0) | (op.gen << 3));
+        const uint8_t u = (uint8_t)((op.unit + 1) | (op.unit_form << 3));
         put(f, sizeof f);
         put(m, sizeof m);
         put(&c, 1);
+        put(&u, 1);
     }
     return k;
 }
@@ -1454,6 +1554,17 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
             break;
         case RC_SWAPDIAG:  // 4-entry diagonal with the qubit order exchanged
             put(0, 0), put(1, 2), put(2, 1), put(3, 3);
+            break;
+        case RC_UNIT:  // off-pivot entry / real pivot, per row
+            for (int row = 0; row < 2; row++) {
+                const int c = (rc.aux >> row) & 1;
+                const double pv = g[2 * (2 * row + c)];
+                dst[2 * row] = g[2 * (2 * row + 1 - c)] / pv;
+                dst[2 * row + 1] = g[2 * (2 * row + 1 - c) + 1] / pv;
+            }
+            break;
+        case RC_SCALE:
+            dst[0] = 1.0, dst[1] = 0.0;
             break;
         default: break;
         }
@@ -1497,6 +1608,15 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
                 dst[2 * (row * d + c) + 1] = col[2 * reg_of(row) + 1];
             }
         }
+    }
+    // third pass: pass factors of unit-pivot ops (real pivots m[0][c])
+    for (size_t r = 0; r < plan.pool.size(); r++) {
+        const PoolRecipe &rc = plan.pool[r];
+        if (rc.scale_srcs.empty()) continue;
+        double f = 1.0;
+        for (const auto &src : rc.scale_srcs) f *= mats[2 * ((int64_t)src.first + src.second)];
+        double *dst = out + 2 * offs[r];
+        for (int k = 0; k < 2 * rc.count; k++) dst[k] *= f;
     }
 }
 

@@ -23,6 +23,12 @@ struct PlanOp {
     bool is_real = false;   // every matrix entry has a zero imaginary part (all batch elements)
     int src = -1;           // index into the caller's op array
     int gen = 0;            // kPlanGrad: SC_GEN_* of the probe; mat holds its gradient slot
+    // Uncontrolled 2x2 with a unit-pivot form for every batch element: row r of the matrix has a real pivot
+    // entry in column bit r of `unit` (both pivots equal, nonzero, and the other entries at most 2^40 times
+    // larger), so M = p * M' with M' unit on its pivots.  -1: no such form.  unit_form: 0 complex, 1 real,
+    // 2 purely imaginary off-pivot entries.
+    int unit = -1;
+    int unit_form = 0;
 };
 
 // Adjoint passes: a probe that accumulates <lambda| G |psi> for the Pauli generator G on q0 (and q1 for ZZ).
@@ -57,6 +63,8 @@ enum RecipeKind : int {
     RC_SWAP4 = 3,     // the 4x4 at mat with its two qubits exchanged
     RC_SWAPDIAG = 4,  // the 4-entry diagonal at mat with its two qubits exchanged
     RC_DIAG16 = 5,    // diagonal of the fused (diagonal) fused_ops over the 16 register indices
+    RC_UNIT = 6,      // (m[0][1-c0] / m[0][c0], m[1][1-c1] / m[1][c1]) of the 2x2 at mat, pivot columns aux
+    RC_SCALE = 7,     // one entry 1 (then scaled)
 };
 
 struct PoolRecipe {
@@ -66,6 +74,10 @@ struct PoolRecipe {
     std::vector<SweepOp> fused_ops;  // RC_FUSED / RC_DIAG16: register ops in order (.mat = recipe index)
     std::vector<int> fused_srcs;     // RC_FUSED / RC_DIAG16: per fused op, recipe index of its matrix (-1 none)
     std::vector<int> fused_bits;     // RC_FUSED: register bits of the block, most significant index bit first
+    int aux = 0;                     // RC_UNIT: pivot column bits
+    // Every entry is multiplied by the product of the real pivots m[0][c] of these (mat, c) sources: the
+    // common factors of a pass's unit-pivot ops, applied once in the pass.
+    std::vector<std::pair<int, int>> scale_srcs;
 };
 
 struct SweepPlan {
@@ -83,10 +95,13 @@ struct SweepCostModel {
     double op_overhead;  // fixed cost of every register op
     bool const_ops;      // ops are compile-time constants (runtime-compiled pass kernels)
     double pass_cap;     // most estimated FP64 operations per amplitude a pass takes on (0: no limit)
+    bool unit_1q;        // single-qubit ops with a unit-pivot form use it (SW_U1U) and fold their pivots
 };
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
-constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0};
-constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0};
+constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false};
+constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true};
+// Adjoint passes: a probe between a unit-pivot op and the op its pivots fold into would see scaled states.
+constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, false};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).

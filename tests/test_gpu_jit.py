@@ -7,7 +7,7 @@ import pytest
 
 import bruteforce as bf
 import oracle
-from circuits import random_circuit, random_state
+from circuits import random_circuit, random_state, unit_pivot_circuit
 from statecraft_amd import capi, workloads
 from statecraft_amd.circuit import lower
 from test_plan_host import _diag_real_circuit
@@ -121,3 +121,41 @@ def test_jit_random_layered_20q(jit_ctx):
     ops, mats = lower(workloads.random_layered(n, 20, 7))
     got = run(jit_ctx, n, ops, mats)
     assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS
+
+
+@pytest.mark.parametrize("n,general,diagonals", [(13, False, False), (15, True, True)])
+def test_jit_unit_pivot_forms(jit_ctx, n, general, diagonals):
+    """Ry / Rx / H as unit-pivot ops with the pass's common factor in a table, a dense op or a scaling op."""
+    rng = np.random.default_rng(40 + n)
+    ops, mats = unit_pivot_circuit(n, 150, rng, general, diagonals)
+    init = random_state(n, rng)
+    got = run(jit_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+
+
+def test_jit_batched_unit_pivots_per_element(jit_ctx):
+    """Per-element matrices: the unit-pivot form is used only where it holds for every element.  Op 0 is Y in
+    element 1 (zero pivots) and Ry elsewhere, so it keeps the general form; the others differ in angle only."""
+    rng = np.random.default_rng(77)
+    n, B = 14, 3
+    ops, mats0 = unit_pivot_circuit(n, 120, rng, False, True)
+    ops[0]["kind"], ops[0]["q0"], ops[0]["q1"], ops[0]["mat"] = capi.OP_SINGLE, 3, -1, len(mats0)
+    mats = []
+    for b in range(B):
+        m = np.array(mats0, dtype=np.complex128)
+        single = np.nonzero(ops["kind"] == capi.OP_SINGLE)[0]
+        for i in single[1:]:
+            o = int(ops[i]["mat"])
+            g = m[o:o + 4]
+            if g[1] != 0 and g[0] == g[3] and g[0].imag == 0:  # Ry / Rx: another angle per element
+                t = 0.37 * (b + 1)
+                c, s_ = np.cos(t), np.sin(t)
+                rot = np.array([[c, -s_], [s_, c]])
+                m[o:o + 4] = (g.reshape(2, 2) @ rot if g[1].imag == 0 else g.reshape(2, 2)).reshape(-1)
+        extra = np.array([0, -1j, 1j, 0]) if b == 1 else np.array([0.6, -0.8, 0.8, 0.6])
+        mats.append(np.concatenate([m, extra]))
+    mats = np.stack(mats)
+    init = np.stack([random_state(n, rng) for _ in range(B)])
+    got = run(jit_ctx, n, ops, mats, init, batch=B, batched=True)
+    for b in range(B):
+        assert np.max(np.abs(got[b] - oracle.run_ops(n, ops, mats[b], init[b]))) <= EPS, b

@@ -120,6 +120,29 @@ def test_pauli_expectations(hip_ctx):
     st.close()
 
 
+@pytest.mark.parametrize("n,B", [(13, 3), (20, 2)])
+def test_pauli_terms_each_term(hip_ctx, n, B):
+    """Per-term expectations: 35 diagonal terms (two launches of one x-group), repeated x masks, Y factors."""
+    rng = np.random.default_rng(n)
+    states = np.stack([random_state(n, rng) for _ in range(B)])
+    full = (1 << n) - 1
+    xs = [0] * 35 + [int(x) for x in rng.integers(1, 1 << n, 6)] * 2 + [1 << (n - 1), 1]
+    zs = [int(z) for z in rng.integers(0, 1 << n, len(xs))]
+    zs[0], zs[1], zs[-1] = 0, full, full
+    st = capi.State(hip_ctx, n, B)
+    st.upload(states)
+    got = st.expectation_terms(xs, zs)
+    assert got.shape == (B, len(xs))
+    for b in range(B):
+        want = [oracle.expect_pauli(states[b], [x], [z], [1.0]) for x, z in zip(xs, zs)]
+        assert np.max(np.abs(got[b] - want)) <= EPS, b
+    assert np.allclose(got[:, 0], 1.0, atol=EPS)  # identity term: the norm
+    # the weighted sum agrees with the per-term values
+    cs = rng.normal(size=len(xs))
+    assert np.max(np.abs(st.expectation(xs, zs, cs) - got @ cs)) <= EPS
+    st.close()
+
+
 @pytest.fixture(params=[0, 2], ids=["per-op", "adjoint-passes"])
 def adj_ctx(hip_ctx, request):
     """The reverse sweep per op (SC_OPT_JIT 0), or as runtime-compiled adjoint passes (n >= 12)."""

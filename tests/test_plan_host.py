@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from circuits import random_circuit, random_state
+from circuits import random_circuit, random_state, unit_pivot_circuit
 from statecraft_amd import capi, workloads
 from statecraft_amd.circuit import Hadamard, PauliZ, QuantumCircuit, lower, rotation_y, rotation_z, zz_phase
 
@@ -101,6 +101,69 @@ def test_jit_pass_kernels_compile_without_device(tmp_path):
     srcs = sorted(tmp_path.glob("sc_passes_*.hip"))
     assert srcs and all(p.with_suffix(".co").stat().st_size > 0 for p in srcs)
     assert sum(p.read_text().count("extern \"C\" __global__") for p in srcs) == stats["passes"]
+
+
+@pytest.mark.parametrize("n,general,diagonals", [(13, False, False), (14, True, True), (15, True, False)])
+def test_unit_pivot_forms_match_oracle(n, general, diagonals, tmp_path):
+    """Single-qubit gates whose rows have equal real pivots (Ry, Rx, H) run in runtime-compiled passes as
+    unit-pivot ops (one FMA per component for real or imaginary ratios) and the pass applies their common
+    factor once: scaled into a diagonal table or dense op of the pass, or -- with only such gates and CNOTs
+    -- by a scaling op.  Gates with zero or complex pivots (Y, a phase times Ry, random unitaries) keep the
+    general form."""
+    rng = np.random.default_rng(n)
+    ops, mats = unit_pivot_circuit(n, 150, rng, general, diagonals)
+    init = random_state(n, rng)
+    want = oracle.run_ops(n, ops, mats, init)
+    s_int, out_int = capi.plan_sweeps_host(n, ops, mats, 4, init, False, False)
+    s_jit, out_jit = capi.plan_sweeps_host(n, ops, mats, 4, init, False, True)
+    assert np.max(np.abs(out_int - want)) <= 1e-12
+    assert np.max(np.abs(out_jit - want)) <= 1e-12
+    assert s_jit["fp64_ops_per_amp"] < (0.9 if general else 0.75) * s_int["fp64_ops_per_amp"]
+    r = capi.jit_compile_host(n, ops, mats, 4, False, 1, str(tmp_path))
+    assert r["kernels"] == s_jit["passes"]
+    src = "".join(p.read_text() for p in tmp_path.glob("sc_passes_*.hip"))
+    assert "op_u1u<" in src
+
+
+def test_unit_pivot_factor_without_a_table_uses_a_scaling_op(tmp_path):
+    """Ry layers and CNOT chains only: no pass has a diagonal table or dense op to take the common factor."""
+    n = 13
+    rng = np.random.default_rng(3)
+    ops = np.zeros(3 * (2 * n - 1), dtype=capi.OP_DTYPE)
+    mats = []
+    i = 0
+    for _ in range(3):
+        for q in range(n):
+            t = rng.uniform(0, 6.3)
+            ops[i]["kind"], ops[i]["q0"], ops[i]["q1"], ops[i]["mat"], ops[i]["param"] = capi.OP_SINGLE, q, -1, len(mats), -1
+            mats.extend([np.cos(t / 2), -np.sin(t / 2), np.sin(t / 2), np.cos(t / 2)])
+            i += 1
+        for q in range(n - 1):
+            ops[i]["kind"], ops[i]["q0"], ops[i]["q1"], ops[i]["mat"], ops[i]["param"] = capi.OP_CNOT, q, q + 1, -1, -1
+            i += 1
+    mats = np.array(mats, dtype=np.complex128)
+    init = random_state(n, rng)
+    stats, out = capi.plan_sweeps_host(n, ops, mats, 4, init, False, True)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+    capi.jit_compile_host(n, ops, mats, 4, False, 1, str(tmp_path))
+    src = "".join(p.read_text() for p in tmp_path.glob("sc_passes_*.hip"))
+    assert "op_scale(" in src and "op_u1r<" not in src
+
+
+def test_qaoa_rx_layers_cost_two_operations_per_qubit():
+    """Rx (imaginary off-pivot ratios) and H (pivots in the first column) cost 2 FP64 operations per
+    amplitude each in runtime-compiled passes, against 8 and 4 in their general complex and real forms."""
+    n = 16
+    circ = workloads.qaoa_ring(n, 4, [0.3, 0.5, 0.7, 0.9], [0.2, 0.4, 0.6, 1.3], np.linspace(0.5, 1.5, n))
+    ops, mats = lower(circ)
+    init = random_state(n, np.random.default_rng(5))
+    s_int, _ = capi.plan_sweeps_host(n, ops, mats, 4, None, False, False)
+    s_jit, out = capi.plan_sweeps_host(n, ops, mats, 4, init, False, True)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+    # 5 single-qubit layers of n gates at 2 operations each, plus the tables of the 4 n ZZ diagonals (at most
+    # 2.5 operations per diagonal once gathered)
+    assert s_jit["fp64_ops_per_amp"] <= 2 * 5 * n + 2.5 * 4 * n
+    assert s_jit["fp64_ops_per_amp"] < 0.5 * s_int["fp64_ops_per_amp"]
 
 
 @pytest.mark.parametrize("n,depth", [(16, 6), (20, 8)])
