@@ -928,6 +928,7 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
         SC_HIP_CHECK(hipMemcpyAsync(du.p, uniforms, B * n_draws * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     SC_HIP_CHECK(hipMemsetAsync(noise_err_flag(dwork.p, (int)B), 0, sizeof(int), ctx->stream));
     uint64_t draw = 0, seg_begin = 0;
+    bool norm_ready = false;  // the first channel reads its weights from the state
     for (uint64_t i = 0; i < n_ops; i++) {
         if (!noise_after[i] && i + 1 < n_ops) continue;
         // unitary segment [seg_begin, i]
@@ -938,8 +939,10 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
         int qs[2] = {ops[i].q0, ops[i].q1};
         int nq = (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
         for (int j = 0; j < nq; j++) {
-            rc = launch_channel_batched(ctx, st, qs[j], kraus, 4, (const double *)du.p, n_draws, draw++, dwork.p);
+            rc = launch_channel_batched(ctx, st, qs[j], kraus, 4, (const double *)du.p, n_draws, draw++, dwork.p,
+                                        norm_ready);
             if (rc) return rc;
+            norm_ready = true;
         }
     }
     int err = 0;

@@ -6,9 +6,15 @@
 //   k_kraus_partial  one read sweep: all K branch weights of every trajectory (fixed-order partials)
 //   k_kraus_select   one thread per trajectory: fixed-order total, r = u * total (nextDouble(bound)
 //                    semantics), sampleWeightedIndex, renormalisation factor 1/sqrt(w_chosen)
-//   k_kraus_apply    one read+write sweep: K_chosen * factor on every trajectory
+//   k_kraus_apply    one read+write sweep: K_chosen * factor on every trajectory, accumulating the norm
+//                    of what it writes
 // ||K_j psi||^2 is exactly the chosen branch weight, so the norm sweep of the reference is not repeated;
-// the factor differs from the reference's LASSQ-based one only by rounding (<= a few ulp).
+// the factor differs from the reference's LASSQ-based one only by rounding (<= a few ulp).  When every
+// K_j is a multiple of a unitary (depolarising channels: sqrt(1-p) I, sqrt(p/3) X, Y, Z), the weights are
+// c_j ||psi||^2 and the norm comes from the previous channel's apply sweep, so a channel after the first
+// costs one read+write sweep of the batch.
+#include <algorithm>
+
 #include "sc_internal.hpp"
 
 namespace sc {
@@ -64,14 +70,25 @@ __global__ void __launch_bounds__(kBlock) k_kraus_partial(const double2 *__restr
 }
 
 // One thread per trajectory.  u[elem * u_stride + draw] is the trajectory's SplittableRandom.nextDouble().
-__global__ void k_kraus_select(const double *__restrict__ partial, int K, int batch, const double *__restrict__ u,
-                               uint64_t u_stride, uint64_t draw, int *__restrict__ choice, double *__restrict__ scale,
-                               int *__restrict__ err) {
+// Branch weights: with norm_parts, every K_j is proportional to a unitary (K_j^dag K_j = c_j I), so
+// ||K_j psi||^2 = c_j ||psi||^2 with ||psi||^2 the sum the previous channel's apply sweep accumulated over
+// the state it wrote (the unitary segment in between keeps the norm); otherwise the K weight partials of
+// k_kraus_partial.  Writes the chosen operator times the renormalisation factor to kmat[elem].
+__global__ void k_kraus_select(const double *__restrict__ partial, const double *__restrict__ norm_parts,
+                               int nparts, Kraus4 kr, double4 cj, int K, int batch, const double *__restrict__ u,
+                               uint64_t u_stride, uint64_t draw, Mat2 *__restrict__ kmat, int *__restrict__ err) {
     int elem = blockIdx.x * blockDim.x + threadIdx.x;
     if (elem >= batch) return;
     double w[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = 0; b < kPartials; b++)
-        for (int j = 0; j < K; j++) w[j] += partial[((uint64_t)elem * kPartials + b) * 4 + j];
+    if (norm_parts) {
+        double nrm = 0.0;
+        for (int b = 0; b < nparts; b++) nrm += norm_parts[(uint64_t)elem * nparts + b];
+        const double c[4] = {cj.x, cj.y, cj.z, cj.w};
+        for (int j = 0; j < K; j++) w[j] = c[j] * nrm;
+    } else {
+        for (int b = 0; b < kPartials; b++)
+            for (int j = 0; j < K; j++) w[j] += partial[((uint64_t)elem * kPartials + b) * 4 + j];
+    }
     double total = 0.0;
     bool bad = false;
     for (int j = 0; j < K; j++) {
@@ -79,87 +96,116 @@ __global__ void k_kraus_select(const double *__restrict__ partial, int K, int ba
         total += w[j];
     }
     if (!isfinite(total) || total <= 1e-15) bad = true;  // MIN_KRAUS_WEIGHT, StatevectorEngine.java:36
+    int idx = K - 1;
+    double f = 1.0;
     if (bad) {
         err[0] = 1;
-        choice[elem] = 0;
-        scale[elem] = 1.0;
-        return;
-    }
-    double r = u[(uint64_t)elem * u_stride + draw] * total;  // SplittableRandom.nextDouble(bound)
-    if (r >= total) r = nextafter(total, 0.0);
-    int idx = K - 1;
-    double cum = 0.0;
-    for (int j = 0; j < K; j++) {  // sampleWeightedIndex :579-588
-        cum += w[j];
-        if (r < cum) {
-            idx = j;
-            break;
+        idx = 0;
+    } else {
+        double r = u[(uint64_t)elem * u_stride + draw] * total;  // SplittableRandom.nextDouble(bound)
+        if (r >= total) r = nextafter(total, 0.0);
+        double cum = 0.0;
+        for (int j = 0; j < K; j++) {  // sampleWeightedIndex :579-588
+            cum += w[j];
+            if (r < cum) {
+                idx = j;
+                break;
+            }
         }
+        const double ns = w[idx];
+        if (!isfinite(ns) || ns == 0.0) err[0] = 2;
+        else f = fabs(ns - 1.0) > 1e-12 ? 1.0 / sqrt(ns) : 1.0;  // renormalizeState :590-600
     }
-    choice[elem] = idx;
-    double ns = w[idx];
-    if (!isfinite(ns) || ns == 0.0) {
-        err[0] = 2;
-        scale[elem] = 1.0;
-        return;
-    }
-    scale[elem] = fabs(ns - 1.0) > 1e-12 ? 1.0 / sqrt(ns) : 1.0;  // renormalizeState :590-600
+    const double *g = idx == 0 ? kr.k[0].v : idx == 1 ? kr.k[1].v : idx == 2 ? kr.k[2].v : kr.k[3].v;
+    Mat2 m;
+    for (int e = 0; e < 8; e++) m.v[e] = g[e] * f;
+    kmat[elem] = m;
 }
 
-__global__ void __launch_bounds__(kBlock) k_kraus_apply(double2 *__restrict__ amp, int n, int target, Kraus4 kr,
-                                                        const int *__restrict__ choice, const double *__restrict__ scale,
-                                                        uint64_t total) {
+// grid (nparts, elements): the chosen operator of each trajectory on its pairs (the element is uniform per
+// workgroup, so its matrix is a scalar load), and per workgroup the sum of |out|^2 into
+// norm_parts[elem * nparts + block] for the next channel's weights.
+__global__ void __launch_bounds__(kBlock) k_kraus_apply(double2 *__restrict__ amp, int n, int target, int nparts,
+                                                        uint64_t e0, const Mat2 *__restrict__ kmat,
+                                                        double *__restrict__ norm_parts) {
+    const uint64_t elem = e0 + blockIdx.y;
+    const uint64_t pairs = 1ULL << (n - 1);
     const uint64_t stride = 1ULL << target;
-    const int lg = n - 1;
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < total; i += (uint64_t)gridDim.x * kBlock) {
-        uint64_t elem = i >> lg;
-        uint64_t p = i & ((1ULL << lg) - 1ULL);
-        uint64_t lo = p & (stride - 1ULL);
-        uint64_t idx0 = ((p ^ lo) << 1) | lo;
-        double2 *base = amp + (elem << n);
-        int c = choice[elem];
-        double f = scale[elem];
-        const double *g = c == 0 ? kr.k[0].v : c == 1 ? kr.k[1].v : c == 2 ? kr.k[2].v : kr.k[3].v;
-        double2 a0 = base[idx0], a1 = base[idx0 | stride];
+    double2 *base = amp + (elem << n);
+    const Mat2 m = kmat[elem];
+    double acc = 0.0;
+    for (uint64_t p = blockIdx.x * (uint64_t)kBlock + threadIdx.x; p < pairs; p += (uint64_t)nparts * kBlock) {
+        const uint64_t lo = p & (stride - 1ULL);
+        const uint64_t idx0 = ((p ^ lo) << 1) | lo;
+        const double2 a0 = base[idx0], a1 = base[idx0 | stride];
         double2 o0, o1;
-        apply2(g, a0, a1, o0, o1);
-        base[idx0] = make_double2(o0.x * f, o0.y * f);
-        base[idx0 | stride] = make_double2(o1.x * f, o1.y * f);
+        apply2(m.v, a0, a1, o0, o1);
+        base[idx0] = o0;
+        base[idx0 | stride] = o1;
+        acc += o0.x * o0.x + o0.y * o0.y + o1.x * o1.x + o1.y * o1.y;
     }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) norm_parts[elem * (uint64_t)nparts + blockIdx.x] = sh[0];
 }
 
 }  // namespace
 
-// Scratch layout expected in `work`: partial [batch*kPartials*4] doubles, choice [batch] ints, scale [batch]
-// doubles, err [1] int.  Returns the size needed when work == nullptr.
+// Scratch layout in `work`: weight partials [batch*kPartials*4] doubles, norm partials [batch*kPartials]
+// doubles, chosen operators [batch] Mat2, err [1] int.  noise_work_bytes gives the size.
 size_t noise_work_bytes(int batch) {
-    return (size_t)batch * kPartials * 4 * sizeof(double) + (size_t)batch * (sizeof(int) + sizeof(double)) + 64;
+    return (size_t)batch * kPartials * 5 * sizeof(double) + (size_t)batch * sizeof(Mat2) + 64;
+}
+
+// K_j^dag K_j = c_j I for every j (c_j returned), so branch weights follow from the norm.
+static bool unitary_proportional(const double *kraus, int K, double *c) {
+    for (int j = 0; j < K; j++) {
+        const double *g = kraus + 8 * j;
+        // columns (g00, g10) and (g01, g11): equal squared lengths, orthogonal
+        const double n0 = g[0] * g[0] + g[1] * g[1] + g[4] * g[4] + g[5] * g[5];
+        const double n1 = g[2] * g[2] + g[3] * g[3] + g[6] * g[6] + g[7] * g[7];
+        const double dr = g[0] * g[2] + g[1] * g[3] + g[4] * g[6] + g[5] * g[7];
+        const double di = g[0] * g[3] - g[1] * g[2] + g[4] * g[7] - g[5] * g[6];
+        if (n0 != n1 || dr != 0.0 || di != 0.0) return false;
+        c[j] = n0;
+    }
+    return true;
 }
 
 int launch_channel_batched(Context *ctx, State *st, int target, const double *kraus, int K, const double *u_dev,
-                           uint64_t u_stride, uint64_t draw, void *work) {
+                           uint64_t u_stride, uint64_t draw, void *work, bool norm_ready) {
     Kraus4 kr;
     for (int j = 0; j < 4; j++)
         for (int e = 0; e < 8; e++) kr.k[j].v[e] = j < K ? kraus[8 * j + e] : 0.0;
+    const uint64_t B = (uint64_t)st->batch;
     double *partial = (double *)work;
-    double *scale = partial + (size_t)st->batch * kPartials * 4;
-    int *choice = (int *)(scale + st->batch);
-    int *err = choice + st->batch;
-    {
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 16.0 * (double)((uint64_t)st->batch << st->n));
-        hipLaunchKernelGGL(k_kraus_partial, dim3(kPartials, st->batch), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
-                           target, K, kr, partial);
+    double *norm_parts = partial + B * kPartials * 4;
+    Mat2 *kmat = (Mat2 *)(norm_parts + B * kPartials);
+    int *err = (int *)(kmat + B);
+    const uint64_t pairs = 1ULL << (st->n - 1);
+    const int nparts = (int)std::max<uint64_t>(1, std::min<uint64_t>(kPartials, pairs / (4ULL * kBlock)));
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool from_norm = norm_ready && unitary_proportional(kraus, K, c);
+    if (!from_norm) {
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 16.0 * (double)(B << st->n));
+        hipLaunchKernelGGL(k_kraus_partial, dim3(kPartials, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp,
+                           st->n, target, K, kr, partial);
         SC_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_kraus_select, dim3((st->batch + 63) / 64), dim3(64), 0, ctx->stream, partial, K, st->batch,
-                       u_dev, u_stride, draw, choice, scale, err);
+    hipLaunchKernelGGL(k_kraus_select, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, ctx->stream, partial,
+                       from_norm ? (const double *)norm_parts : nullptr, nparts, kr, make_double4(c[0], c[1], c[2], c[3]),
+                       K, (int)B, u_dev, u_stride, draw, kmat, err);
     SC_HIP_CHECK(hipGetLastError());
-    uint64_t total = (uint64_t)st->batch << (st->n - 1);
-    uint64_t blocks = std::min<uint64_t>((total + kBlock - 1) / kBlock, (uint64_t)ctx->cu_count * 16);
-    {
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 32.0 * (double)((uint64_t)st->batch << st->n));
-        hipLaunchKernelGGL(k_kraus_apply, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, st->amp, st->n, target,
-                           kr, choice, scale, total);
+    for (uint64_t e0 = 0; e0 < B; e0 += 65535) {
+        const unsigned cnt = (unsigned)std::min<uint64_t>(65535, B - e0);
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 32.0 * (double)((uint64_t)cnt << st->n));
+        hipLaunchKernelGGL(k_kraus_apply, dim3((unsigned)nparts, cnt), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
+                           target, nparts, e0, (const Mat2 *)kmat, norm_parts);
         SC_HIP_CHECK(hipGetLastError());
     }
     return SC_OK;
@@ -167,9 +213,8 @@ int launch_channel_batched(Context *ctx, State *st, int target, const double *kr
 
 int *noise_err_flag(void *work, int batch) {
     double *partial = (double *)work;
-    double *scale = partial + (size_t)batch * kPartials * 4;
-    int *choice = (int *)(scale + batch);
-    return choice + batch;
+    Mat2 *kmat = (Mat2 *)(partial + (size_t)batch * kPartials * 5);
+    return (int *)(kmat + batch);
 }
 
 }  // namespace sc
