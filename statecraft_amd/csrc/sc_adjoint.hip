@@ -906,13 +906,18 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
     Context *ctx = st->ctx;
     int rc = set_device(ctx);
     if (rc) return rc;
-    std::vector<const double *> elems{mats};
-    std::vector<PlanOp> pops;
-    if ((rc = analyze_ops(st->n, ops, n_ops, elems, n_mat_complex, pops))) return rc;
-    if (pops.size() != n_ops) return fail(SC_E_ARG, "trajectory circuits must not contain measurements");
+    {
+        std::vector<const double *> elems{mats};
+        std::vector<PlanOp> pops;
+        if ((rc = analyze_ops(st->n, ops, n_ops, elems, n_mat_complex, pops))) return rc;
+        if (pops.size() != n_ops) return fail(SC_E_ARG, "trajectory circuits must not contain measurements");
+    }
+    auto channel_qubits = [&](uint64_t i) {
+        return (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
+    };
     uint64_t draws = 0;
     for (uint64_t i = 0; i < n_ops; i++)
-        if (noise_after[i]) draws += (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
+        if (noise_after[i]) draws += channel_qubits(i);
     if (draws != n_draws)
         return fail(SC_E_ARG, "expected " + std::to_string(draws) + " uniforms per trajectory, got " +
                                   std::to_string(n_draws));
@@ -920,36 +925,93 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
     const double a = std::sqrt(1.0 - p), s = std::sqrt(p / 3.0);
     const double kraus[32] = {a, 0, 0, 0, 0, 0, a, 0,  0, 0, s, 0,  s, 0, 0, 0,
                               0, 0, 0, -s, 0, s, 0, 0, s, 0, 0, 0, 0, 0, -s, 0};
+    // Every Kraus operator is a multiple of a unitary, K_j^dag K_j = c_j I, so the reference's branch weight
+    // ||K_j psi||^2 (StatevectorEngine.computeBranchWeight :511-530) is c_j ||psi||^2.  ||psi||^2 is read
+    // from the live states once; unitary ops keep it and a channel that picks j and renormalises by f leaves
+    // c_j ||psi||^2 f^2.  Each trajectory's branches therefore follow from its uniforms before any amplitude
+    // moves, and the noisy circuit becomes one circuit per trajectory: the ops with, after every noisy op,
+    // the chosen K_j times its renormalisation factor as a single-qubit op on each channel qubit.
+    double c[4];
+    for (int j = 0; j < 4; j++) {
+        const double *g = kraus + 8 * j;
+        c[j] = g[0] * g[0] + g[1] * g[1] + g[4] * g[4] + g[5] * g[5];
+    }
     const uint64_t B = (uint64_t)st->batch;
-    DevBuf du, dwork;
-    if ((rc = du.alloc(B * std::max<uint64_t>(n_draws, 1) * sizeof(double)))) return rc;
-    if ((rc = dwork.alloc(noise_work_bytes((int)B)))) return rc;
-    if (n_draws > 0)
-        SC_HIP_CHECK(hipMemcpyAsync(du.p, uniforms, B * n_draws * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    SC_HIP_CHECK(hipMemsetAsync(noise_err_flag(dwork.p, (int)B), 0, sizeof(int), ctx->stream));
-    uint64_t draw = 0, seg_begin = 0;
-    bool norm_ready = false;  // the first channel reads its weights from the state
-    for (uint64_t i = 0; i < n_ops; i++) {
-        if (!noise_after[i] && i + 1 < n_ops) continue;
-        // unitary segment [seg_begin, i]
-        rc = apply_ops_impl(st, ops + seg_begin, i + 1 - seg_begin, mats, n_mat_complex, 0, false);
-        if (rc) return rc;
-        seg_begin = i + 1;
-        if (!noise_after[i]) continue;
-        int qs[2] = {ops[i].q0, ops[i].q1};
-        int nq = (ops[i].q1 >= 0 && ops[i].kind != SC_OP_SINGLE && ops[i].kind != SC_OP_MCU) ? 2 : 1;
-        for (int j = 0; j < nq; j++) {
-            rc = launch_channel_batched(ctx, st, qs[j], kraus, 4, (const double *)du.p, n_draws, draw++, dwork.p,
-                                        norm_ready);
-            if (rc) return rc;
-            norm_ready = true;
+    std::vector<double> nrm(B);
+    {
+        const uint64_t zero = 0;
+        if ((rc = pauli_terms(ctx, st, st->amp, &zero, &zero, 1, nrm.data()))) return rc;
+    }
+    const uint64_t stride = n_mat_complex + 4 * n_draws;  // complex entries per trajectory
+    std::vector<double> emats(2 * stride * B);
+    for (uint64_t b = 0; b < B; b++) {
+        double *m = emats.data() + 2 * stride * b;
+        if (n_mat_complex) std::memcpy(m, mats, 2 * n_mat_complex * sizeof(double));
+        double N = nrm[b];
+        for (uint64_t d = 0; d < n_draws; d++) {
+            double w[4], total = 0.0;
+            bool bad = false;
+            for (int j = 0; j < 4; j++) {
+                w[j] = c[j] * N;
+                if (!std::isfinite(w[j]) || w[j] < 0.0) bad = true;
+                total += w[j];
+            }
+            if (bad || !std::isfinite(total) || total <= 1e-15)  // MIN_KRAUS_WEIGHT, StatevectorEngine.java:36
+                return fail(SC_E_STATE, "invalid Kraus branch weights in trajectory " + std::to_string(b));
+            double r = uniforms[b * n_draws + d] * total;  // SplittableRandom.nextDouble(bound)
+            if (r >= total) r = std::nextafter(total, 0.0);
+            int idx = 3;
+            double cum = 0.0;
+            for (int j = 0; j < 4; j++) {  // sampleWeightedIndex :579-588
+                cum += w[j];
+                if (r < cum) {
+                    idx = j;
+                    break;
+                }
+            }
+            const double ns = w[idx];
+            if (!std::isfinite(ns) || ns == 0.0)
+                return fail(SC_E_STATE, "invalid Kraus branch weights in trajectory " + std::to_string(b));
+            const double f = std::fabs(ns - 1.0) > 1e-12 ? 1.0 / std::sqrt(ns) : 1.0;  // renormalizeState :590-600
+            for (int e = 0; e < 8; e++) m[2 * (n_mat_complex + 4 * d) + e] = kraus[8 * idx + e] * f;
+            N = ns * f * f;
         }
     }
-    int err = 0;
-    SC_HIP_CHECK(hipMemcpyAsync(&err, noise_err_flag(dwork.p, (int)B), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (err) return fail(SC_E_STATE, "invalid Kraus branch weights in a trajectory");
-    return SC_OK;
+    std::vector<sc_op> nops;
+    std::vector<bool> inserted;
+    uint64_t d = 0;
+    for (uint64_t i = 0; i < n_ops; i++) {
+        nops.push_back(ops[i]);
+        inserted.push_back(false);
+        if (!noise_after[i]) continue;
+        const int qs[2] = {ops[i].q0, ops[i].q1};
+        for (int j = 0; j < channel_qubits(i); j++) {
+            sc_op o;
+            std::memset(&o, 0, sizeof(o));
+            o.kind = SC_OP_SINGLE;
+            o.q0 = qs[j];
+            o.q1 = -1;
+            o.mat = (int32_t)(n_mat_complex + 4 * d++);
+            o.param = -1;
+            nops.push_back(o);
+            inserted.push_back(true);
+        }
+    }
+    std::vector<const double *> elems;
+    for (uint64_t b = 0; b < B; b++) elems.push_back(emats.data() + 2 * stride * b);
+    std::vector<PlanOp> pops;
+    if ((rc = analyze_ops(st->n, nops.data(), nops.size(), elems, stride, pops))) return rc;
+    // the channel ops' matrices change from call to call: plan them as general complex 2x2, so that every
+    // call of one circuit shares its plan and pass kernels
+    for (size_t k = 0; k < pops.size(); k++) {
+        if (!inserted[k]) continue;
+        PlanOp &op = pops[k];
+        op.is_diag = op.is_x = op.is_real = false;
+        op.unit = -1;
+        op.targets = 1ULL << op.q0;
+        op.diag = 0;
+    }
+    return apply_planned(st, pops, elems, true);
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sc_internal.hpp"
@@ -225,7 +226,24 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
         if (!plan.ops.empty()) std::memcpy(h + pass_bytes, plan.ops.data(), plan.ops.size() * sizeof(SweepOp));
     }
     double *hpool = (double *)(h + pass_bytes + ops_bytes);
-    for (size_t e = 0; e < nelem_pool; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e);
+    // per-element pools are independent: large batches build them on several host threads
+    const size_t pool_work = nelem_pool * (plan.pool.size() + 1);
+    const unsigned nthreads = (unsigned)std::min<size_t>(
+        {nelem_pool, pool_work >= 4096 ? (size_t)std::max(1u, std::thread::hardware_concurrency()) : 1, 16});
+    auto build_range = [&](size_t e0, size_t e1) {
+        for (size_t e = e0; e < e1; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e);
+    };
+    if (nthreads <= 1) {
+        build_range(0, nelem_pool);
+    } else {
+        std::vector<std::thread> pool_threads;
+        const size_t per = (nelem_pool + nthreads - 1) / nthreads;
+        for (unsigned t = 0; t < nthreads; t++) {
+            const size_t e0 = t * per, e1 = std::min(nelem_pool, e0 + per);
+            if (e0 < e1) pool_threads.emplace_back(build_range, e0, e1);
+        }
+        for (auto &th : pool_threads) th.join();
+    }
     if (total > ctx->plan_dev_bytes) {
         if (ctx->plan_dev) SC_HIP_CHECK(hipFree(ctx->plan_dev));
         ctx->plan_dev = nullptr;
@@ -265,6 +283,36 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
 
 }  // namespace
 
+int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
+                  bool per_elem) {
+    Context *ctx = st->ctx;
+    ctx->last_sweeps = ctx->last_launches = ctx->last_blocks = 0;
+    if (plan_ops.empty()) return SC_OK;
+    int mode = ctx->exec_mode;
+    if (mode == SC_EXEC_SWEEP && st->n < kChunkBits) mode = SC_EXEC_FUSED;
+    int rc = SC_OK;
+    if (per_elem && mode != SC_EXEC_SWEEP) {
+        // element-by-element with a one-element view of the state
+        for (int b = 0; b < st->batch; b++) {
+            State view = *st;
+            view.amp = st->amp + ((uint64_t)b << st->n);
+            view.batch = 1;
+            rc = mode == SC_EXEC_FUSED ? exec_fused(ctx, &view, plan_ops, elems[b])
+                                       : exec_direct(ctx, &view, plan_ops, elems[b]);
+            if (rc) return rc;
+        }
+    } else if (mode == SC_EXEC_DIRECT) {
+        rc = exec_direct(ctx, st, plan_ops, elems[0]);
+    } else if (mode == SC_EXEC_FUSED) {
+        rc = exec_fused(ctx, st, plan_ops, elems[0]);
+    } else {
+        rc = exec_sweep(ctx, st, plan_ops, elems);
+    }
+    if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
 int apply_ops_impl(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                    uint64_t mat_stride, bool per_elem) {
     Context *ctx = st->ctx;
@@ -279,30 +327,7 @@ int apply_ops_impl(State *st, const sc_op *ops, uint64_t n_ops, const double *ma
     std::vector<PlanOp> plan_ops;
     rc = analyze_ops(st->n, ops, n_ops, elems, per_elem ? mat_stride : n_mat_complex, plan_ops);
     if (rc) return rc;
-    ctx->last_sweeps = ctx->last_launches = ctx->last_blocks = 0;
-    if (plan_ops.empty()) return SC_OK;
-    int mode = ctx->exec_mode;
-    if (mode == SC_EXEC_SWEEP && st->n < kChunkBits) mode = SC_EXEC_FUSED;
-    if (per_elem && mode != SC_EXEC_SWEEP) {
-        // element-by-element with a one-element view of the state
-        for (int b = 0; b < st->batch; b++) {
-            State view = *st;
-            view.amp = st->amp + ((uint64_t)b << st->n);
-            view.batch = 1;
-            rc = mode == SC_EXEC_FUSED ? exec_fused(ctx, &view, plan_ops, elems[b])
-                                       : exec_direct(ctx, &view, plan_ops, elems[b]);
-            if (rc) return rc;
-        }
-    } else if (mode == SC_EXEC_DIRECT) {
-        rc = exec_direct(ctx, st, plan_ops, mats);
-    } else if (mode == SC_EXEC_FUSED) {
-        rc = exec_fused(ctx, st, plan_ops, mats);
-    } else {
-        rc = exec_sweep(ctx, st, plan_ops, elems);
-    }
-    if (rc) return rc;
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    return SC_OK;
+    return apply_planned(st, plan_ops, elems, per_elem);
 }
 
 }  // namespace sc

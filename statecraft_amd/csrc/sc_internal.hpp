@@ -186,13 +186,7 @@ int launch_kraus_weights(Context *ctx, const double2 *amp, int n, int target, co
                          double *weights_host);
 int launch_scale_state(Context *ctx, double2 *amp, uint64_t count, double factor);
 
-// batched Kraus channels (sc_noise.hip)
-size_t noise_work_bytes(int batch);
-// One depolarising-type channel on every trajectory of the batch.  norm_ready: the previous call's apply
-// sweep left the norm partials of the current state in `work` (only unitary ops ran since).
-int launch_channel_batched(Context *ctx, State *st, int target, const double *kraus, int K, const double *u_dev,
-                           uint64_t u_stride, uint64_t draw, void *work, bool norm_ready);
-int *noise_err_flag(void *work, int batch);
+
 
 // op application shared by the C-ABI entry points (sc_capi.hip)
 int apply_ops_impl(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,

@@ -1574,16 +1574,14 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
         const PoolRecipe &rc = plan.pool[r];
         double *dst = out + 2 * offs[r];
         if (rc.kind == RC_DIAG16) {
-            for (int reg = 0; reg < kRegs; reg++) {
-                double col[2 * kRegs] = {0};
-                col[2 * reg] = 1.0;
-                for (size_t t = 0; t < rc.fused_ops.size(); t++) {
-                    const SweepOp &so = rc.fused_ops[t];
-                    host_apply(so, so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr, col, 0u, 0ULL);
-                }
-                dst[2 * reg] = col[2 * reg];
-                dst[2 * reg + 1] = col[2 * reg + 1];
+            // every fused op is diagonal: applied to the all-ones vector they leave the table
+            double col[2 * kRegs];
+            for (int reg = 0; reg < kRegs; reg++) col[2 * reg] = 1.0, col[2 * reg + 1] = 0.0;
+            for (size_t t = 0; t < rc.fused_ops.size(); t++) {
+                const SweepOp &so = rc.fused_ops[t];
+                host_apply(so, so.mat >= 0 ? out + 2 * offs[so.mat] : nullptr, col, 0u, 0ULL);
             }
+            std::memcpy(dst, col, sizeof(col));
             continue;
         }
         // Block index l has bit (m-1-j) <-> register bit fused_bits[j]; the block's ops touch no other

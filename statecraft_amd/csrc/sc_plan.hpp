@@ -41,6 +41,10 @@ constexpr int kPlanGrad = 100;
 // with SC_E_UNSUPPORTED (StatevectorEngine.java:92-95).
 int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const double *> &mats_all,
                 uint64_t n_mat_complex, std::vector<PlanOp> &out);
+// Apply analysed ops to a state (one matrix array per batch element with per_elem, else elems[0] for all),
+// in the context's execution mode (sc_capi.hip).
+int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
+                  bool per_elem);
 
 bool ops_conflict(const PlanOp &a, const PlanOp &b);
 
