@@ -188,7 +188,7 @@ int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zm
                          int n_terms, double *out);
 /* Every term on its own: out[b * n_terms + t] = <psi_b|P_t|psi_b> (real: each P_t is the Hermitian Pauli
  * string of its masks).  Terms sharing an x-mask are evaluated together in one read of the state (up to
- * 32 per read), so a diagonal observable of many terms costs one pass. */
+ * 16 per read), so a diagonal observable of many terms costs one pass per 16 terms. */
 int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
                                double *out);
 /* Adjoint gradient of E(theta) = <psi(theta)|H|psi(theta)>.  On entry `st` holds the initial states; the

@@ -81,7 +81,7 @@ __device__ __forceinline__ uint64_t expand_mask(uint64_t x, uint64_t mask) {
 // when |x & z_t| is even, 2i s(i) Im q when odd.  The kernel therefore visits each pair once (index bit
 // ctz(x) clear) and accumulates s(i) Re q or s(i) Im q per term (bit t of `imag`); for x = 0 it visits every
 // index with q = |psi_i|^2.  The host applies the factor 2 and the sign of i^{|x & z_t|}.
-constexpr int kTermsPerLaunch = 32;
+constexpr int kTermsPerLaunch = 16;  // more would spill the terms' scalar registers
 struct TermGroup {
     uint64_t x;
     uint64_t z[kTermsPerLaunch];
@@ -90,50 +90,70 @@ struct TermGroup {
     int32_t nt;
 };
 
-// grid (parts, elements): part[((e0 + elem) * rows + row_t) * parts + block]
+// grid (parts, elements), parts a power of two: part[((e0 + elem) * rows + row_t) * parts + block].
+// A thread visits h = h0 + k * S (S = parts * kBlock) with h0 < S, so the index i (h with a zero inserted at
+// bit ctz(x) for pairs) is c | u: c fixed per thread, u the same for every thread of an iteration.  The sign
+// (-1)^parity(i & z_t) is the thread's constant sign times a uniform one, so each term costs one FMA with a
+// scalar +-1 per amplitude and the thread's sign is applied once at the end.  NT >= g.nt terms are unrolled.
+template <int NT>
 __global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restrict__ amp, int n, uint64_t e0,
                                                         TermGroup g, int rows, int parts, double *__restrict__ part) {
     const uint64_t elem = e0 + blockIdx.y;
     const double2 *base = amp + (elem << n);
     const bool pairs = g.x != 0;
     const uint64_t count = pairs ? (1ULL << (n - 1)) : (1ULL << n);
-    const int pbit = pairs ? __builtin_ctzll(g.x) : 0;
-    double acc[kTermsPerLaunch];
+    const int pbit = pairs ? __builtin_ctzll(g.x) : 63;
+    auto spread = [&](uint64_t h) {  // h with a zero inserted at bit pbit (pairs), else h
+        const uint64_t lo = h & ((1ULL << pbit) - 1ULL);
+        return pairs ? (((h ^ lo) << 1) | lo) : h;
+    };
+    const uint64_t S = (uint64_t)parts * kBlock;
+    const uint64_t h0 = blockIdx.x * (uint64_t)kBlock + threadIdx.x;
+    const uint64_t c = spread(h0);
+    double acc[NT];
 #pragma unroll
-    for (int t = 0; t < kTermsPerLaunch; t++) acc[t] = 0.0;
-    for (uint64_t h = blockIdx.x * (uint64_t)kBlock + threadIdx.x; h < count; h += (uint64_t)parts * kBlock) {
-        uint64_t i = h;
-        double qr, qi;
-        if (pairs) {
-            const uint64_t lo = h & ((1ULL << pbit) - 1ULL);
-            i = ((h ^ lo) << 1) | lo;
-            const double2 a = base[i], b = base[i ^ g.x];
-            qr = fma(b.x, a.x, b.y * a.y);
-            qi = fma(b.x, a.y, -b.y * a.x);
-        } else {
-            const double2 a = base[i];
-            qr = fma(a.x, a.x, a.y * a.y);
-            qi = 0.0;
+    for (int t = 0; t < NT; t++) acc[t] = 0.0;
+    for (uint64_t k0 = 0; k0 < count; k0 += 4 * S) {
+        double qr[4], qi[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t hk = k0 + (uint64_t)j * S;
+            qr[j] = 0.0;
+            qi[j] = 0.0;
+            if (hk + h0 < count) {
+                const uint64_t i = c | spread(hk);
+                if (pairs) {
+                    const double2 a = base[i], b = base[i ^ g.x];
+                    qr[j] = fma(b.x, a.x, b.y * a.y);
+                    qi[j] = fma(b.x, a.y, -b.y * a.x);
+                } else {
+                    const double2 a = base[i];
+                    qr[j] = fma(a.x, a.x, a.y * a.y);
+                }
+            }
         }
 #pragma unroll
-        for (int t = 0; t < kTermsPerLaunch; t++) {
-            if (t < g.nt) {
-                const double v = ((g.imag >> t) & 1u) ? qi : qr;
-                acc[t] += (__popcll(i & g.z[t]) & 1) ? -v : v;
+        for (int j = 0; j < 4; j++) {
+            const uint64_t u = spread(k0 + (uint64_t)j * S);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const double v = ((g.imag >> t) & 1u) ? qi[j] : qr[j];
+                acc[t] = fma((__popcll(u & g.z[t]) & 1) ? -1.0 : 1.0, v, acc[t]);
             }
         }
     }
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+        if (__popcll(c & g.z[t]) & 1) acc[t] = -acc[t];
     // wavefront sums, then the workgroup's four wavefronts through LDS
-    __shared__ double sh[kBlock / 64][kTermsPerLaunch];
+    __shared__ double sh[kBlock / 64][NT];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int t = 0; t < kTermsPerLaunch; t++) {
-        if (t < g.nt) {
-            double v = acc[t];
+    for (int t = 0; t < NT; t++) {
+        double v = acc[t];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) sh[wave][t] = v;
-        }
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sh[wave][t] = v;
     }
     __syncthreads();
     if ((int)threadIdx.x < g.nt) {
@@ -362,10 +382,11 @@ int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm,
     const uint64_t rows = (uint64_t)n_terms;
     // partial sums per row: enough workgroups to fill the GPU, at least 8 amplitudes per thread
     const uint64_t half = n > 0 ? (1ULL << (n - 1)) : 1ULL;
-    uint64_t parts = std::max<uint64_t>(1, (4ULL * ctx->cu_count + B - 1) / B);
+    uint64_t parts = std::max<uint64_t>(1, (8ULL * ctx->cu_count + B - 1) / B);
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, half / (8ULL * kBlock)));
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, (32ULL << 20) / (rows * B)));
     parts = std::min<uint64_t>(parts, 4096);
+    parts = 1ULL << (63 - __builtin_clzll(parts));  // a power of two (the kernel's index split)
     const size_t part_bytes = (size_t)(B * rows * parts) * sizeof(double);
     int rc = ensure_scratch(ctx, part_bytes + (size_t)(B * rows) * sizeof(double));
     if (rc) return rc;
@@ -384,11 +405,14 @@ int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm,
         gterms[k].push_back(t);
     }
     for (size_t k = 0; k < gx.size(); k++) {
-        for (size_t c0 = 0; c0 < gterms[k].size(); c0 += kTermsPerLaunch) {
+        // launches of equal size, at most kTermsPerLaunch terms each
+        const size_t nl = (gterms[k].size() + kTermsPerLaunch - 1) / kTermsPerLaunch;
+        const size_t per = (gterms[k].size() + nl - 1) / nl;
+        for (size_t c0 = 0; c0 < gterms[k].size(); c0 += per) {
             TermGroup g;
             std::memset(&g, 0, sizeof(g));
             g.x = gx[k];
-            g.nt = (int)std::min<size_t>(kTermsPerLaunch, gterms[k].size() - c0);
+            g.nt = (int)std::min<size_t>(per, gterms[k].size() - c0);
             for (int j = 0; j < g.nt; j++) {
                 const int t = gterms[k][c0 + j];
                 g.z[j] = zm[t];
@@ -398,8 +422,19 @@ int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm,
             for (uint64_t e0 = 0; e0 < B; e0 += 65535) {
                 const unsigned cnt = (unsigned)std::min<uint64_t>(65535, B - e0);
                 ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 16.0 * (double)((uint64_t)cnt << n));
-                hipLaunchKernelGGL(k_pauli_terms, dim3((unsigned)parts, cnt), dim3(kBlock), 0, ctx->stream, amp, n,
-                                   e0, g, (int)rows, (int)parts, part);
+                const dim3 grid((unsigned)parts, cnt);
+                if (g.nt <= 1)
+                    hipLaunchKernelGGL(k_pauli_terms<1>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
+                                       (int)parts, part);
+                else if (g.nt <= 4)
+                    hipLaunchKernelGGL(k_pauli_terms<4>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
+                                       (int)parts, part);
+                else if (g.nt <= 8)
+                    hipLaunchKernelGGL(k_pauli_terms<8>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
+                                       (int)parts, part);
+                else
+                    hipLaunchKernelGGL(k_pauli_terms<16>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
+                                       (int)rows, (int)parts, part);
                 SC_HIP_CHECK(hipGetLastError());
             }
         }
@@ -526,6 +561,7 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
     if (rc) return rc;
     if (count > 0) SC_HIP_CHECK(hipMemsetAsync(dpart.p, 0, count * (uint64_t)gsub * 2 * sizeof(double), ctx->stream));
     double *part = (double *)dpart.p;
+    std::vector<double> corr(count, 1.0);  // probe factors of unit-pivot ops, per (slot, element)
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
     for (int64_t k = (int64_t)n_ckpt; k >= 0; k--) {
         const uint64_t begin = C > 0 ? (uint64_t)k * C : 0;
@@ -550,6 +586,12 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
         if (plan.passes.empty()) continue;
         // matrix pools of the inverse matrices, one per element when they differ
         const uint64_t E = inv_elems.size();
+        for (uint64_t e = 0; e < E && !plan.probe_pivots.empty(); e++) {
+            std::vector<std::pair<uint64_t, double>> pf;
+            probe_factors(plan, inv_elems[e], pf);
+            for (const auto &x : pf)
+                for (uint64_t b = (E > 1 ? e : 0); b < (E > 1 ? e + 1 : B); b++) corr[x.first * B + b] = x.second;
+        }
         std::vector<double> hpool(2 * std::max<uint64_t>(plan.pool_complex, 1) * E);
         for (uint64_t e = 0; e < E; e++) build_pool(plan, inv_elems[e], hpool.data() + 2 * plan.pool_complex * e);
         DevBuf dpool;
@@ -578,6 +620,7 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
         SC_HIP_CHECK(hipGetLastError());
         SC_HIP_CHECK(hipMemcpyAsync(res, out, count * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint64_t i = 0; i < count; i++) res[2 * i + 1] *= corr[i];
     }
     return SC_OK;
 }
@@ -653,6 +696,9 @@ int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const
     build_pool(plan, inv.data(), pool.data());
     std::vector<double> sums(2 * (size_t)std::max(n_slots, 1), 0.0);
     execute_plan_host(num_qubits, plan, pool.data(), psi_inout, lam_inout, sums.data());
+    std::vector<std::pair<uint64_t, double>> pf;
+    probe_factors(plan, inv.data(), pf);
+    for (const auto &x : pf) sums[2 * x.first + 1] *= x.second;
     std::fill(grads, grads + n_params, 0.0);
     for (uint64_t i = 0; i < n_ops; i++)
         if (slot[i] >= 0) grads[ops[i].param] += 2.0 * ops[i].coeff * sums[2 * slot[i] + 1];

@@ -137,8 +137,14 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "  __shared__ v2d lds[kChunk];\n"
          "  CV2 *mats = (CV2 *)mats_arg;\n"
          "  const uint32_t tid = threadIdx.x;\n"
-         "  uint64_t c = blockIdx.x;\n"
-         "  if (c >= total_chunks) return;\n"
+         // with one matrix pool per element (mat_stride != 0) a workgroup takes a contiguous range of chunks,
+         // so that consecutive chunks share their element's matrices in the scalar cache; otherwise chunks
+         // are dealt round-robin
+         "  const bool blocked = mat_stride != 0;\n"
+         "  const uint64_t span = blocked ? (total_chunks + gridDim.x - 1) / gridDim.x : 0;\n"
+         "  uint64_t c = blocked ? blockIdx.x * span : blockIdx.x;\n"
+         "  const uint64_t c_end = blocked ? (c + span < total_chunks ? c + span : total_chunks) : total_chunks;\n"
+         "  if (c >= c_end) return;\n"
          "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
          "  auto cbase = [](uint64_t x) -> uint64_t { return "
       << deposit_expr("x", ps.outer_mask) << "; };\n";
@@ -176,10 +182,10 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     s << "  for (;;) {\n"
          "    const uint64_t elem = (uint64_t)elem_begin + (c >> log_chunks);\n"
          "    CV2 *emats = mats + elem * mat_stride;\n"
-         "    const uint64_t next = c + gridDim.x;\n"
-         "    const uint64_t ncb = next < total_chunks ? cbase(next & chunk_mask) : 0ULL;\n";
+         "    const uint64_t next = blocked ? c + 1 : c + gridDim.x;\n"
+         "    const uint64_t ncb = next < c_end ? cbase(next & chunk_mask) : 0ULL;\n";
     auto next_loads = [&](const char *dst) {
-        s << "    if (next < total_chunks) {\n"
+        s << "    if (next < c_end) {\n"
              "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gthread);\n";
         regs_load(dst, "src");
         s << "    }\n";
@@ -211,7 +217,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "    v2d *dst = amp + (elem << n) + cb + opaque(gthread);\n";
     regs_store("dst", "a");
     s << "    }\n"
-         "    if (next >= total_chunks) break;\n"
+         "    if (next >= c_end) break;\n"
          "    c = next;\n"
          "    cb = ncb;\n";
     sync((ps.wave_local & 2) != 0);  // every wavefront has read its store-phase slots
@@ -311,11 +317,11 @@ std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps) {
             const char *P0 = in_a ? "a" : "p", *P1 = in_a ? "p" : "a";
             const char *L0 = in_a ? "b" : "q", *L1 = in_a ? "q" : "b";
             if (op.kind == SW_GRAD) {
+                // only Im <lambda|G|psi> enters a gradient: the real part stays zero
                 s << "    {\n    double re = 0.0, im = 0.0;\n    grad_acc<" << (int)op.ta << ", " << (int)op.k0 << ", "
                   << (op.k1 >= 0 ? (int)op.k1 : 0) << ">(" << P0 << ", " << L0 << ", re, im);\n"
-                  << "    for (int m = 32; m >= 1; m >>= 1) { re += __shfl_xor(re, m); im += __shfl_xor(im, m); }\n"
-                  << "    if ((tid & 63u) == 0u) { gacc[tid >> 6][" << k << "][0] += re; gacc[tid >> 6][" << k
-                  << "][1] += im; }\n    }\n";
+                  << "    for (int m = 32; m >= 1; m >>= 1) im += __shfl_xor(im, m);\n"
+                  << "    if ((tid & 63u) == 0u) gacc[tid >> 6][" << k << "][1] += im;\n    }\n";
                 k++;
                 continue;
             }

@@ -461,22 +461,30 @@ __device__ __forceinline__ void cdot_acc(v2d a, v2d b, double &re, double &im) {
     im = im - a.y * b.x;
 }
 
+// Im conj(a) * b accumulated
+__device__ __forceinline__ void cdot_im_acc(v2d a, v2d b, double &im) {
+    im = fma(a.x, b.y, im);
+    im = im - a.y * b.x;
+}
+
+// Im <lambda| G |psi> over the registers (a gradient needs only the imaginary part; re is left alone).
 template <int GEN, int K0, int K1>
 __device__ __forceinline__ void grad_acc(const Regs &psi, const Regs &lam, double &re, double &im) {
+    (void)re;
 #pragma unroll
     for (int r = 0; r < kRegs; r++) {
         const int b0 = (r >> K0) & 1;
         if (GEN == 1) {
-            cdot_acc(lam[r], psi[r ^ (1 << K0)], re, im);
+            cdot_im_acc(lam[r], psi[r ^ (1 << K0)], im);
         } else if (GEN == 2) {  // Y psi: (-i psi1, i psi0)
             const v2d p = psi[r ^ (1 << K0)];
-            cdot_acc(lam[r], b0 ? make_v2d(-p.y, p.x) : make_v2d(p.y, -p.x), re, im);
+            cdot_im_acc(lam[r], b0 ? make_v2d(-p.y, p.x) : make_v2d(p.y, -p.x), im);
         } else if (GEN == 3) {
             const v2d p = psi[r];
-            cdot_acc(lam[r], b0 ? make_v2d(-p.x, -p.y) : p, re, im);
+            cdot_im_acc(lam[r], b0 ? make_v2d(-p.x, -p.y) : p, im);
         } else {
             const v2d p = psi[r];
-            cdot_acc(lam[r], (b0 ^ ((r >> K1) & 1)) ? make_v2d(-p.x, -p.y) : p, re, im);
+            cdot_im_acc(lam[r], (b0 ^ ((r >> K1) & 1)) ? make_v2d(-p.x, -p.y) : p, im);
         }
     }
 }

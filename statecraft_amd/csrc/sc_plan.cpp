@@ -581,6 +581,9 @@ uint32_t nondiag_bits(const SweepOp &op) {
     case SW_SWAP: return (1u << op.k0) | (1u << op.k1);
     case SW_DIAGL:
     case SW_DIAG16: return 0u;
+    // a probe of <lambda| G |psi> sees the same value before and after a diagonal applied to both states when
+    // G is diagonal (Z, ZZ) or acts on other bits (X, Y on k0)
+    case SW_GRAD: return (op.ta == SC_GEN_X || op.ta == SC_GEN_Y) ? 1u << op.k0 : 0u;
     default: return (1u << kRegBits) - 1u;
     }
 }
@@ -1179,10 +1182,26 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                                                   : (op.is_real || op.kind == SC_OP_CNOT || op.kind == SC_OP_SWAP ||
                                                      op.is_x));
             }
-            // ---- gather diagonals: a register diagonal moves forward to the next one when every op in
-            // between leaves its register bits' values alone (acts non-diagonally only on other bits), so
-            // runs such as Ry Rz Ry Rz .. become Ry Ry .. Rz Rz .., whose diagonals fuse into one table.
-            // Processing from the back lets whole groups form.
+            // ---- gather diagonals: a register diagonal moves back to the previous one when every op in
+            // between leaves its register bits' values alone (acts non-diagonally only on other bits) ...
+            auto reg_diag = [&](size_t k) { return register_internal(sops[k]) && is_diag_kind(sops[k]); };
+            for (size_t i = 1; i < sops.size(); i++) {
+                if (!reg_diag(i)) continue;
+                const uint32_t dep = reg_support(sops[i]);
+                size_t j = i;
+                while (j > 0 && !reg_diag(j - 1) && (nondiag_bits(sops[j - 1]) & dep) == 0u) j--;
+                if (j == 0 || j == i || !reg_diag(j - 1)) continue;
+                const SweepOp d = sops[i];
+                const bool dr = sreal[i];
+                for (size_t t = i; t > j; t--) {
+                    sops[t] = sops[t - 1];
+                    sreal[t] = sreal[t - 1];
+                }
+                sops[j] = d;
+                sreal[j] = dr;
+            }
+            // ... and forward to the next one, so runs such as Ry Rz Ry Rz .. become Ry Ry .. Rz Rz .., whose
+            // diagonals fuse into one table.  Processing from the back lets whole groups form.
             for (size_t i = sops.size(); i-- > 0;) {
                 if (!register_internal(sops[i]) || !is_diag_kind(sops[i])) continue;
                 const uint32_t dep = reg_support(sops[i]);
@@ -1301,9 +1320,32 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // ---- the unit-pivot ops' common factor: every amplitude of the chunk is multiplied by it once, in
         // the table of the first uncontrolled matrix op of the pass (a copy of its recipe, scaled), else by a
         // scaling op at the end of the last stage.  The factor is real and commutes with every op.
+        bool pass_probes = false;
+        for (int s = 0; s < pass.nstages; s++)
+            for (int i = pass.stages[s].op_begin; i < pass.stages[s].op_end; i++)
+                pass_probes = pass_probes || plan.ops[i].kind == SW_GRAD;
+        if (!pass_scale.empty() && pass_probes) {
+            // the states a probe sees lack the pivots of the unit-pivot ops before it (fused ones included)
+            std::vector<std::pair<int, int>> done;
+            auto unit_pivot = [&](const SweepOp &so) {
+                if (so.kind == SW_U1U && so.mat >= 0)
+                    done.push_back({plan.pool[so.mat].mat, plan.pool[so.mat].aux & 1});
+            };
+            for (int s = 0; s < pass.nstages; s++)
+                for (int i = pass.stages[s].op_begin; i < pass.stages[s].op_end; i++) {
+                    const SweepOp &so = plan.ops[i];
+                    if (so.kind == SW_GRAD) {
+                        plan.probe_pivots.push_back({so.slot, done});
+                    } else if (so.mat >= 0 && plan.pool[so.mat].kind == RC_FUSED) {
+                        for (const SweepOp &f : plan.pool[so.mat].fused_ops) unit_pivot(f);
+                    } else {
+                        unit_pivot(so);
+                    }
+                }
+        }
         if (!pass_scale.empty()) {
             int target = -1;
-            for (int s = 0; s < pass.nstages && target < 0; s++)
+            for (int s = 0; s < pass.nstages && target < 0 && !pass_probes; s++)
                 for (int i = pass.stages[s].op_begin; i < pass.stages[s].op_end; i++) {
                     const SweepOp &so = plan.ops[i];
                     const bool full_matrix = so.kind == SW_U1 || so.kind == SW_U1R || so.kind == SW_U2 ||
@@ -1525,6 +1567,15 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
         put(&u, 1);
     }
     return k;
+}
+
+void probe_factors(const SweepPlan &plan, const double *mats, std::vector<std::pair<uint64_t, double>> &out) {
+    out.clear();
+    for (const auto &pp : plan.probe_pivots) {
+        double f = 1.0;
+        for (const auto &src : pp.second) f *= mats[2 * ((int64_t)src.first + src.second)];
+        out.push_back({pp.first, f * f});
+    }
 }
 
 void build_pool(const SweepPlan &plan, const double *mats, double *out) {

@@ -92,7 +92,13 @@ struct SweepPlan {
     std::vector<int> standalone;   // unused for now (every op kind fits a sweep)
     int fused_blocks = 0;
     double fp64_ops = 0.0;  // FP64 operations per amplitude of the register ops over all passes
+    // Adjoint plans: per gradient probe (slot), the pivots (caller matrix offset, column) of the unit-pivot
+    // ops its pass ran before it.  Both states then carry the factor 1 / prod(pivots) (the pass's scaling op
+    // comes last), so the probe's sum is prod(pivots)^2 times too small.
+    std::vector<std::pair<uint64_t, std::vector<std::pair<int, int>>>> probe_pivots;
 };
+// prod(pivots)^2 of every probe of a plan for one matrix array: (slot, factor) pairs.
+void probe_factors(const SweepPlan &plan, const double *mats, std::vector<std::pair<uint64_t, double>> &out);
 
 // Cost model of the planner's fusion choices, in FP64 operations per amplitude.
 struct SweepCostModel {
@@ -104,8 +110,9 @@ struct SweepCostModel {
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
 constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false};
 constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true};
-// Adjoint passes: a probe between a unit-pivot op and the op its pivots fold into would see scaled states.
-constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, false};
+// Adjoint passes: as the runtime-compiled passes; a pass with gradient probes applies its unit-pivot factor
+// by a scaling op at its end, and each probe's sum is corrected on the host (SweepPlan::probe_pivots).
+constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).
