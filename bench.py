@@ -46,6 +46,8 @@ def make_context(args) -> capi.Context:
     ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
     ctx.set_option(capi.OPT_JIT, args.jit)
     ctx.set_option(capi.OPT_SWEEP_PASS_CAP, args.pass_cap)
+    if args.jit:
+        ctx.set_option(capi.OPT_CHUNK_BITS, args.chunk_bits)
     return ctx
 
 
@@ -156,7 +158,8 @@ def bench_c2(args):
         copy_gbs = (c_bytes / c_n) / (c_ms / c_n * 1e-3) / 1e9 if c_n else None
         dst.close()
     fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None, bool(args.remap), bool(args.jit),
-                                           args.pass_cap)[0]["fp64_ops_per_amp"] if args.mode == "sweep" else None)
+                                           args.pass_cap, args.chunk_bits if args.jit else 12)[0]["fp64_ops_per_amp"]
+                     if args.mode == "sweep" else None)
     traffic = None
     if args.traffic_csv:
         traffic = pmc_traffic(args.traffic_csv[0], args.traffic_csv[1], 32.0 * (1 << n))
@@ -179,7 +182,7 @@ def bench_c2(args):
             "sweeps_per_step": sweeps, "kernel_launches_per_step": plan["launches"],
             "fused_16x16_blocks": plan["fused_blocks"], "parallelism": "1 GPU",
             "low_qubits": args.low_qubits, "remap": bool(args.remap), "blocks_per_cu": args.blocks_per_cu,
-            "jit": args.jit, "pass_cap": args.pass_cap,
+            "jit": args.jit, "pass_cap": args.pass_cap, "chunk_bits": args.chunk_bits if args.jit else 12,
         },
         "roofline": {
             "bound": "hbm",
@@ -293,7 +296,8 @@ def bench_c3(args):
     res = _line(f"{n}q grover amp-updates/s", credit * args.steps / el, "amp-updates/s", args,
                 1e3 * el / args.steps,
                 {"workload": f"grover-{n}q-8-iterations", "qubits": n, "ops": len(ops), "marked": marked,
-                 "plan": ctx.plan_stats()},
+                 "plan": ctx.plan_stats(), "chunk_bits": args.chunk_bits if args.jit else 12,
+                 "low_qubits": args.low_qubits},
                 marked_probability=p, closed_form=want, rel_error=abs(p - want) / want,
                 roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS, "kernel": "sc_pass_*" if args.jit else "k_sweep"},
@@ -451,7 +455,11 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", default="sweep", choices=list(MODES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true")
-    ap.add_argument("--low-qubits", type=int, default=4)
+    ap.add_argument("--low-qubits", type=int, default=None,
+                    help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")
+    ap.add_argument("--chunk-bits", type=int, default=None, choices=[12, 13],
+                    help="SC_OPT_CHUNK_BITS: local qubits per chunk of the runtime-compiled passes (default 12; 13 "
+                         "for the 33-qubit Grover workload, whose 33-qubit layers fit two 13-qubit passes)")
     ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -464,6 +472,10 @@ def main(argv=None) -> int:
                     help="SC_OPT_JIT: 0 compiled sweep kernel, 1 runtime-compiled pass kernels, 2 the same with "
                          "next-chunk prefetch (kernels are compiled during warmup)")
     args = ap.parse_args(argv)
+    if args.chunk_bits is None:
+        args.chunk_bits = 13 if args.workload == "c3" else 12
+    if args.low_qubits is None:
+        args.low_qubits = 3 if args.chunk_bits == 13 else 4
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",
               file=sys.stderr)

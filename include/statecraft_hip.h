@@ -105,6 +105,9 @@ typedef struct sc_op {
                                    default 0 */
 #define SC_OPT_SWEEP_PASS_CAP 9 /* most estimated FP64 operations per amplitude one sweep pass takes on (0: no
                                    limit; -1: the default of the sweep kernel kind), default -1 */
+#define SC_OPT_CHUNK_BITS 10    /* local qubits per sweep chunk with runtime-compiled passes: 12 (64 KiB of LDS, two
+                                   256-thread workgroups per CU) or 13 (128 KiB, one 512-thread workgroup per CU);
+                                   default 12.  The compiled sweep kernel always uses 12. */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
@@ -226,9 +229,9 @@ int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches,
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, uint64_t *passes,
-                        uint64_t *stages,
-                        uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout);
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, int chunk_bits,
+                        uint64_t *passes, uint64_t *stages, uint64_t *fused_blocks, double *fp64_ops_per_amp,
+                        double *state_inout);
 /* Host-only (no device needed): the reverse sweep of an adjoint gradient as sweep passes (the plan the GPU's
  * runtime-compiled adjoint passes run), executed on the host: psi_inout holds the circuit's output state,
  * lam_inout H applied to it (2^n interleaved complex values each, one batch element); on return they hold
@@ -241,8 +244,8 @@ int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const
  * compile wall time and the code object size; when dump_dir is non-NULL each module's source (.hip) and
  * code object (.co) are written there. */
 int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int threads, uint64_t *kernels,
-                        double *seconds, uint64_t *code_bytes, const char *dump_dir);
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits, int threads,
+                        uint64_t *kernels, double *seconds, uint64_t *code_bytes, const char *dump_dir);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ SC_OK, SC_E_NULL, SC_E_ARG, SC_E_UNSUPPORTED, SC_E_STATE, SC_E_OOM, SC_E_HIP = r
 OP_SINGLE, OP_CNOT, OP_TWO, OP_DIAG2, OP_SWAP, OP_MCU, OP_MEASURE = range(1, 8)
 GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
 (OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP,
- OPT_SWEEP_BLOCKS_PER_CU, OPT_JIT, OPT_SWEEP_PASS_CAP) = range(1, 10)
+ OPT_SWEEP_BLOCKS_PER_CU, OPT_JIT, OPT_SWEEP_PASS_CAP, OPT_CHUNK_BITS) = range(1, 11)
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
 KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 
@@ -130,8 +130,9 @@ def _sig(lib):
         "sc_profile_reset": (i32, [P]),
         "sc_profile_read": (i32, [P, i32, U64P, DP, DP]),
         "sc_last_plan_stats": (i32, [P, U64P, U64P, U64P]),
-        "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, U64P, U64P, U64P, DP, DP]),
-        "sc_jit_compile_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, U64P, DP, U64P, ctypes.c_char_p]),
+        "sc_plan_sweeps_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, U64P, U64P, U64P, DP, DP]),
+        "sc_jit_compile_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, U64P, DP, U64P,
+                                      ctypes.c_char_p]),
         "sc_adjoint_plan_host": (i32, [i32, P, u64, DP, u64, i32, i32, DP, DP, DP]),
     }
     for name, (res, args) in s.items():
@@ -176,7 +177,7 @@ def complex_to_f64(a) -> np.ndarray:
 
 
 def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: bool = True, jit: bool = False,
-                     pass_cap: int = -1):
+                     pass_cap: int = -1, chunk_bits: int = 12):
     """Host-only sweep plan statistics; with `state` (complex 2^n) also executes the plan on the host."""
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
@@ -187,7 +188,7 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
         out = np.array(state, dtype=np.complex128, copy=True)
         ptr = dptr(out.view(np.float64))
     check(lib().sc_plan_sweeps_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(low_qubits),
-                                    1 if remap else 0, 1 if jit else 0, int(pass_cap),
+                                    1 if remap else 0, 1 if jit else 0, int(pass_cap), int(chunk_bits),
                                     ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(f), ptr))
     return {"passes": a.value, "stages": b.value, "fused_blocks": c.value, "fp64_ops_per_amp": f.value}, out
 
@@ -205,13 +206,14 @@ def adjoint_plan_host(n: int, ops, mats, n_params: int, psi, lam, low_qubits: in
 
 
 def jit_compile_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = True, threads: int = 1,
-                     dump_dir: str | None = None, variant: int = 1):
+                     dump_dir: str | None = None, variant: int = 1, chunk_bits: int = 12):
     """Host-only: compile the runtime-compiled pass kernels of a circuit's sweep plan (no device needed)."""
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
     k, sec, size = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
     check(lib().sc_jit_compile_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(low_qubits),
-                                    1 if remap else 0, int(variant), int(threads), ctypes.byref(k), ctypes.byref(sec),
+                                    1 if remap else 0, int(variant), int(chunk_bits), int(threads), ctypes.byref(k),
+                                    ctypes.byref(sec),
                                     ctypes.byref(size), dump_dir.encode() if dump_dir else None))
     return {"kernels": k.value, "seconds": sec.value, "code_bytes": size.value}
 

@@ -196,13 +196,14 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
+    const int cb = (ctx->jit && ctx->chunk_bits == kMaxChunkBits && st->n >= kMaxChunkBits) ? kMaxChunkBits : kChunkBits;
     const std::string key =
-        plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit, cm.pass_cap);
+        plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit, cm.pass_cap, cb);
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
         e->key = key;
-        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm);
+        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm, cb);
         if (rc) return rc;
         if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
         ent = ctx->plan_cache->insert(std::move(e));
@@ -263,13 +264,13 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     const double *dpool = (const double *)(d + pass_bytes + ops_bytes);
     const size_t stride = nelem_pool > 1 ? plan.pool_complex : 0;
     // keep each launch's grid under ~2^20 blocks
-    const int log_chunks = st->n - kChunkBits;
+    const int log_chunks = st->n - plan.chunk_bits;
     int per_launch = std::max(1, (int)std::min<uint64_t>((uint64_t)st->batch, (1ULL << 20) >> log_chunks));
     for (size_t p = 0; p < plan.passes.size(); p++) {
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             int cnt = std::min(per_launch, st->batch - e0);
             rc = records ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
-                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride);
+                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride, plan.chunk_bits);
             if (rc) return rc;
             ctx->last_launches++;
         }
@@ -422,6 +423,10 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value < -1 || value > 100000) return fail(SC_E_ARG, "sweep pass cap must be -1..100000");
         ctx->pass_cap = (int)value;
         return SC_OK;
+    case SC_OPT_CHUNK_BITS:
+        if (value != kChunkBits && value != kMaxChunkBits) return fail(SC_E_ARG, "chunk bits must be 12 or 13");
+        ctx->chunk_bits = (int)value;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -439,6 +444,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_SWEEP_BLOCKS_PER_CU: *value = ctx->sweep_blocks_per_cu; return SC_OK;
     case SC_OPT_JIT: *value = ctx->jit; return SC_OK;
     case SC_OPT_SWEEP_PASS_CAP: *value = ctx->pass_cap; return SC_OK;
+    case SC_OPT_CHUNK_BITS: *value = ctx->chunk_bits; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -752,12 +758,14 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 }
 
 int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, uint64_t *passes,
-                        uint64_t *stages,
-                        uint64_t *fused_blocks, double *fp64_ops_per_amp, double *state_inout) {
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, int chunk_bits,
+                        uint64_t *passes, uint64_t *stages, uint64_t *fused_blocks, double *fp64_ops_per_amp,
+                        double *state_inout) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
     if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
+    if (chunk_bits != kChunkBits && !(chunk_bits == kMaxChunkBits && jit))
+        return fail(SC_E_ARG, "chunk bits must be 12, or 13 with runtime-compiled passes");
     if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
     std::vector<const double *> elems{mats};
     std::vector<PlanOp> pops;
@@ -766,7 +774,7 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     SweepCostModel cm = jit ? kCompiledPassCost : kInterpretedCost;
     if (pass_cap >= 0) cm.pass_cap = (double)pass_cap;
     SweepPlan plan;
-    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, cm))) return rc;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, cm, chunk_bits))) return rc;
     uint64_t ns = 0;
     for (const auto &p : plan.passes) ns += (uint64_t)p.nstages;
     if (passes) *passes = plan.passes.size();
@@ -782,14 +790,16 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
 }
 
 int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
-                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int threads, uint64_t *kernels,
-                        double *seconds, uint64_t *code_bytes, const char *dump_dir) {
+                        uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits, int threads,
+                        uint64_t *kernels, double *seconds, uint64_t *code_bytes, const char *dump_dir) {
     if (n_ops > 0) SC_REQUIRE(ops, "ops");
     if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
     if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
     if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
     if (threads < 1 || threads > 64) return fail(SC_E_ARG, "threads must be 1..64");
     if (jit < 1 || jit > 3) return fail(SC_E_ARG, "jit must be 1, 2 or 3");
+    if (chunk_bits != kChunkBits && !(chunk_bits == kMaxChunkBits && jit != 3))
+        return fail(SC_E_ARG, "chunk bits must be 12, or 13 for forward passes");
     SweepPlan plan;
     int rc;
     if (jit == 3) {  // the circuit's adjoint reverse sweep (parameterized ops carry param / gen)
@@ -804,7 +814,9 @@ int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
         std::vector<const double *> elems{mats};
         std::vector<PlanOp> pops;
         if ((rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops))) return rc;
-        if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost))) return rc;
+        if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost,
+                              chunk_bits)))
+            return rc;
     }
     if (kernels) *kernels = plan.passes.size();
     return jit_compile_host(plan, jit, threads, seconds, code_bytes, dump_dir);

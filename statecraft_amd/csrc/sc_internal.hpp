@@ -64,7 +64,8 @@ struct Context {
     int remap = 1;
     int sweep_blocks_per_cu = 0;
     int jit = 0;
-    int pass_cap = -1;  // SC_OPT_SWEEP_PASS_CAP
+    int pass_cap = -1;
+    int chunk_bits = 12;  // SC_OPT_CHUNK_BITS  // SC_OPT_SWEEP_PASS_CAP
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)
@@ -113,7 +114,7 @@ constexpr int kSwzBits = 4;  // LDS slot bits permuted by the XOR swizzle
 struct SweepOp {
     int8_t kind;
     int8_t k0, k1;   // register bit positions (-1: none)
-    int8_t pad0;
+    uint8_t thi;     // SW_DIAGL with 512 threads: bit 8 of ta (bit 0) and of tb (bit 1)
     int32_t mat;     // complex offset of the op's matrix / factor table in the pass pool
     uint16_t rctrl;  // register-index control mask
     uint16_t tctrl;  // thread-index control mask
@@ -132,7 +133,7 @@ struct SweepStage {
     int32_t wave_local;
     int32_t pad;
     uint32_t lds_off[kRegs];            // swizzled LDS slot offset per register index
-    uint32_t thread_slot[kThreadBits];  // swizzled slot contribution of each thread-index bit
+    uint32_t thread_slot[kMaxThreadBits];  // swizzled slot contribution of each thread-index bit
 };
 
 struct SweepPass {
@@ -141,13 +142,13 @@ struct SweepPass {
     // bit 0: the last stage -> store phase exchange stays inside each wavefront; bit 1: so does the store
     // phase -> next chunk's load phase (the workgroup may overwrite its LDS without a barrier)
     int32_t wave_local;
-    uint64_t glob_thread[kThreadBits];  // load/store phase: global offset of thread-index bit i
+    uint64_t glob_thread[kMaxThreadBits];  // load/store phase: global offset of thread-index bit i
     uint64_t glob_reg[kRegs];       // load/store phase: global offset of register index r
-    uint32_t io_thread_slot[kThreadBits];  // load phase: swizzled slot of thread-index bit i
+    uint32_t io_thread_slot[kMaxThreadBits];  // load phase: swizzled slot of thread-index bit i
     uint32_t io_reg_slot[kRegs];
     // Store phase: thread bit i / register bit j still address chunk positions i / 8+j in HBM, but read the
     // LDS slot of the local qubit the pass moves there.  Equal to io_* when the pass does not remap qubits.
-    uint32_t st_thread_slot[kThreadBits];
+    uint32_t st_thread_slot[kMaxThreadBits];
     uint32_t st_reg_slot[kRegs];
     SweepStage stages[kMaxStages];
 };

@@ -107,7 +107,8 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
     case SW_DIAG16: s << "op_diag16(" << sd << ", " << m << ");"; break;
     case SW_DIAGL:
         s << "op_diagl_sel<" << (k0 < 0 ? kRegBits : k0) << ">(" << sd << ", " << m << ", " << (unsigned)o.rctrl
-          << "u, " << (unsigned)o.tctrl << "u, " << (unsigned)o.ta << "u, " << (unsigned)o.tb << "u, " << (int)o.oa
+          << "u, " << (unsigned)o.tctrl << "u, " << (unsigned)(o.ta | ((o.thi & 1u) << 8)) << "u, "
+          << (unsigned)(o.tb | (((o.thi >> 1) & 1u) << 8)) << "u, " << (int)o.oa
           << ", " << (int)o.ob << ", tid, cb);";
         break;
     case SW_CX: s << "op_cx<" << k0 << ", " << k1 << ">(" << sd << ");"; break;
@@ -130,11 +131,13 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
 // that they arrive while the chunk is computed; otherwise they are issued after the stages, as in k_sweep.
 std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch) {
     std::ostringstream s;
-    s << "extern \"C\" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
+    const int TB = plan.chunk_bits - kRegBits;  // thread-index bits: 8 (256 threads) or 9 (512)
+    s << "extern \"C\" __global__ void __launch_bounds__(" << (1 << TB)
+      << ") __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
          "@K@(sc::v2d *__restrict__ amp, int n, int log_chunks, uint64_t total_chunks,\n"
          "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin) {\n"
          "  using namespace sc;\n"
-         "  __shared__ v2d lds[kChunk];\n"
+         "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n"
          "  CV2 *mats = (CV2 *)mats_arg;\n"
          "  const uint32_t tid = threadIdx.x;\n"
          // with one matrix pool per element (mat_stride != 0) a workgroup takes a contiguous range of chunks,
@@ -148,11 +151,11 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
          "  auto cbase = [](uint64_t x) -> uint64_t { return "
       << deposit_expr("x", ps.outer_mask) << "; };\n";
-    s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, kThreadBits, "+", true) << ";\n";
+    s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, TB, "+", true) << ";\n";
     // LDS addresses are byte offsets (slot * 16), so that each access costs one XOR
     s << "  char *const ldsb = (char *)lds;\n";
-    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, kThreadBits, "^", false, 16) << ";\n";
-    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, kThreadBits, "^", false, 16) << ";\n";
+    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, TB, "^", false, 16) << ";\n";
+    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, TB, "^", false, 16) << ";\n";
     auto regs_load = [&](const char *dst, const char *srcp) {
         for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << ps.glob_reg[r] << "];\n";
     };
@@ -196,7 +199,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         s << "    {\n"
              "    const uint32_t t = opaque(tid);\n"
              "    const uint32_t ts = "
-          << lane_expr("t", stg.thread_slot, kThreadBits, "^", false, 16) << ";\n";
+          << lane_expr("t", stg.thread_slot, TB, "^", false, 16) << ";\n";
         lds_read("a", "ts", stg.lds_off);
         bool in_a = true;
         for (int o = stg.op_begin; o < stg.op_end; o++) {
@@ -555,17 +558,21 @@ int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *se
 }
 
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
-                     int elem_count, size_t mat_stride) {
-    int log_chunks = st->n - kChunkBits;
+                     int elem_count, size_t mat_stride, int chunk_bits) {
+    int log_chunks = st->n - chunk_bits;
     uint64_t total = (uint64_t)elem_count << log_chunks;
-    const uint64_t grid = std::min<uint64_t>(total, (uint64_t)sweep_grid_size(ctx));
+    // 12-qubit chunks: as many 256-thread workgroups per CU as k_sweep's occupancy allows (two 64 KiB LDS
+    // chunks); 13-qubit chunks: one 512-thread workgroup per CU (128 KiB of LDS)
+    const int threads = 1 << (chunk_bits - kRegBits);
+    const uint64_t resident = chunk_bits == kChunkBits ? (uint64_t)sweep_grid_size(ctx) : (uint64_t)ctx->cu_count;
+    const uint64_t grid = std::min<uint64_t>(total, resident);
     v2d *amp = reinterpret_cast<v2d *>(st->amp);
     const v2d *mats = reinterpret_cast<const v2d *>(mats_dev);
     int n = st->n;
     uint64_t stride = (uint64_t)mat_stride;
     void *args[] = {&amp, &n, &log_chunks, &total, &mats, &stride, &elem_begin};
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
-    SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, kThreads, 1, 1, 0, ctx->stream, args, nullptr));
+    SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, threads, 1, 1, 0, ctx->stream, args, nullptr));
     return SC_OK;
 }
 

@@ -31,6 +31,10 @@ constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
 constexpr int kThreadBits = kChunkBits - kRegBits;  // 8
 constexpr int kThreads = 1 << kThreadBits;          // 256
+// Runtime-compiled passes may also use 13-qubit chunks (128 KiB of LDS, 512 threads, one workgroup per CU);
+// plan tables are sized for the larger geometry.
+constexpr int kMaxChunkBits = 13;
+constexpr int kMaxThreadBits = kMaxChunkBits - kRegBits;  // 9
 
 // Register-level op kinds of a stage.  Every kind reads the 16 source registers and writes all 16
 // destination registers (the kernel alternates between two register sets).  Any op whose outer

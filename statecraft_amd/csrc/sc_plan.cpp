@@ -22,6 +22,10 @@ namespace {
 
 int popc(uint64_t x) { return __builtin_popcountll(x); }
 
+// Thread-index masks of a SW_DIAGL's two bit sources (9 bits with 512-thread chunks).
+uint32_t diag_ta(const SweepOp &op) { return op.ta | ((op.thi & 1u) << 8); }
+uint32_t diag_tb(const SweepOp &op) { return op.tb | (((op.thi >> 1) & 1u) << 8); }
+
 bool is_exact(double v, double want) { return v == want; }
 
 // 2x2 complex g (8 doubles): diagonal when both off-diagonal entries are exactly zero.
@@ -462,8 +466,8 @@ void host_apply(const SweepOp &op, const double *m, double *a, uint32_t tid, uin
         break;
     case SW_DIAGL: {
         if (!lane) break;
-        const int bl = ((tid & op.tb) != 0u || (op.ob >= 0 && ((cb >> op.ob) & 1ULL))) ? 1 : 0;
-        const int al = ((tid & op.ta) != 0u || (op.oa >= 0 && ((cb >> op.oa) & 1ULL))) ? 1 : 0;
+        const int bl = ((tid & diag_tb(op)) != 0u || (op.ob >= 0 && ((cb >> op.ob) & 1ULL))) ? 1 : 0;
+        const int al = ((tid & diag_ta(op)) != 0u || (op.oa >= 0 && ((cb >> op.oa) & 1ULL))) ? 1 : 0;
         for (int r = 0; r < kRegs; r++) {
             if (((uint32_t)r & op.rctrl) != op.rctrl) continue;
             const int A = op.k0 >= 0 ? (r >> op.k0) & 1 : al;
@@ -592,7 +596,7 @@ uint32_t nondiag_bits(const SweepOp &op) {
 // into a fused block or diagonal table.
 bool register_internal(const SweepOp &op) {
     if (op.tctrl != 0u || op.octrl != 0ULL) return false;
-    if (op.kind == SW_DIAGL) return op.ta == 0u && op.tb == 0u && op.oa < 0 && op.ob < 0 && op.k0 >= 0;
+    if (op.kind == SW_DIAGL) return diag_ta(op) == 0u && diag_tb(op) == 0u && op.oa < 0 && op.ob < 0 && op.k0 >= 0;
     return op.kind == SW_U1 || op.kind == SW_U1R || op.kind == SW_U1C || op.kind == SW_X || op.kind == SW_CX ||
            op.kind == SW_U2 || op.kind == SW_U2R || op.kind == SW_SWAP || op.kind == SW_U1U;
 }
@@ -600,11 +604,15 @@ bool register_internal(const SweepOp &op) {
 }  // namespace
 
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
-                int max_fuse_bits, const SweepCostModel &cm) {
+                int max_fuse_bits, const SweepCostModel &cm, int chunk_bits) {
     plan = SweepPlan();
-    if (n < kChunkBits) return fail(SC_E_ARG, "sweep plans need at least 12 qubits");
+    if (chunk_bits != kChunkBits && chunk_bits != kMaxChunkBits)
+        return fail(SC_E_ARG, "sweep chunks hold 12 or 13 qubits");
+    const int CB = chunk_bits, TB = CB - kRegBits;  // local qubits per chunk, thread-index bits
+    plan.chunk_bits = CB;
+    if (n < CB) return fail(SC_E_ARG, "sweep plans need at least " + std::to_string(CB) + " qubits");
     const uint64_t full = (n == 64) ? ~0ULL : ((1ULL << n) - 1ULL);
-    const int lb = std::max(0, std::min(low_qubits, kChunkBits));
+    const int lb = std::max(0, std::min(low_qubits, CB));
     const uint64_t lowmask = (1ULL << lb) - 1ULL;
 
     std::map<std::tuple<int, int, int>, int> recipe_index;  // (kind, mat, aux) -> recipe
@@ -658,7 +666,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 const PlanOp &op = ops[i];
                 const double w = plan_op_fp64(op);
                 const uint64_t need = local_need(op);
-                const bool fits = grow ? popc(LL | need) <= kChunkBits : (need & ~LL) == 0;
+                const bool fits = grow ? popc(LL | need) <= CB : (need & ~LL) == 0;
                 if (b.blocked(op) || !fits || popc(op.targets) > kRegBits ||
                     (cm.pass_cap > 0.0 && w > 0.0 && wk + w > cm.pass_cap)) {
                     b.mark(op);
@@ -676,13 +684,13 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // first guess: the targets of the earliest takeable ops, filled with the lowest positions
         take(L, true, &L, nullptr, nullptr);
         if (!fixup) {
-            for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
+            for (int p = 0; p < n && popc(L) < CB; p++) L |= 1ULL << logi[p];
             // hill climbing: exchange one free local qubit for an outer target qubit while that lets the
             // pass take on more ops (on layered circuits this halves the number of passes)
             uint64_t cand = 0;
             for (int i : remaining) cand |= local_need(ops[i]);
             int best = take(L, false, nullptr, nullptr, nullptr);
-            for (int iter = 0; iter < 4 * kChunkBits; iter++) {
+            for (int iter = 0; iter < 4 * CB; iter++) {
                 bool improved = false;
                 for (uint64_t fa = L & ~fixed; fa && !improved; fa &= fa - 1) {
                     const uint64_t a = fa & (~fa + 1ULL);
@@ -706,15 +714,15 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         if (fixup) {
             // bring logical 0..lb-1 home, then misplaced qubits together with the occupant of their home
             for (int q = 0; q < lb; q++)
-                if (popc(L | (1ULL << q)) <= kChunkBits) L |= 1ULL << q;
+                if (popc(L | (1ULL << q)) <= CB) L |= 1ULL << q;
             for (int q = 0; q < n; q++) {
                 if (phys[q] == q) continue;
                 uint64_t add = (1ULL << q) | (1ULL << logi[q]);
-                if (popc(L | add) <= kChunkBits) L |= add;
+                if (popc(L | add) <= CB) L |= add;
             }
         }
         // ---- local qubits: fill to 12 with the qubits at the lowest unused physical positions
-        for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
+        for (int p = 0; p < n && popc(L) < CB; p++) L |= 1ULL << logi[p];
         // ---- split into stages
         std::vector<StageBuild> stages;
         std::vector<int> rest = sched;
@@ -787,10 +795,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             next.swap(merged);
         }
         // ---- local qubits: fill to 12 with the qubits at the lowest unused physical positions
-        for (int p = 0; p < n && popc(L) < kChunkBits; p++) L |= 1ULL << logi[p];
-        if (popc(L) != kChunkBits) return fail(SC_E_STATE, "internal error: a sweep pass needs 12 local qubits");
+        for (int p = 0; p < n && popc(L) < CB; p++) L |= 1ULL << logi[p];
+        if (popc(L) != CB) return fail(SC_E_STATE, "internal error: a sweep pass needs 12 local qubits");
         if (plan.passes.size() > 4 * ops.size() + 64) return fail(SC_E_STATE, "internal error: sweep planning stalled");
-        int lq[kChunkBits];  // local position j -> physical position (ascending)
+        int lq[kMaxChunkBits];  // local position j -> physical position (ascending)
         int lp[64];          // logical qubit -> local position, -1 when outer
         uint64_t Lphys = 0;
         for (int q = 0; q < 64; q++) lp[q] = -1;
@@ -810,20 +818,20 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         pass.nstages = (int)stages.size();
 
         // ---- placement after the pass: out_q[k] is the logical qubit written to chunk position k.
-        int out_q[kChunkBits];
+        int out_q[kMaxChunkBits];
         {
-            bool used[kChunkBits] = {false};
+            bool used[kMaxChunkBits] = {false};
             uint64_t placed = 0;
             std::vector<int> front;  // qubits wanted at the low positions, in priority order
             if (!remap) {
-                for (int k = 0; k < kChunkBits; k++) {
+                for (int k = 0; k < CB; k++) {
                     out_q[k] = logi[lq[k]];
                     used[k] = true;
                     placed |= 1ULL << out_q[k];
                 }
             } else if (fixup || next.empty()) {
                 // restore: every local qubit whose home is one of the chunk's positions goes home
-                for (int k = 0; k < kChunkBits; k++)
+                for (int k = 0; k < CB; k++)
                     if ((L >> lq[k]) & 1ULL) {
                         out_q[k] = lq[k];
                         used[k] = true;
@@ -835,7 +843,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 uint64_t T = 0;
                 for (int i : next) {
                     const PlanOp &op = ops[i];
-                    if (b2.blocked(op) || popc(T | op.targets) > kChunkBits) {
+                    if (b2.blocked(op) || popc(T | op.targets) > CB) {
                         b2.mark(op);
                         continue;
                     }
@@ -850,7 +858,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     placed |= 1ULL << front[k];
                 }
                 // everything else stays where it is when its position is free
-                for (int k = 0; k < kChunkBits; k++) {
+                for (int k = 0; k < CB; k++) {
                     int q = logi[lq[k]];
                     if (!used[k] && !((placed >> q) & 1ULL)) {
                         out_q[k] = q;
@@ -861,7 +869,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             }
             // leftovers fill the remaining positions in order
             int k = 0;
-            for (int j = 0; j < kChunkBits; j++) {
+            for (int j = 0; j < CB; j++) {
                 int q = logi[lq[j]];
                 if ((placed >> q) & 1ULL) continue;
                 while (used[k]) k++;
@@ -870,13 +878,13 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 placed |= 1ULL << q;
             }
         }
-        int pi[kChunkBits];  // chunk position k reads local position pi[k]
-        for (int k = 0; k < kChunkBits; k++) pi[k] = lp[out_q[k]];
+        int pi[kMaxChunkBits];  // chunk position k reads local position pi[k]
+        for (int k = 0; k < CB; k++) pi[k] = lp[out_q[k]];
 
         // per stage: register positions / thread positions
         struct StageLayout {
             int rpos[kRegBits];
-            int tpos[kThreadBits];
+            int tpos[kMaxThreadBits];
             bool wave_local;
         };
         std::vector<StageLayout> layouts(stages.size());
@@ -889,19 +897,19 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             // spare register bits go to the local qubits the stage's ops use as controls / diagonal sources
             // (most uses first): such ops then resolve per register index instead of per lane, and can fuse
             if (popc(rp) < kRegBits) {
-                int uses[kChunkBits] = {0};
+                int uses[kMaxChunkBits] = {0};
                 for (int oi : stages[s].ops)
                     for (int q = 0; q < n; q++)
                         if (((ops[oi].diag >> q) & 1ULL) && lp[q] >= 0) uses[lp[q]]++;
                 while (popc(rp) < kRegBits) {
                     int best = -1;
-                    for (int p = 0; p < kChunkBits; p++)
+                    for (int p = 0; p < CB; p++)
                         if (!((rp >> p) & 1u) && uses[p] > 0 && (best < 0 || uses[p] > uses[best])) best = p;
                     if (best < 0) break;
                     rp |= 1u << best;
                 }
             }
-            for (int p = kChunkBits - 1; p >= 0 && popc(rp) < kRegBits; p--)
+            for (int p = CB - 1; p >= 0 && popc(rp) < kRegBits; p--)
                 if (!((rp >> p) & 1u)) rp |= 1u << p;
             stage_rp[s] = rp;
         }
@@ -912,14 +920,14 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // starting from the load phase's pair (6, 7) and ending at the store phase's (pi[6], pi[7]).
         {
             const int S = (int)stages.size();
-            const int kPairs = kChunkBits * kChunkBits;
+            const int kPairs = CB * CB;
             const int INF = 1 << 28;
             std::vector<std::vector<int>> cost(S, std::vector<int>(kPairs, INF)), from(S, std::vector<int>(kPairs, -1));
             auto ok = [&](int s, int w) {
-                const int a = w / kChunkBits, b = w % kChunkBits;
+                const int a = w / CB, b = w % CB;
                 return a != b && !((stage_rp[s] >> a) & 1u) && !((stage_rp[s] >> b) & 1u);
             };
-            const int w_load = 6 * kChunkBits + 7, w_store = pi[6] * kChunkBits + pi[7];
+            const int w_load = 6 * CB + 7, w_store = pi[6] * CB + pi[7];
             for (int w = 0; w < kPairs; w++)
                 if (S > 0 && ok(0, w)) cost[0][w] = (w != w_load);
             for (int s = 1; s < S; s++)
@@ -949,24 +957,32 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             }
             for (int s = 0; s < S; s++) {
                 const uint32_t rp = stage_rp[s];
-                const int a = wsel[s] / kChunkBits, b = wsel[s] % kChunkBits;
+                const int a = wsel[s] / CB, b = wsel[s] % CB;
                 int rj = 0, tj = 0;
-                for (int p = 0; p < kChunkBits; p++) {
-                    if ((rp >> p) & 1u) layouts[s].rpos[rj++] = p;
-                    else if (p != a && p != b) layouts[s].tpos[tj++] = p;
+                for (int p = 0; p < CB; p++) {
+                    if ((rp >> p) & 1u) {
+                        layouts[s].rpos[rj++] = p;
+                    } else if (p != a && p != b) {
+                        layouts[s].tpos[tj++] = p;
+                        if (tj == 6) tj = 8;  // 512 threads: thread bit 8 is a third wave-index bit
+                    }
                 }
                 layouts[s].tpos[6] = a;
                 layouts[s].tpos[7] = b;
-                layouts[s].wave_local = wsel[s] == (s == 0 ? w_load : wsel[s - 1]);
+                // with 512 threads the exchange stays in the wavefront only if bit 8 keeps its position too:
+                // such chunks always synchronise the workgroup
+                layouts[s].wave_local = TB == 8 && wsel[s] == (s == 0 ? w_load : wsel[s - 1]);
             }
             pass_wave_local = 0;
-            if (S > 0 ? wsel[S - 1] == w_store : w_load == w_store) pass_wave_local |= 1;
-            if (w_store == w_load) pass_wave_local |= 2;
+            if (TB == 8) {
+                if (S > 0 ? wsel[S - 1] == w_store : w_load == w_store) pass_wave_local |= 1;
+                if (w_store == w_load) pass_wave_local |= 2;
+            }
             pass.wave_local = pass_wave_local;
         }
         // ---- swizzle: hv[p] for local bits p >= kSwzBits, chosen so every stage's lanes 0..3 (and 0..2)
         // land on distinct low slot bits.
-        uint32_t hv[kChunkBits] = {0};
+        uint32_t hv[kMaxChunkBits] = {0};
         {
             uint64_t rng = 0x243F6A8885A308D3ULL;
             auto next_rand = [&]() {
@@ -976,10 +992,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 return rng;
             };
             int best_bad = 1 << 30;
-            uint32_t best[kChunkBits] = {0};
+            uint32_t best[kMaxChunkBits] = {0};
             for (int attempt = 0; attempt < 4000 && best_bad > 0; attempt++) {
-                uint32_t cand[kChunkBits] = {0};
-                for (int p = kSwzBits; p < kChunkBits; p++) cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & 15u);
+                uint32_t cand[kMaxChunkBits] = {0};
+                for (int p = kSwzBits; p < CB; p++) cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & 15u);
                 int bad = 0;
                 for (size_t s = 0; s <= stages.size(); s++) {  // every stage, then the store phase
                     std::vector<uint32_t> c16, c8;
@@ -1001,12 +1017,12 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         }
         auto slot_of = [&](uint32_t l) -> uint32_t {
             uint32_t h = 0;
-            for (int p = kSwzBits; p < kChunkBits; p++)
+            for (int p = kSwzBits; p < CB; p++)
                 if ((l >> p) & 1u) h ^= hv[p];
             return l ^ h;
         };
         // load / store phase layout: thread bit i -> local i, register bit j -> local 8 + j
-        for (int i = 0; i < kThreadBits; i++) {
+        for (int i = 0; i < TB; i++) {
             pass.glob_thread[i] = 1ULL << lq[i];
             pass.io_thread_slot[i] = slot_of(1u << i);
         }
@@ -1015,18 +1031,18 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             uint32_t l = 0;
             for (int j = 0; j < kRegBits; j++)
                 if ((r >> j) & 1) {
-                    g |= 1ULL << lq[kThreadBits + j];
-                    l |= 1u << (kThreadBits + j);
+                    g |= 1ULL << lq[TB + j];
+                    l |= 1u << (TB + j);
                 }
             pass.glob_reg[r] = g;
             pass.io_reg_slot[r] = slot_of(l);
         }
         // store phase: chunk position k takes the value of local position pi[k]
-        for (int i = 0; i < kThreadBits; i++) pass.st_thread_slot[i] = slot_of(1u << pi[i]);
+        for (int i = 0; i < TB; i++) pass.st_thread_slot[i] = slot_of(1u << pi[i]);
         for (int r = 0; r < kRegs; r++) {
             uint32_t l = 0;
             for (int j = 0; j < kRegBits; j++)
-                if ((r >> j) & 1) l |= 1u << pi[kThreadBits + j];
+                if ((r >> j) & 1) l |= 1u << pi[TB + j];
             pass.st_reg_slot[r] = slot_of(l);
         }
         // ---- stage ops
@@ -1035,7 +1051,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             SweepStage &stg = pass.stages[s];
             const StageLayout &lay = layouts[s];
             stg.wave_local = lay.wave_local ? 1 : 0;
-            for (int i = 0; i < kThreadBits; i++) stg.thread_slot[i] = slot_of(1u << lay.tpos[i]);
+            for (int i = 0; i < TB; i++) stg.thread_slot[i] = slot_of(1u << lay.tpos[i]);
             for (int r = 0; r < kRegs; r++) {
                 uint32_t l = 0;
                 for (int j = 0; j < kRegBits; j++)
@@ -1060,7 +1076,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 if (lp[q] >= 0) {
                     b.reg = reg_of(q);
                     if (b.reg < 0)
-                        for (int i = 0; i < kThreadBits; i++)
+                        for (int i = 0; i < TB; i++)
                             if (lay.tpos[i] == lp[q]) b.t = 1u << i;
                 } else {
                     b.opos = phys[q];
@@ -1119,6 +1135,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                         so.kind = SW_DIAGL;
                         so.k0 = (int8_t)a.reg;
                         so.ta = (uint8_t)a.t;
+                        so.thi = (uint8_t)((a.t >> 8) & 1u);
                         so.oa = (int8_t)a.opos;
                         so.mat = recipe_for(RC_DIAG1, op.mat, 4);
                     } else if (op.is_x) {
@@ -1166,6 +1183,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                     so.k1 = (int8_t)b.reg;
                     so.tb = (uint8_t)b.t;
                     so.ob = (int8_t)b.opos;
+                    so.thi = (uint8_t)(((a.t >> 8) & 1u) | (((b.t >> 8) & 1u) << 1));
                     so.mat = recipe_for(rk, op.mat, 4);
                     break;
                 }
@@ -1377,7 +1395,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             plan.ops[target].mat = (int)plan.pool.size() - 1;
         }
         plan.passes.push_back(pass);
-        for (int k = 0; k < kChunkBits; k++) {
+        for (int k = 0; k < CB; k++) {
             phys[out_q[k]] = lq[k];
             logi[lq[k]] = out_q[k];
         }
@@ -1461,10 +1479,12 @@ void host_grad(const SweepOp &op, const double *psi, const double *lam, double *
 }  // namespace
 
 void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double *state, double *lam, double *grads) {
-    const int log_chunks = n - kChunkBits;
+    const int CB = plan.chunk_bits, TB = CB - kRegBits;
+    const uint32_t threads = 1u << TB;
+    const int log_chunks = n - CB;
     const int ns = lam ? 2 : 1;
     double *states[2] = {state, lam};
-    std::vector<double> lds[2] = {std::vector<double>(2 * kChunk), std::vector<double>(2 * kChunk)};
+    std::vector<double> lds[2] = {std::vector<double>(2ULL << CB), std::vector<double>(2ULL << CB)};
     double a[2][2 * kRegs];
     for (const SweepPass &pass : plan.passes) {
         for (uint64_t chunk = 0; chunk < (1ULL << log_chunks); chunk++) {
@@ -1476,10 +1496,10 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 om &= om - 1ULL;
             }
             for (int k = 0; k < ns; k++)
-                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                for (uint32_t tid = 0; tid < threads; tid++) {
                     uint64_t g = 0;
                     uint32_t s = 0;
-                    for (int i = 0; i < kThreadBits; i++)
+                    for (int i = 0; i < TB; i++)
                         if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.io_thread_slot[i];
                     for (int r = 0; r < kRegs; r++) {
                         uint64_t gi = chunk_base + g + pass.glob_reg[r];
@@ -1490,9 +1510,9 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 }
             for (int st = 0; st < pass.nstages; st++) {
                 const SweepStage &stg = pass.stages[st];
-                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                for (uint32_t tid = 0; tid < threads; tid++) {
                     uint32_t ts = 0;
-                    for (int i = 0; i < kThreadBits; i++)
+                    for (int i = 0; i < TB; i++)
                         if ((tid >> i) & 1u) ts ^= stg.thread_slot[i];
                     for (int k = 0; k < ns; k++)
                         for (int r = 0; r < kRegs; r++) {
@@ -1515,10 +1535,10 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 }
             }
             for (int k = 0; k < ns; k++)
-                for (uint32_t tid = 0; tid < (uint32_t)kThreads; tid++) {
+                for (uint32_t tid = 0; tid < threads; tid++) {
                     uint64_t g = 0;
                     uint32_t s = 0;
-                    for (int i = 0; i < kThreadBits; i++)
+                    for (int i = 0; i < TB; i++)
                         if ((tid >> i) & 1u) g += pass.glob_thread[i], s ^= pass.st_thread_slot[i];
                     for (int r = 0; r < kRegs; r++) {
                         uint64_t gi = chunk_base + g + pass.glob_reg[r];
@@ -1549,11 +1569,11 @@ PlanCache::Entry *PlanCache::insert(std::unique_ptr<Entry> e) {
 }
 
 std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
-                           int jit, double pass_cap) {
+                           int jit, double pass_cap, int chunk_bits) {
     std::string k;
     k.reserve(32 + ops.size() * 48);
     auto put = [&](const void *p, size_t bytes) { k.append((const char *)p, bytes); };
-    const int32_t head[5] = {n, low_qubits, remap ? 1 : 0, max_fuse_bits, jit};
+    const int32_t head[6] = {n, low_qubits, remap ? 1 : 0, max_fuse_bits, jit, chunk_bits};
     put(head, sizeof head);
     put(&pass_cap, sizeof pass_cap);
     for (const PlanOp &op : ops) {

@@ -85,6 +85,7 @@ struct PoolRecipe {
 };
 
 struct SweepPlan {
+    int chunk_bits = kChunkBits;  // local qubits per chunk: 12, or 13 (runtime-compiled passes only)
     std::vector<SweepPass> passes;
     std::vector<SweepOp> ops;      // all passes' ops (stage ranges index this)
     std::vector<PoolRecipe> pool;  // recipes; SweepOp.mat = complex offset of the recipe's output
@@ -117,7 +118,8 @@ constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true};
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
-                int max_fuse_bits = kRegBits, const SweepCostModel &cm = kInterpretedCost);
+                int max_fuse_bits = kRegBits, const SweepCostModel &cm = kInterpretedCost,
+                int chunk_bits = kChunkBits);
 // Plans of recently executed circuits.  A plan depends on the op structure (kinds, qubits, controls,
 // matrix offsets and the diagonal / real / exchange classification of every matrix) and the options that
 // shape it, not on matrix values, so circuits that differ only in parameters share one entry.
@@ -135,7 +137,7 @@ struct PlanCache {
     Entry *insert(std::unique_ptr<Entry> e);  // evicts the least recently used entry when full
 };
 std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
-                           int jit, double pass_cap);
+                           int jit, double pass_cap, int chunk_bits = kChunkBits);
 
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);

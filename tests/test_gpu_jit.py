@@ -159,3 +159,66 @@ def test_jit_batched_unit_pivots_per_element(jit_ctx):
     got = run(jit_ctx, n, ops, mats, init, batch=B, batched=True)
     for b in range(B):
         assert np.max(np.abs(got[b] - oracle.run_ops(n, ops, mats[b], init[b]))) <= EPS, b
+
+
+@pytest.fixture(params=[1, 2], ids=["jit-13", "jit-prefetch-13"])
+def jit13_ctx(hip_ctx, request):
+    """Runtime-compiled passes over 13-qubit chunks (512 threads, 128 KiB of LDS, one workgroup per CU)."""
+    hip_ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_SWEEP)
+    hip_ctx.set_option(capi.OPT_JIT, request.param)
+    hip_ctx.set_option(capi.OPT_CHUNK_BITS, 13)
+    yield hip_ctx
+    hip_ctx.set_option(capi.OPT_CHUNK_BITS, 12)
+    hip_ctx.set_option(capi.OPT_JIT, 0)
+    hip_ctx.set_option(capi.OPT_LOW_QUBITS, 4)
+    hip_ctx.set_option(capi.OPT_REMAP, 1)
+
+
+@pytest.mark.parametrize("n,low,remap", [(13, 0, 1), (14, 3, 0), (16, 4, 1), (18, 3, 0), (17, 8, 1)])
+def test_jit13_random_circuits(jit13_ctx, n, low, remap):
+    rng = np.random.default_rng(53 * n + low)
+    ops, mats = random_circuit(n, 300, rng)
+    init = random_state(n, rng)
+    jit13_ctx.set_option(capi.OPT_LOW_QUBITS, low)
+    jit13_ctx.set_option(capi.OPT_REMAP, remap)
+    got = run(jit13_ctx, n, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+
+
+def test_jit13_small_states_fall_back_to_twelve(jit13_ctx):
+    """A 12-qubit state has no 13-qubit chunk: the plan uses 12."""
+    rng = np.random.default_rng(4)
+    ops, mats = random_circuit(12, 80, rng)
+    init = random_state(12, rng)
+    got = run(jit13_ctx, 12, ops, mats, init)
+    assert np.max(np.abs(got - oracle.run_ops(12, ops, mats, init))) <= EPS
+
+
+def test_jit13_batched_unit_pivots_and_layered(jit13_ctx):
+    rng = np.random.default_rng(19)
+    n, B = 15, 3
+    ops, mats0 = unit_pivot_circuit(n, 150, rng, True, True)
+    mats = np.stack([np.asarray(mats0) * np.exp(1j * 0.2 * b) for b in range(B)])
+    init = np.stack([random_state(n, rng) for _ in range(B)])
+    got = run(jit13_ctx, n, ops, mats, init, batch=B, batched=True)
+    for b in range(B):
+        assert np.max(np.abs(got[b] - oracle.run_ops(n, ops, mats[b], init[b]))) <= EPS, b
+    jit13_ctx.set_option(capi.OPT_REMAP, 0)
+    n = 20
+    ops, mats = lower(workloads.random_layered(n, 12, 5))
+    got = run(jit13_ctx, n, ops, mats)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS
+
+
+def test_jit13_grover_closed_form(jit13_ctx):
+    """Grover on 20 qubits with 3 low positions: the marked probability follows the closed form."""
+    jit13_ctx.set_option(capi.OPT_LOW_QUBITS, 3)
+    jit13_ctx.set_option(capi.OPT_REMAP, 0)
+    n, it = 20, 3
+    circ, marked = workloads.grover(n, it, 7)
+    ops, mats = lower(circ)
+    st = capi.State(jit13_ctx, n)
+    st.apply(ops, mats)
+    p = abs(st.read(marked)[0]) ** 2
+    st.close()
+    assert abs(p - workloads.grover_marked_probability(n, it)) <= 1e-12

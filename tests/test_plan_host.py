@@ -260,3 +260,41 @@ def test_adjoint_plan_matches_parameter_shift(kind, n):
             want = (energy(tp) - energy(tm)) / (2 * h)
         tol = 1e-10 if shift is not None else 1e-6
         assert abs(g[k] - want) <= tol, (k, g[k], want)
+
+
+@pytest.mark.parametrize("n,low,remap", [(13, 0, False), (13, 4, True), (14, 3, False), (15, 4, True), (16, 2, False),
+                                         (17, 4, False)])
+def test_thirteen_qubit_chunks_match_oracle(n, low, remap):
+    """13-qubit chunks (512 threads, 9 thread-index bits, 128 KiB of LDS) for runtime-compiled passes: every op
+    kind, thread-bit diagonal sources on the ninth thread bit included."""
+    rng = np.random.default_rng(100 + n)
+    ops, mats = random_circuit(n, 90, rng)
+    init = random_state(n, rng)
+    stats, out = capi.plan_sweeps_host(n, ops, mats, low, init, remap, True, chunk_bits=13)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+    assert stats["passes"] >= 1
+
+
+def test_thirteen_qubit_chunks_cut_grover_passes(tmp_path):
+    """Grover's full-width layers between multi-controlled Z's: 12-qubit chunks whose 4 low positions were local
+    in the previous pass cover at most 28 qubits of a layer in two passes, so a 33-qubit layer needs three;
+    13-qubit chunks with 3 low positions cover 33 in two."""
+    circ, _ = workloads.grover(33, 8, 33)
+    ops, mats = lower(circ)
+    s12, _ = capi.plan_sweeps_host(33, ops, mats, 4, None, False, True)
+    s13, _ = capi.plan_sweeps_host(33, ops, mats, 3, None, False, True, chunk_bits=13)
+    assert s12["passes"] == 52 and s13["passes"] == 35
+    small, _ = lower(workloads.grover(15, 2, 3)[0]), None
+    init = random_state(15, np.random.default_rng(1))
+    st, out = capi.plan_sweeps_host(15, *small, 3, init, False, True, chunk_bits=13)
+    assert np.max(np.abs(out - oracle.run_ops(15, *small, init))) <= 1e-12
+    r = capi.jit_compile_host(15, *small, 3, False, 1, str(tmp_path), chunk_bits=13)
+    assert r["kernels"] == st["passes"]
+    src = "".join(p.read_text() for p in tmp_path.glob("sc_passes_*.hip"))
+    assert "__launch_bounds__(512)" in src and "lds[8192]" in src
+
+
+def test_thirteen_qubit_chunks_need_runtime_compiled_passes():
+    ops, mats = random_circuit(14, 10, np.random.default_rng(2))
+    with pytest.raises(capi.IllegalArgumentException):
+        capi.plan_sweeps_host(14, ops, mats, 4, None, False, False, chunk_bits=13)
