@@ -645,6 +645,27 @@ int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const doub
     Context *ctx = st->ctx;
     if ((rc = set_device(ctx))) return rc;
     const uint64_t count = 1ULL << m;
+    bool identity = (m == st->n);
+    for (int k = 0; identity && k < m; k++) identity = measured[k] == k;
+    if (identity && st->n >= kChunkBits) {
+        // every qubit in index order: sampled straight from the state, without a 2^n probability array
+        DevBuf io;
+        if ((rc = io.alloc((size_t)shots * 24))) return rc;
+        double *u_dev = (double *)io.p;
+        int64_t *o_dev = (int64_t *)((char *)io.p + (size_t)shots * 8);
+        double *g_dev = (double *)((char *)io.p + (size_t)shots * 16);
+        SC_HIP_CHECK(hipMemcpyAsync(u_dev, uniforms, (size_t)shots * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = sample_full_state(ctx, st, elem, u_dev, shots, o_dev, boundary_gap ? g_dev : nullptr))) return rc;
+        SC_HIP_CHECK(hipMemcpyAsync(outcomes, o_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<double> gaps;
+        if (boundary_gap) {
+            gaps.resize((size_t)shots);
+            SC_HIP_CHECK(hipMemcpyAsync(gaps.data(), g_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (boundary_gap) *boundary_gap = gaps.empty() ? 0.0 : *std::min_element(gaps.begin(), gaps.end());
+        return SC_OK;
+    }
     DevBuf p, io;
     if ((rc = p.alloc(count * sizeof(double)))) return rc;
     if ((rc = io.alloc((size_t)shots * 24))) return rc;

@@ -179,6 +179,9 @@ int launch_probabilities_marginal(Context *ctx, State *st, int elem, const int32
 int device_sum(Context *ctx, const double *x, uint64_t count, double *result_host);
 int launch_scale(Context *ctx, double *x, uint64_t count, double factor);
 int launch_cumulative(Context *ctx, double *p, uint64_t count);
+// Shot sampling over all qubits in index order straight from the state (no probability array).
+int sample_full_state(Context *ctx, State *st, int elem, const double *u_dev, int64_t shots, int64_t *out_dev,
+                      double *gap_dev);
 int launch_search(Context *ctx, const double *cum, uint64_t count, const double *u_dev, int64_t shots,
                   int64_t *out_dev, double *gap_dev);
 int launch_norm_sq(Context *ctx, const double2 *amp, uint64_t count, double *result_host);

@@ -266,6 +266,109 @@ __global__ void __launch_bounds__(kBlock) k_tile_scan(double *__restrict__ x, ui
 
 __global__ void k_set_last_one(double *x, uint64_t count) { x[count - 1] = 1.0; }
 
+// ---- full-register sampling without a probability array ------------------------------------------------
+// k_tile_scan's in-tile order applied to the probabilities of tile t of the state (|a|^2, times scale):
+// the tile goes to LDS (padded as there), each thread accumulates its 16 consecutive values into v[], and
+// the thread totals are scanned in sh[].  Returns the thread's exclusive prefix within the tile; the tile
+// total is sh[kBlock - 1].
+__device__ double state_tile_scan(const double2 *__restrict__ amp, uint64_t base, uint64_t count, double scale,
+                                  double *tile, double *sh, double (&v)[kScanPer]) {
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        const int l = threadIdx.x + j * kBlock;
+        const uint64_t i = base + l;
+        double p = i < count ? prob_nofma(amp[i]) : 0.0;
+        if (scale != 1.0) p *= scale;
+        tile[l + l / kScanPer] = p;
+    }
+    __syncthreads();
+    double run = 0.0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        const int l = threadIdx.x * kScanPer + j;
+        run += tile[l + l / kScanPer];
+        v[j] = run;
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int s = 1; s < kBlock; s <<= 1) {
+        const double add = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    return threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0;
+}
+
+// totals[t]: the last in-tile cumulative value of tile t
+__global__ void __launch_bounds__(kBlock) k_state_tile_totals(const double2 *__restrict__ amp, uint64_t count,
+                                                              double scale, double *__restrict__ totals) {
+    __shared__ double sh[kBlock];
+    __shared__ double tile[kScanTile + kScanTile / kScanPer];
+    double v[kScanPer];
+    (void)state_tile_scan(amp, (uint64_t)blockIdx.x * kScanTile, count, scale, tile, sh, v);
+    if (threadIdx.x == kBlock - 1) totals[blockIdx.x] = sh[kBlock - 1];
+}
+
+// One workgroup per shot: the tile whose cumulative end (ends = inclusive scan of the tile totals, last
+// forced to 1.0) first reaches u, then inside it the smallest index whose cumulative value reaches u -- the
+// result of java.util.Arrays.binarySearch + StatevectorEngine.sampleOutcome (:325-338) on the cumulative
+// array sc_sample otherwise materialises.
+__global__ void __launch_bounds__(kBlock) k_search_state_tiles(const double2 *__restrict__ amp, uint64_t count,
+                                                               double scale, const double *__restrict__ ends,
+                                                               uint64_t ntiles, const double *__restrict__ u,
+                                                               int64_t shots, int64_t *__restrict__ out,
+                                                               double *__restrict__ gap) {
+    __shared__ double sh[kBlock];
+    __shared__ double tile[kScanTile + kScanTile / kScanPer];
+    __shared__ uint64_t best[kBlock];
+    for (int64_t s = blockIdx.x; s < shots; s += gridDim.x) {
+        const double value = u[s];
+        uint64_t t = ntiles - 1;
+        if (value < 1.0) {
+            uint64_t lo = 0, hi = ntiles - 1;  // smallest t with ends[t] >= value (ends[ntiles - 1] = 1.0)
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (ends[mid] >= value) hi = mid;
+                else lo = mid + 1;
+            }
+            t = lo;
+        }
+        const uint64_t base = t * (uint64_t)kScanTile;
+        double v[kScanPer];
+        const double prefix = state_tile_scan(amp, base, count, scale, tile, sh, v) + (t > 0 ? ends[t - 1] : 0.0);
+        uint64_t cand = ~0ULL;
+#pragma unroll
+        for (int j = 0; j < kScanPer; j++) {
+            const int l = threadIdx.x * kScanPer + j;
+            const uint64_t i = base + l;
+            const double c = i == count - 1 ? 1.0 : prefix + v[j];
+            tile[l + l / kScanPer] = c;
+            if (i < count && cand == ~0ULL && (value >= 1.0 ? i == count - 1 : c >= value)) cand = i;
+        }
+        best[threadIdx.x] = cand;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) best[threadIdx.x] = best[threadIdx.x] < best[threadIdx.x + w] ? best[threadIdx.x]
+                                                                                                  : best[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            uint64_t r = best[0];
+            if (r == ~0ULL) r = base + kScanTile < count ? base + kScanTile : count - 1;  // rounding at a tile end
+            out[s] = (int64_t)r;
+            if (gap) {
+                const uint64_t l = r - base;
+                double g = fabs(value - (r < base + kScanTile ? tile[l + l / kScanPer] : ends[t]));
+                if (r > base && r < base + kScanTile) g = fmin(g, fabs(value - tile[(l - 1) + (l - 1) / kScanPer]));
+                else if (r == base && t > 0) g = fmin(g, fabs(value - ends[t - 1]));
+                gap[s] = g;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- shot search: java.util.Arrays.binarySearch + StatevectorEngine.sampleOutcome (:325-338) -------------
 __global__ void __launch_bounds__(kBlock) k_search(const double *__restrict__ cum, int64_t len,
                                                    const double *__restrict__ u, int64_t shots,
@@ -431,6 +534,46 @@ int launch_search(Context *ctx, const double *cum, uint64_t count, const double 
     hipLaunchKernelGGL(k_search, dim3(grid_for(ctx, (uint64_t)shots)), dim3(kBlock), 0, ctx->stream, cum,
                        (int64_t)count, u_dev, shots, out_dev, gap_dev);
     SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
+}
+
+// Sampling over every qubit in index order: no 2^n probability array.  One read of the state gives the
+// tile totals (k_tile_scan's order), whose sum is the normalisation (StatevectorEngine.normalize :298-312:
+// rescaled only when off by more than 1e-12, then the totals are read again with the factor); the totals'
+// inclusive scan, last forced to 1.0, gives the tile ends, and each shot rescans one tile.
+int sample_full_state(Context *ctx, State *st, int elem, const double *u_dev, int64_t shots, int64_t *out_dev,
+                      double *gap_dev) {
+    const uint64_t count = st->dim();
+    const uint64_t ntiles = (count + kScanTile - 1) / kScanTile;
+    const double2 *amp = st->amp + ((uint64_t)elem << st->n);
+    DevBuf ends_buf;
+    int rc = ends_buf.alloc(ntiles * sizeof(double));
+    if (rc) return rc;
+    double *ends = (double *)ends_buf.p;
+    {
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)count);
+        hipLaunchKernelGGL(k_state_tile_totals, dim3((unsigned)ntiles), dim3(kBlock), 0, ctx->stream, amp, count, 1.0,
+                           ends);
+        SC_HIP_CHECK(hipGetLastError());
+    }
+    double sum = 0.0;
+    if ((rc = device_sum(ctx, ends, ntiles, &sum))) return rc;
+    if (sum == 0.0) return fail(SC_E_STATE, "State collapsed to zero probability amplitudes");
+    double scale = 1.0;
+    if (std::fabs(sum - 1.0) > 1e-12) {
+        scale = 1.0 / sum;
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)count);
+        hipLaunchKernelGGL(k_state_tile_totals, dim3((unsigned)ntiles), dim3(kBlock), 0, ctx->stream, amp, count,
+                           scale, ends);
+        SC_HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = launch_cumulative(ctx, ends, ntiles))) return rc;
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
+    const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)shots, (uint64_t)ctx->cu_count * 8);
+    hipLaunchKernelGGL(k_search_state_tiles, dim3(grid), dim3(kBlock), 0, ctx->stream, amp, count, scale,
+                       (const double *)ends, ntiles, u_dev, shots, out_dev, gap_dev);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // ends_buf is released on return
     return SC_OK;
 }
 
