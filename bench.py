@@ -375,7 +375,8 @@ def bench_c4(args):
     return _line("batched QAOA instances/s (12-22 q, p=4) + GHZ-20 depolarising trajectories/s",
                  n_inst / t_qaoa, "instances/s", args, 1e3 * (t_qaoa + t_traj),
                  {"workload": "qaoa-ring-batched+ghz20-trajectories", "instances": n_inst,
-                  "trajectories": nt},
+                  "trajectories": nt, "chunk_bits": args.chunk_bits if args.jit else 12,
+                  "low_qubits": args.low_qubits},
                  qaoa_seconds=t_qaoa, mean_cut_expectation=float(np.mean(mean_c)),
                  trajectories_per_s=nt / t_traj, trajectory_seconds=t_traj,
                  ghz_parity_mean=float(np.mean(parity)))
@@ -459,7 +460,7 @@ def main(argv=None) -> int:
                     help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")
     ap.add_argument("--chunk-bits", type=int, default=None, choices=[12, 13],
                     help="SC_OPT_CHUNK_BITS: local qubits per chunk of the runtime-compiled passes (default 12; 13 "
-                         "for the 33-qubit Grover workload, whose 33-qubit layers fit two 13-qubit passes)")
+                         "for c3, whose 33-qubit layers fit two 13-qubit passes, and for c4)")
     ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -472,10 +473,13 @@ def main(argv=None) -> int:
                     help="SC_OPT_JIT: 0 compiled sweep kernel, 1 runtime-compiled pass kernels, 2 the same with "
                          "next-chunk prefetch (kernels are compiled during warmup)")
     args = ap.parse_args(argv)
+    # per-workload sweep geometry (measured): 13-qubit chunks cut the passes of Grover's 33-qubit layers (with 3
+    # low positions) and of the batched 20-22 qubit circuits; the 30-qubit layered circuit is the same either way
+    chunk_default, low_default = {"c3": (13, 3), "c4": (13, 4)}.get(args.workload, (12, 4))
     if args.chunk_bits is None:
-        args.chunk_bits = 13 if args.workload == "c3" else 12
+        args.chunk_bits = chunk_default
     if args.low_qubits is None:
-        args.low_qubits = 3 if args.chunk_bits == 13 else 4
+        args.low_qubits = low_default if args.chunk_bits == chunk_default else (3 if args.chunk_bits == 13 else 4)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",
               file=sys.stderr)
