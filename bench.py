@@ -460,7 +460,7 @@ def main(argv=None) -> int:
                     help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")
     ap.add_argument("--chunk-bits", type=int, default=None, choices=[12, 13],
                     help="SC_OPT_CHUNK_BITS: local qubits per chunk of the runtime-compiled passes (default 12; 13 "
-                         "for c3, whose 33-qubit layers fit two 13-qubit passes, and for c4)")
+                         "for c3, whose 33-qubit layers fit two 13-qubit passes, and for c4 and c5)")
     ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -474,8 +474,9 @@ def main(argv=None) -> int:
                          "next-chunk prefetch (kernels are compiled during warmup)")
     args = ap.parse_args(argv)
     # per-workload sweep geometry (measured): 13-qubit chunks cut the passes of Grover's 33-qubit layers (with 3
-    # low positions) and of the batched 20-22 qubit circuits; the 30-qubit layered circuit is the same either way
-    chunk_default, low_default = {"c3": (13, 3), "c4": (13, 4)}.get(args.workload, (12, 4))
+    # low positions), of the batched 20-22 qubit circuits and of the 24-qubit forward runs; the 30-qubit layered
+    # circuit takes the same time either way (adjoint passes always use 12)
+    chunk_default, low_default = {"c3": (13, 3), "c4": (13, 4), "c5": (13, 4)}.get(args.workload, (12, 4))
     if args.chunk_bits is None:
         args.chunk_bits = chunk_default
     if args.low_qubits is None:
