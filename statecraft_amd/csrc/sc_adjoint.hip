@@ -820,8 +820,17 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         }
         SC_HIP_CHECK(hipMemcpyAsync((void *)dgoff, goff.data(), goff.size() * 4, hipMemcpyHostToDevice, ctx->stream));
         ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, (16.0 * (double)gx.size() + 16.0) * (double)total);
-        hipLaunchKernelGGL(k_pauli_sum, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, lam, n,
-                           total, dgx, dgoff, (int)gx.size(), dtz, dtc);
+        // a few elements per launch: the partner amplitudes psi_{i^x} of every X mask then come from a working
+        // set the last-level cache can hold, instead of the whole batch's
+        const uint64_t per = std::max<uint64_t>(1, (256ULL << 20) / (16ULL << n));
+        for (uint64_t e0 = 0; e0 < B; e0 += per) {
+            const uint64_t cnt = std::min<uint64_t>(per, B - e0);
+            const uint64_t sub_total = cnt << n;
+            hipLaunchKernelGGL(k_pauli_sum, dim3(grid_for(ctx, sub_total)), dim3(kBlock), 0, ctx->stream,
+                               st->amp + (e0 << n), lam + (e0 << n), n, sub_total, dgx, dgoff, (int)gx.size(), dtz,
+                               dtc);
+            SC_HIP_CHECK(hipGetLastError());
+        }
         SC_HIP_CHECK(hipGetLastError());
         SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // dterms is released at the end of this block
     }
