@@ -767,17 +767,21 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     uint64_t n_ckpt = (C > 0 && n_ops > 0) ? (n_ops - 1) / C : 0;
     if (n_ckpt > 0 && (rc = dckpt.alloc(state_bytes * n_ckpt))) return rc;
     {
+        // segment k ends in checkpoint k, segment k + 1 starts from it: with runtime-compiled passes the
+        // segment's last pass writes the checkpoint and the next segment's first pass reads it, no copies
         uint64_t begin = 0;
+        const double2 *cur = st->amp;
         for (uint64_t k = 0; k <= n_ckpt; k++) {
-            uint64_t end = (k < n_ckpt) ? (k + 1) * C : n_ops;
+            const uint64_t end = (k < n_ckpt) ? (k + 1) * C : n_ops;
+            double2 *dst = k < n_ckpt ? (double2 *)((char *)dckpt.p + state_bytes * k) : st->amp;
             if (end > begin) {
-                rc = apply_ops_impl(st, ops + begin, end - begin, mats, n_mat_complex,
-                                    mats_per_element ? n_mat_complex : 0, mats_per_element != 0);
+                rc = apply_ops_between(st, ops + begin, end - begin, mats, n_mat_complex,
+                                       mats_per_element ? n_mat_complex : 0, mats_per_element != 0, cur, dst);
                 if (rc) return rc;
+            } else if (cur != dst) {
+                SC_HIP_CHECK(hipMemcpyAsync(dst, cur, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
             }
-            if (k < n_ckpt)
-                SC_HIP_CHECK(hipMemcpyAsync((char *)dckpt.p + state_bytes * k, st->amp, state_bytes,
-                                            hipMemcpyDeviceToDevice, ctx->stream));
+            cur = dst;
             begin = end;
         }
     }

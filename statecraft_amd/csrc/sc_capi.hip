@@ -190,7 +190,10 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 }
 
 // mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
-int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems) {
+// src / dst (runtime-compiled passes only): the first pass reads the states from src and the last one
+// writes them to dst; passes in between work in st->amp.  nullptr means st->amp.
+int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems,
+               const double2 *src = nullptr, double2 *dst = nullptr) {
     // ---- the plan (and its runtime-compiled kernels) of this circuit structure, cached per context
     const int max_fuse = std::min(ctx->max_fused, kRegBits);
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
@@ -211,7 +214,16 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     ent->used = ++ctx->plan_cache->tick;
     const SweepPlan &plan = ent->plan;
     const std::vector<hipFunction_t> &fns = ent->fns;
-    if (plan.passes.empty()) return SC_OK;
+    if (!src) src = st->amp;
+    if (!dst) dst = st->amp;
+    if (plan.passes.empty()) {
+        if (src != dst)
+            SC_HIP_CHECK(hipMemcpyAsync(dst, src, ((size_t)st->batch << st->n) * sizeof(double2),
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        return SC_OK;
+    }
+    if (fns.empty() && (src != st->amp || dst != st->amp))
+        return fail(SC_E_STATE, "internal error: out-of-place sweeps need runtime-compiled passes");
     // ---- upload: pass and op records (compiled sweep kernel only) and every element's matrix pool
     const bool records = fns.empty();
     const size_t pass_bytes = records ? plan.passes.size() * sizeof(SweepPass) : 0;
@@ -269,8 +281,10 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     for (size_t p = 0; p < plan.passes.size(); p++) {
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             int cnt = std::min(per_launch, st->batch - e0);
+            const double2 *in = p == 0 ? src : st->amp;
+            double2 *out = p + 1 == plan.passes.size() ? dst : st->amp;
             rc = records ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
-                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride, plan.chunk_bits);
+                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride, plan.chunk_bits, in, out);
             if (rc) return rc;
             ctx->last_launches++;
         }
@@ -310,6 +324,37 @@ int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vec
         rc = exec_sweep(ctx, st, plan_ops, elems);
     }
     if (rc) return rc;
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int apply_ops_between(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                      uint64_t mat_stride, bool per_elem, const double2 *src, double2 *dst) {
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const size_t bytes = ((size_t)st->batch << st->n) * sizeof(double2);
+    if (!(ctx->jit && ctx->exec_mode == SC_EXEC_SWEEP && st->n >= kChunkBits)) {
+        // in place, with copies at the ends
+        if (src != st->amp) SC_HIP_CHECK(hipMemcpyAsync(st->amp, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        if ((rc = apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, mat_stride, per_elem))) return rc;
+        if (dst != st->amp) SC_HIP_CHECK(hipMemcpyAsync(dst, st->amp, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return SC_OK;
+    }
+    std::vector<const double *> elems;
+    if (per_elem) {
+        for (int b = 0; b < st->batch; b++) elems.push_back(mats + 2 * mat_stride * (uint64_t)b);
+    } else {
+        elems.push_back(mats);
+    }
+    std::vector<PlanOp> plan_ops;
+    if ((rc = analyze_ops(st->n, ops, n_ops, elems, per_elem ? mat_stride : n_mat_complex, plan_ops))) return rc;
+    ctx->last_sweeps = ctx->last_launches = ctx->last_blocks = 0;
+    if (plan_ops.empty()) {
+        if (src != dst) SC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return SC_OK;
+    }
+    if ((rc = exec_sweep(ctx, st, plan_ops, elems, src, dst))) return rc;
     SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SC_OK;
 }

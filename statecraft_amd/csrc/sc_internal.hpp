@@ -195,6 +195,11 @@ int launch_scale_state(Context *ctx, double2 *amp, uint64_t count, double factor
 // op application shared by the C-ABI entry points (sc_capi.hip)
 int apply_ops_impl(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                    uint64_t mat_stride, bool per_elem);
+// The same with the states read from src and left in dst (either may be st->amp); st->amp is the work
+// buffer in between.  Runtime-compiled sweep passes read and write the two buffers directly, other
+// execution modes copy.
+int apply_ops_between(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                      uint64_t mat_stride, bool per_elem, const double2 *src, double2 *dst);
 
 // A device buffer owned for the duration of one API call.
 struct DevBuf {

@@ -134,7 +134,9 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     const int TB = plan.chunk_bits - kRegBits;  // thread-index bits: 8 (256 threads) or 9 (512)
     s << "extern \"C\" __global__ void __launch_bounds__(" << (1 << TB)
       << ") __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
-         "@K@(sc::v2d *__restrict__ amp, int n, int log_chunks, uint64_t total_chunks,\n"
+         // amp and out may be the same buffer: every chunk is read once and written once, by one workgroup,
+         // so no access of one pointer can observe an access of the other
+         "@K@(sc::v2d *__restrict__ amp, sc::v2d *__restrict__ out, int n, int log_chunks, uint64_t total_chunks,\n"
          "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin) {\n"
          "  using namespace sc;\n"
          "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n"
@@ -217,7 +219,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     s << "    }\n";
     if (!prefetch) next_loads("p");
     s << "    {\n"
-         "    v2d *dst = amp + (elem << n) + cb + opaque(gthread);\n";
+         "    v2d *dst = out + (elem << n) + cb + opaque(gthread);\n";
     regs_store("dst", "a");
     s << "    }\n"
          "    if (next >= c_end) break;\n"
@@ -558,7 +560,7 @@ int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *se
 }
 
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
-                     int elem_count, size_t mat_stride, int chunk_bits) {
+                     int elem_count, size_t mat_stride, int chunk_bits, const double2 *src, double2 *dst) {
     int log_chunks = st->n - chunk_bits;
     uint64_t total = (uint64_t)elem_count << log_chunks;
     // 12-qubit chunks: as many 256-thread workgroups per CU as k_sweep's occupancy allows (two 64 KiB LDS
@@ -566,11 +568,13 @@ int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *ma
     const int threads = 1 << (chunk_bits - kRegBits);
     const uint64_t resident = chunk_bits == kChunkBits ? (uint64_t)sweep_grid_size(ctx) : (uint64_t)ctx->cu_count;
     const uint64_t grid = std::min<uint64_t>(total, resident);
-    v2d *amp = reinterpret_cast<v2d *>(st->amp);
+    // a pass reads every chunk from src and writes it to dst (the same addresses; in place when equal)
+    v2d *amp = reinterpret_cast<v2d *>(const_cast<double2 *>(src ? src : st->amp));
+    v2d *out = reinterpret_cast<v2d *>(dst ? dst : st->amp);
     const v2d *mats = reinterpret_cast<const v2d *>(mats_dev);
     int n = st->n;
     uint64_t stride = (uint64_t)mat_stride;
-    void *args[] = {&amp, &n, &log_chunks, &total, &mats, &stride, &elem_begin};
+    void *args[] = {&amp, &out, &n, &log_chunks, &total, &mats, &stride, &elem_begin};
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
     SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, threads, 1, 1, 0, ctx->stream, args, nullptr));
     return SC_OK;
