@@ -224,7 +224,8 @@ int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_m
 /* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks);
 /* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
- * (n >= 12) as a context with the given low-qubit, remap, jit and pass-cap options would, and reports its passes,
+ * (n >= 12) as a context with the given low-qubit, remap, jit, pass-cap and chunk-bit (12, or 13 with jit)
+ * options would, and reports its passes,
  * stages, fused 16x16 blocks and FP64 operations per amplitude.  When state_inout is non-NULL (2^n
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */
@@ -239,8 +240,8 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
 int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                          int n_params, int low_qubits, double *psi_inout, double *lam_inout, double *grads);
 /* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
- * SC_OPT_JIT path; jit = 3: the adjoint passes of the circuit's reverse sweep) and compiles them with hipRTC
- * on `threads` threads.  Reports the kernel count, the
+ * SC_OPT_JIT path, with chunk_bits local qubits per chunk; jit = 3: the adjoint passes of the circuit's
+ * reverse sweep, always 12) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the
  * compile wall time and the code object size; when dump_dir is non-NULL each module's source (.hip) and
  * code object (.co) are written there. */
 int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
