@@ -2,8 +2,8 @@
 //
 // Host orchestration only: validation, planning (sc_plan.cpp), HBM ownership,
 // uploads, launches and the reference's host-side rules (normalise threshold,
-// zero-probability and invalid-norm errors).  Kernels live in sc_gates.hip,
-// sc_sweep.hip, sc_measure.hip, sc_noise.hip and sc_adjoint.hip.
+// zero-probability and invalid-norm errors).  Kernels live in sc_gates.hip, sc_sweep.hip, sc_measure.hip
+// and sc_adjoint.hip (observables, adjoint gradients and batched trajectories).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
