@@ -669,6 +669,7 @@ int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zm
     int rc = check_terms(st, xmask, zmask, coeff, n_terms);
     if (rc) return rc;
     if ((rc = set_device(st->ctx))) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     return expectation(st->ctx, st, st->amp, xmask, zmask, coeff, n_terms, out);
 }
 
@@ -679,6 +680,7 @@ int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64
     int rc = check_terms(st, xmask, zmask, nullptr, n_terms, false);
     if (rc) return rc;
     if ((rc = set_device(st->ctx))) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     return pauli_terms(st->ctx, st, st->amp, xmask, zmask, n_terms, out);
 }
 
@@ -725,6 +727,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     int rc = check_terms(st, xmask, zmask, coeff, n_terms);
     if (rc) return rc;
     if ((rc = set_device(ctx))) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     const int n = st->n;
     const uint64_t B = (uint64_t)st->batch;
     std::vector<const double *> elems;
@@ -965,6 +968,7 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
     Context *ctx = st->ctx;
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     {
         std::vector<const double *> elems{mats};
         std::vector<PlanOp> pops;

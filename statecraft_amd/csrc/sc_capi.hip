@@ -216,6 +216,11 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     const std::vector<hipFunction_t> &fns = ent->fns;
     if (!src) src = st->amp;
     if (!dst) dst = st->amp;
+    if (plan.passes.empty() || fns.empty()) {
+        int rc0 = materialize_basis(st);
+        if (rc0) return rc0;
+    }
+    const long long basis = st->pending_basis;  // generated by the first pass (its src is not read)
     if (plan.passes.empty()) {
         if (src != dst)
             SC_HIP_CHECK(hipMemcpyAsync(dst, src, ((size_t)st->batch << st->n) * sizeof(double2),
@@ -284,11 +289,13 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
             const double2 *in = p == 0 ? src : st->amp;
             double2 *out = p + 1 == plan.passes.size() ? dst : st->amp;
             rc = records ? launch_sweep(ctx, st, dpass + p, dops, dpool, e0, cnt, stride)
-                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride, plan.chunk_bits, in, out);
+                         : launch_sweep_jit(ctx, st, fns[p], dpool, e0, cnt, stride, plan.chunk_bits, in, out,
+                                            p == 0 ? basis : -1);
             if (rc) return rc;
             ctx->last_launches++;
         }
         ctx->last_sweeps++;
+        st->pending_basis = -1;
     }
     ctx->last_blocks += plan.fused_blocks;
     // the pinned staging buffer and plan_dev are reused by the next call
@@ -298,6 +305,20 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
 
 }  // namespace
 
+int materialize_basis(State *st) {
+    if (st->pending_basis < 0) return SC_OK;
+    Context *ctx = st->ctx;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemsetAsync(st->amp, 0, ((size_t)st->batch << st->n) * sizeof(double2), ctx->stream));
+    hipLaunchKernelGGL(k_set_basis, dim3((st->batch + 255) / 256), dim3(256), 0, ctx->stream, st->amp, st->n, st->batch,
+                       (uint64_t)st->pending_basis);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    st->pending_basis = -1;
+    return SC_OK;
+}
+
 int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
                   bool per_elem) {
     Context *ctx = st->ctx;
@@ -306,6 +327,8 @@ int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vec
     int mode = ctx->exec_mode;
     if (mode == SC_EXEC_SWEEP && st->n < kChunkBits) mode = SC_EXEC_FUSED;
     int rc = SC_OK;
+    // only runtime-compiled sweeps generate a pending basis state themselves
+    if (!(mode == SC_EXEC_SWEEP && ctx->jit) && (rc = materialize_basis(st))) return rc;
     if (per_elem && mode != SC_EXEC_SWEEP) {
         // element-by-element with a one-element view of the state
         for (int b = 0; b < st->batch; b++) {
@@ -565,14 +588,7 @@ int sc_state_qubits(const sc_state *st, int *num_qubits, int *batch) {
 int sc_state_set_basis(sc_state *st, uint64_t index) {
     SC_REQUIRE(st, "state");
     if (st->n < 64 && (index >> st->n) != 0) return fail(SC_E_ARG, "basis index out of range");
-    Context *ctx = st->ctx;
-    int rc = set_device(ctx);
-    if (rc) return rc;
-    SC_HIP_CHECK(hipMemsetAsync(st->amp, 0, ((size_t)st->batch << st->n) * sizeof(double2), ctx->stream));
-    hipLaunchKernelGGL(k_set_basis, dim3((st->batch + 255) / 256), dim3(256), 0, ctx->stream, st->amp, st->n, st->batch,
-                       index);
-    SC_HIP_CHECK(hipGetLastError());
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    st->pending_basis = (int64_t)index;  // written by the next operation that needs it
     return SC_OK;
 }
 
@@ -585,6 +601,7 @@ int sc_state_upload(sc_state *st, const double *interleaved, uint64_t n_complex)
     int rc = set_device(st->ctx);
     if (rc) return rc;
     SC_HIP_CHECK(hipMemcpy(st->amp, interleaved, n_complex * sizeof(double2), hipMemcpyHostToDevice));
+    st->pending_basis = -1;  // every amplitude was written
     return SC_OK;
 }
 
@@ -596,6 +613,7 @@ int sc_state_download(sc_state *st, double *interleaved, uint64_t n_complex) {
                                   ", got " + std::to_string(n_complex));
     int rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     SC_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
     SC_HIP_CHECK(hipMemcpy(interleaved, st->amp, n_complex * sizeof(double2), hipMemcpyDeviceToHost));
     return SC_OK;
@@ -607,7 +625,9 @@ int sc_state_copy(sc_state *dst, const sc_state *src) {
     if (dst->n != src->n || dst->batch != src->batch) return fail(SC_E_ARG, "state shapes differ");
     int rc = set_device(dst->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(const_cast<sc_state *>(src)))) return rc;
     if ((rc = launch_copy(dst->ctx, src->amp, dst->amp, (uint64_t)src->batch << src->n))) return rc;
+    dst->pending_basis = -1;
     SC_HIP_CHECK(hipStreamSynchronize(dst->ctx->stream));
     return SC_OK;
 }
@@ -619,6 +639,7 @@ int sc_state_read_range(sc_state *st, int elem, uint64_t offset, uint64_t count,
     if (offset > st->dim() || count > st->dim() - offset) return fail(SC_E_ARG, "amplitude range out of bounds");
     int rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     SC_HIP_CHECK(hipStreamSynchronize(st->ctx->stream));
     if (count > 0)
         SC_HIP_CHECK(hipMemcpy(interleaved, st->amp + ((uint64_t)elem << st->n) + offset, count * sizeof(double2),
@@ -669,6 +690,7 @@ int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, dou
     if (rc) return rc;
     rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     DevBuf p;
     const uint64_t count = 1ULL << m;
     if ((rc = p.alloc(count * sizeof(double)))) return rc;
@@ -689,6 +711,7 @@ int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const doub
     if (rc) return rc;
     Context *ctx = st->ctx;
     if ((rc = set_device(ctx))) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     const uint64_t count = 1ULL << m;
     bool identity = (m == st->n);
     for (int k = 0; identity && k < m; k++) identity = measured[k] == k;
@@ -748,6 +771,7 @@ int sc_norm_sq(sc_state *st, int elem, double *out) {
     if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
     int rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     return launch_norm_sq(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), out);
 }
 
@@ -757,6 +781,7 @@ int sc_probability_checksum(sc_state *st, int elem, double out[2]) {
     if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
     int rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     return launch_checksum(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), out);
 }
 
@@ -769,6 +794,7 @@ int sc_kraus_weights(sc_state *st, int elem, int target, const double *kraus, in
     if (target < 0 || target >= st->n) return fail(SC_E_ARG, "noise channel qubit out of range: " + std::to_string(target));
     int rc = set_device(st->ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     return launch_kraus_weights(st->ctx, st->amp + ((uint64_t)elem << st->n), st->n, target, kraus, k, weights);
 }
 
@@ -780,6 +806,7 @@ int sc_apply_kraus(sc_state *st, int elem, int target, const double *kraus_op, d
     Context *ctx = st->ctx;
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
     State view = *st;
     view.amp = st->amp + ((uint64_t)elem << st->n);
     view.batch = 1;

@@ -86,8 +86,14 @@ struct State {
     int n = 0;
     int batch = 1;
     double2 *amp = nullptr;  // batch << n amplitudes
+    // >= 0: every element is the basis state |pending_basis>, not yet written to amp.  sc_state_set_basis only
+    // records it; the next runtime-compiled sweep generates it in its first pass, anything else writes it
+    // first (materialize_basis).
+    int64_t pending_basis = -1;
     uint64_t dim() const { return 1ULL << n; }
 };
+// Writes a pending basis state to amp (no-op otherwise).
+int materialize_basis(State *st);
 
 int ensure_scratch(Context *ctx, size_t bytes);
 int ensure_host_scratch(Context *ctx, size_t bytes);

@@ -137,7 +137,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          // amp and out may be the same buffer: every chunk is read once and written once, by one workgroup,
          // so no access of one pointer can observe an access of the other
          "@K@(sc::v2d *__restrict__ amp, sc::v2d *__restrict__ out, int n, int log_chunks, uint64_t total_chunks,\n"
-         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin) {\n"
+         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin, long long basis) {\n"
          "  using namespace sc;\n"
          "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n"
          "  CV2 *mats = (CV2 *)mats_arg;\n"
@@ -161,6 +161,13 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     auto regs_load = [&](const char *dst, const char *srcp) {
         for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << ps.glob_reg[r] << "];\n";
     };
+    // basis >= 0: the input of every element is the basis state |basis>, generated instead of loaded
+    auto regs_basis = [&](const char *dst, const char *chunk_base) {
+        s << "    const uint64_t i0 = " << chunk_base << " + gthread;\n";
+        for (int r = 0; r < kRegs; r++)
+            s << "    " << dst << "[" << r << "] = (i0 + " << ps.glob_reg[r]
+              << "ULL == (uint64_t)basis) ? make_v2d(1.0, 0.0) : make_v2d(0.0, 0.0);\n";
+    };
     auto regs_store = [&](const char *dstp, const char *src) {
         for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << ps.glob_reg[r] << "] = " << src << "[" << r << "];\n";
     };
@@ -178,9 +185,13 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     const bool into_first = S > 0 ? ps.stages[0].wave_local != 0 : (ps.wave_local & 1) != 0;
     s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
     s << "  uint64_t cb = cbase(c & chunk_mask);\n"
-         "  {\n"
+         "  if (basis >= 0) {\n";
+    regs_basis("p", "cb");
+    s << "  } else {\n"
          "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gthread);\n";
     regs_load("p", "src");
+    s << "  }\n";
+    s << "  {\n";
     lds_write("sthread", ps.io_reg_slot, "p");
     s << "  }\n";
     sync(into_first);
@@ -190,7 +201,9 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "    const uint64_t next = blocked ? c + 1 : c + gridDim.x;\n"
          "    const uint64_t ncb = next < c_end ? cbase(next & chunk_mask) : 0ULL;\n";
     auto next_loads = [&](const char *dst) {
-        s << "    if (next < c_end) {\n"
+        s << "    if (next < c_end && basis >= 0) {\n";
+        regs_basis(dst, "ncb");
+        s << "    } else if (next < c_end) {\n"
              "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gthread);\n";
         regs_load(dst, "src");
         s << "    }\n";
@@ -560,7 +573,8 @@ int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *se
 }
 
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
-                     int elem_count, size_t mat_stride, int chunk_bits, const double2 *src, double2 *dst) {
+                     int elem_count, size_t mat_stride, int chunk_bits, const double2 *src, double2 *dst,
+                     long long basis) {
     int log_chunks = st->n - chunk_bits;
     uint64_t total = (uint64_t)elem_count << log_chunks;
     // 12-qubit chunks: as many 256-thread workgroups per CU as k_sweep's occupancy allows (two 64 KiB LDS
@@ -574,7 +588,7 @@ int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *ma
     const v2d *mats = reinterpret_cast<const v2d *>(mats_dev);
     int n = st->n;
     uint64_t stride = (uint64_t)mat_stride;
-    void *args[] = {&amp, &out, &n, &log_chunks, &total, &mats, &stride, &elem_begin};
+    void *args[] = {&amp, &out, &n, &log_chunks, &total, &mats, &stride, &elem_begin, &basis};
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
     SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, threads, 1, 1, 0, ctx->stream, args, nullptr));
     return SC_OK;

@@ -33,9 +33,10 @@ int adjoint_plan_host(int n, const sc_op *ops, uint64_t n_ops, const double *mat
 int jit_adjoint_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out);
 std::vector<uint64_t> jit_pass_probes(const SweepPlan &plan, size_t pass);
 // src / dst: where the pass reads and writes the states (nullptr: st->amp; equal pointers: in place).
+// basis >= 0: the pass's input is |basis> in every element (generated, src unread).
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
                      int elem_count, size_t mat_stride, int chunk_bits, const double2 *src = nullptr,
-                     double2 *dst = nullptr);
+                     double2 *dst = nullptr, long long basis = -1);
 // Workgroups of a sweep launch that are resident at once (occupancy x CUs, capped by the context option).
 int sweep_grid_size(Context *ctx);
 

@@ -250,3 +250,41 @@ def test_plan_cache_reuse_and_classification(hip_ctx):
             m[o:o + 4] = [np.array([1, 0, 0, 1j]), np.array([0, 1, 1, 0]), np.array([1, 1, 1, -1]) / np.sqrt(2)][variant]
         got = run_gpu(hip_ctx, n, ops, m, init, capi.EXEC_SWEEP)
         assert np.max(np.abs(got - oracle.run_ops(n, ops, m, init))) <= EPS, variant
+
+
+def test_basis_state_is_written_when_first_needed(hip_ctx):
+    """sc_state_set_basis only records the basis state: a runtime-compiled sweep generates it in its first
+    pass, every other operation writes it first.  Each reader sees |k> in every batch element."""
+    n, B, k = 13, 2, 4097
+    want = np.zeros(1 << n, complex)
+    want[k] = 1.0
+    st = capi.State(hip_ctx, n, B)
+    st.set_basis(k)
+    assert np.array_equal(st.download().reshape(B, -1), np.stack([want] * B))
+    st.set_basis(k)
+    assert st.read(k, 1, elem=1)[0] == 1.0 and st.norm_sq(1) == 1.0
+    st.set_basis(k)
+    assert st.probabilities()[k] == 1.0
+    st.set_basis(k)
+    assert np.all(st.expectation([0], [1], [1.0]) == -1.0)  # <Z_0> on |4097>: bit 0 is set
+    st.close()
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+def test_circuit_from_a_basis_state(hip_ctx, jit):
+    """A circuit applied right after set_basis: generated in the first pass (runtime-compiled sweeps) or
+    written before the first op (other paths); both equal the oracle from the basis state."""
+    rng = np.random.default_rng(31 + jit)
+    n, k = 15, 12345
+    ops, mats = random_circuit(n, 120, rng)
+    hip_ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_SWEEP)
+    hip_ctx.set_option(capi.OPT_JIT, jit)
+    init = np.zeros(1 << n, complex)
+    init[k] = 1.0
+    st = capi.State(hip_ctx, n)
+    st.set_basis(k)
+    st.apply(ops, mats)
+    got = st.download()
+    st.close()
+    hip_ctx.set_option(capi.OPT_JIT, 0)
+    assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
