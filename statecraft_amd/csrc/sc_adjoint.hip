@@ -432,6 +432,9 @@ int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm,
                 else if (g.nt <= 8)
                     hipLaunchKernelGGL(k_pauli_terms<8>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
                                        (int)parts, part);
+                else if (g.nt <= 12)
+                    hipLaunchKernelGGL(k_pauli_terms<12>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
+                                       (int)rows, (int)parts, part);
                 else
                     hipLaunchKernelGGL(k_pauli_terms<16>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
                                        (int)rows, (int)parts, part);
