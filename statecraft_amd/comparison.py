@@ -1,7 +1,7 @@
 """Comparison-runner front end: the command line and JSON output of the reference's
 StatecraftComparisonRunner (engines/src/test/java/com/omaarr90/statecraft/engines/comparison/
 StatecraftComparisonRunner.java), served by the HIP engine, so the external benchmark harness
-(scripts/benchmark/comparison/run_comparison.py) can run this build as one more runner.
+(scripts/benchmark/comparison/run_comparison.py) can run this engine as one more runner.
 
     python -m statecraft_amd.comparison --circuit-json circuit.json [--fixture-id ID] [--category C]
         [--warmup-runs 1] [--timed-runs 5] [--parallelism 1] [--noise-type T --noise-qubits 0,1
