@@ -971,7 +971,6 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
     Context *ctx = st->ctx;
     int rc = set_device(ctx);
     if (rc) return rc;
-    if ((rc = materialize_basis(st))) return rc;
     {
         std::vector<const double *> elems{mats};
         std::vector<PlanOp> pops;
@@ -1006,7 +1005,9 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
     std::vector<double> nrm(B);
     {
         const uint64_t zero = 0;
-        if ((rc = pauli_terms(ctx, st, st->amp, &zero, &zero, 1, nrm.data()))) return rc;
+        // a pending basis state has norm 1 exactly; the trajectory circuit's first pass then generates it
+        if (st->pending_basis >= 0) std::fill(nrm.begin(), nrm.end(), 1.0);
+        else if ((rc = pauli_terms(ctx, st, st->amp, &zero, &zero, 1, nrm.data()))) return rc;
     }
     const uint64_t stride = n_mat_complex + 4 * n_draws;  // complex entries per trajectory
     std::vector<double> emats(2 * stride * B);
