@@ -160,13 +160,25 @@ int sc_apply_ops_batched(sc_state *st, const sc_op *ops, uint64_t n_ops, const d
 int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, double *out);
 /* performMeasurement :229-257 = normalize :298-312 + cumulative :314-323 + sampleOutcome :325-338 for each
  * uniform u (SplittableRandom.nextDouble() values drawn by the host in shot order).  outcomes[s] in [0, 2^m).
- * boundary_gap (may be NULL) receives min over shots of |u - cum[outcome]|, |u - cum[outcome-1]|. */
+ * The marginal probabilities, the normalising sum and the cumulative array are the reference's sequential
+ * sums bit for bit (computed segment-parallel, exactly), so the outcomes equal the reference's for the same
+ * state and uniforms.  boundary_gap (may be NULL) receives min over shots of |u - cum[outcome]|,
+ * |u - cum[outcome-1]|.  A state holding NaN or infinite amplitudes fails with SC_E_STATE. */
 int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const double *uniforms, int64_t shots,
               int64_t *outcomes, double *boundary_gap);
+/* Statistics of the last sc_sample on this context: out[0..2] segments of 4096 cumulative values summed as
+ * exact integers / serially / all zero, out[3] gather round trips, out[4] 1 when normalize rescaled. */
+int sc_last_sample_stats(sc_ctx *ctx, uint64_t out[5]);
 /* Sum of |a_i|^2 (ComplexArrays.norm2Sq, ComplexArrays.java:74-143) for batch element `elem`. */
 int sc_norm_sq(sc_state *st, int elem, double *out);
 /* Probability checksum: out[0] = sum_i p_i, out[1] = sum_i p_i * w(i) with w(i) = splitmix64(i)*2^-64. */
 int sc_probability_checksum(sc_state *st, int elem, double out[2]);
+/* Largest |psi_i - r_i| over the 2^n amplitudes of batch element `elem`, where r is a two-level reference state:
+ * r_index = a and r_i = b elsewhere (complex, interleaved).  A whole-state comparison without a host copy, e.g.
+ * against |0...0> after a circuit and its inverse, or against a closed form such as Grover's
+ * a|m> + b sum_{x != m} |x>.  No reference counterpart (StateVector comparisons copy to the host). */
+int sc_state_max_deviation(sc_state *st, int elem, uint64_t index, const double a[2], const double b[2],
+                           double *out);
 
 /* ---- noise (Monte-Carlo Kraus unravelling) -------------------------------- */
 /* computeBranchWeight StatevectorEngine.java:511-530 for k single-qubit Kraus operators (4 complex each) on
@@ -239,6 +251,15 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
  * the states uncomputed to the circuit's input and grads[n_params] the gradient. */
 int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                          int n_params, int low_qubits, double *psi_inout, double *lam_inout, double *grads);
+/* Host-only (no device needed): the exact segment-parallel running sums sc_sample uses, on the CPU: cum[i]
+ * = ((values[0] + values[1]) + ...) + values[i] summed left to right, as `cumulative` (StatevectorEngine.java
+ * :314-323) does (without forcing the last entry).  stats (may be NULL) receives the sc_last_sample_stats
+ * counts out[0..3]. */
+int sc_cumulative_host(const double *values, uint64_t count, double *cum, uint64_t *stats);
+/* Host-only (no device needed): sc_sample's exact flow on a host state of num_qubits <= 30 qubits (2^n
+ * interleaved complex values); stats (may be NULL) receives the 5 sc_last_sample_stats counts. */
+int sc_sample_host(int num_qubits, const double *state, const int32_t *measured, int m, const double *uniforms,
+                   int64_t shots, int64_t *outcomes, double *boundary_gap, uint64_t *stats);
 /* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
  * SC_OPT_JIT path, with chunk_bits local qubits per chunk; jit = 3: the adjoint passes of the circuit's
  * reverse sweep, always 12) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the

@@ -219,6 +219,16 @@ void orc_probabilities(const double *state, int n, const int32_t *measured, int 
     const uint64_t outcomes = 1ULL << m;
     memset(out, 0, outcomes * sizeof(double));
     const uint64_t dimension = 1ULL << n;
+    int identity = (m == n);
+    for (int bit = 0; identity && bit < m; bit++) identity = measured[bit] == bit;
+    if (identity) { /* every qubit in order: outcome == basis, probabilities[basis] = 0.0 + p */
+        for (uint64_t basis = 0; basis < dimension; basis++) {
+            double re = state[basis << 1], im = state[(basis << 1) + 1];
+            double p = (re * re) + (im * im);
+            if (p != 0.0) out[basis] += p;
+        }
+        return;
+    }
     for (uint64_t basis = 0; basis < dimension; basis++) {
         double re = state[basis << 1], im = state[(basis << 1) + 1];
         double p = (re * re) + (im * im);

@@ -43,7 +43,10 @@ def hipcc() -> str:
 
 
 # Per-source extra flags.
-EXTRA: dict = {}
+EXTRA: dict = {
+    # the exact sequential sums round every product and sum on its own
+    "sc_seqsum.hip": ["-ffp-contract=off"],
+}
 
 
 def _compile(src: Path) -> Path:

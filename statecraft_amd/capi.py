@@ -81,6 +81,7 @@ EXPORTED = [
     "sc_expectation_pauli_terms",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
+    "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
 ]
 
 _lib = None
@@ -134,6 +135,10 @@ def _sig(lib):
         "sc_jit_compile_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, U64P, DP, U64P,
                                       ctypes.c_char_p]),
         "sc_adjoint_plan_host": (i32, [i32, P, u64, DP, u64, i32, i32, DP, DP, DP]),
+        "sc_last_sample_stats": (i32, [P, U64P]),
+        "sc_state_max_deviation": (i32, [P, i32, u64, DP, DP, DP]),
+        "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
+        "sc_sample_host": (i32, [i32, DP, I32P, i32, DP, i64, I64P, DP, U64P]),
     }
     for name, (res, args) in s.items():
         f = getattr(lib, name)
@@ -218,6 +223,31 @@ def jit_compile_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = True,
     return {"kernels": k.value, "seconds": sec.value, "code_bytes": size.value}
 
 
+SAMPLE_STATS = ("fast_segments", "serial_segments", "zero_segments", "gather_rounds", "rescaled")
+
+
+def cumulative_host(values):
+    """Host-only: the exact segment-parallel running sums sc_sample uses (returns cum, stats)."""
+    v = as_f64(values)
+    cum = np.empty_like(v)
+    st = np.zeros(5, dtype=np.uint64)
+    check(lib().sc_cumulative_host(dptr(v), len(v), dptr(cum), st.ctypes.data_as(U64P)))
+    return cum, dict(zip(SAMPLE_STATS[:4], (int(x) for x in st[:4])))
+
+
+def sample_host(n: int, state, measured, uniforms):
+    """Host-only: sc_sample's exact flow on a host state (returns outcomes, boundary gap, stats)."""
+    a = complex_to_f64(state)
+    meas = np.ascontiguousarray(measured, np.int32)
+    u = as_f64(uniforms)
+    out = np.empty(len(u), dtype=np.int64)
+    gap = ctypes.c_double()
+    st = np.zeros(5, dtype=np.uint64)
+    check(lib().sc_sample_host(int(n), dptr(a), meas.ctypes.data_as(I32P), len(meas), dptr(u), len(u),
+                               out.ctypes.data_as(I64P), ctypes.byref(gap), st.ctypes.data_as(U64P)))
+    return out, gap.value, dict(zip(SAMPLE_STATS, (int(x) for x in st)))
+
+
 class Context:
     """One device + one HIP stream (StatevectorEngine(StatevectorExecutionConfig) analogue)."""
 
@@ -258,6 +288,12 @@ class Context:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         check(lib().sc_last_plan_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"sweeps": a.value, "launches": b.value, "fused_blocks": c.value}
+
+    def sample_stats(self):
+        """Exact-sum statistics of the last sample() on this context."""
+        st = np.zeros(5, dtype=np.uint64)
+        check(lib().sc_last_sample_stats(self.h, st.ctypes.data_as(U64P)))
+        return dict(zip(SAMPLE_STATS, (int(x) for x in st)))
 
     def adjoint_stats(self):
         a, b = ctypes.c_uint64(), ctypes.c_double()
@@ -353,6 +389,14 @@ class State:
         check(lib().sc_sample(self.h, elem, meas.ctypes.data_as(I32P), len(meas), dptr(u), len(u),
                               out.ctypes.data_as(I64P), ctypes.byref(gap)))
         return out, gap.value
+
+    def max_deviation(self, index: int, a: complex, b: complex = 0.0, elem: int = 0) -> float:
+        """max_i |psi_i - r_i| with r_index = a, r_i = b elsewhere, reduced on the device."""
+        av = np.array([complex(a).real, complex(a).imag])
+        bv = np.array([complex(b).real, complex(b).imag])
+        v = ctypes.c_double()
+        check(lib().sc_state_max_deviation(self.h, elem, int(index), dptr(av), dptr(bv), ctypes.byref(v)))
+        return v.value
 
     def norm_sq(self, elem: int = 0) -> float:
         v = ctypes.c_double()

@@ -15,6 +15,7 @@
 #include "sc_internal.hpp"
 #include "sc_jit.hpp"
 #include "sc_plan.hpp"
+#include "sc_seqsum.hpp"
 
 namespace sc {
 
@@ -675,11 +676,14 @@ static int check_measured(const sc_state *st, int elem, const int32_t *measured,
     return SC_OK;
 }
 
+// computeMeasurementProbabilities (:274-296), exactly: every qubit in order -> |a_i|^2; a few complement bits ->
+// one thread per outcome running the reference's loop; otherwise the segment-parallel exact sums.
 static int probabilities_dev(sc_state *st, int elem, const int32_t *measured, int m, double *out_dev) {
     bool identity = (m == st->n);
     for (int k = 0; identity && k < m; k++) identity = measured[k] == k;
     if (identity) return launch_probabilities_full(st->ctx, st, elem, out_dev);
-    return launch_probabilities_marginal(st->ctx, st, elem, measured, m, out_dev);
+    if (st->n - m <= seq::kSegBits) return launch_probabilities_marginal(st->ctx, st, elem, measured, m, out_dev);
+    return seq::exact_marginals(st->ctx, st->amp + ((uint64_t)elem << st->n), st->n, measured, m, out_dev);
 }
 
 int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, double *out) {
@@ -700,6 +704,8 @@ int sc_probabilities(sc_state *st, int elem, const int32_t *measured, int m, dou
     return SC_OK;
 }
 
+// performMeasurement (:229-257) for given uniforms: exact marginals (unless every qubit is measured in order),
+// then the exact normalize / cumulative / sampleOutcome of sc_seqsum.hip.
 int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const double *uniforms, int64_t shots,
               int64_t *outcomes, double *boundary_gap) {
     SC_REQUIRE(st, "state");
@@ -712,48 +718,28 @@ int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const doub
     Context *ctx = st->ctx;
     if ((rc = set_device(ctx))) return rc;
     if ((rc = materialize_basis(st))) return rc;
-    const uint64_t count = 1ULL << m;
     bool identity = (m == st->n);
     for (int k = 0; identity && k < m; k++) identity = measured[k] == k;
-    if (identity && st->n >= kChunkBits) {
-        // every qubit in index order: sampled straight from the state, without a 2^n probability array
-        DevBuf io;
-        if ((rc = io.alloc((size_t)shots * 24))) return rc;
-        double *u_dev = (double *)io.p;
-        int64_t *o_dev = (int64_t *)((char *)io.p + (size_t)shots * 8);
-        double *g_dev = (double *)((char *)io.p + (size_t)shots * 16);
-        SC_HIP_CHECK(hipMemcpyAsync(u_dev, uniforms, (size_t)shots * 8, hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = sample_full_state(ctx, st, elem, u_dev, shots, o_dev, boundary_gap ? g_dev : nullptr))) return rc;
-        SC_HIP_CHECK(hipMemcpyAsync(outcomes, o_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
-        std::vector<double> gaps;
-        if (boundary_gap) {
-            gaps.resize((size_t)shots);
-            SC_HIP_CHECK(hipMemcpyAsync(gaps.data(), g_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
-        }
-        SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        if (boundary_gap) *boundary_gap = gaps.empty() ? 0.0 : *std::min_element(gaps.begin(), gaps.end());
-        return SC_OK;
-    }
     DevBuf p, io;
-    if ((rc = p.alloc(count * sizeof(double)))) return rc;
+    if (!identity) {
+        if ((rc = p.alloc((1ULL << m) * sizeof(double)))) return rc;
+        if ((rc = probabilities_dev(st, elem, measured, m, (double *)p.p))) return rc;
+    }
     if ((rc = io.alloc((size_t)shots * 24))) return rc;
-    if ((rc = probabilities_dev(st, elem, measured, m, (double *)p.p))) return rc;
-    // StatevectorEngine.normalize (:298-312)
-    double sum = 0.0;
-    if ((rc = device_sum(ctx, (double *)p.p, count, &sum))) return rc;
-    if (sum == 0.0) return fail(SC_E_STATE, "State collapsed to zero probability amplitudes");
-    if (std::fabs(sum - 1.0) > 1e-12)
-        if ((rc = launch_scale(ctx, (double *)p.p, count, 1.0 / sum))) return rc;
-    // cumulative (:314-323)
-    if ((rc = launch_cumulative(ctx, (double *)p.p, count))) return rc;
     double *u_dev = (double *)io.p;
     int64_t *o_dev = (int64_t *)((char *)io.p + (size_t)shots * 8);
     double *g_dev = (double *)((char *)io.p + (size_t)shots * 16);
     SC_HIP_CHECK(hipMemcpyAsync(u_dev, uniforms, (size_t)shots * 8, hipMemcpyHostToDevice, ctx->stream));
-    {
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
-        if ((rc = launch_search(ctx, (double *)p.p, count, u_dev, shots, o_dev, g_dev))) return rc;
-    }
+    seq::SampleStats stats;
+    if ((rc = seq::exact_sample(ctx, st->amp + ((uint64_t)elem << st->n), st->n, measured, m,
+                                identity ? nullptr : (const double *)p.p, u_dev, shots, o_dev,
+                                boundary_gap ? g_dev : nullptr, &stats)))
+        return rc;
+    ctx->last_sample[0] = stats.fast;
+    ctx->last_sample[1] = stats.serial;
+    ctx->last_sample[2] = stats.zero;
+    ctx->last_sample[3] = stats.gather_rounds;
+    ctx->last_sample[4] = stats.rescaled ? 1 : 0;
     SC_HIP_CHECK(hipMemcpyAsync(outcomes, o_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> gaps;
     if (boundary_gap) {
@@ -761,7 +747,16 @@ int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const doub
         SC_HIP_CHECK(hipMemcpyAsync(gaps.data(), g_dev, (size_t)shots * 8, hipMemcpyDeviceToHost, ctx->stream));
     }
     SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < shots; s++)
+        if (outcomes[s] < 0) return fail(SC_E_STATE, "exact cumulative search found no outcome for a shot");
     if (boundary_gap) *boundary_gap = gaps.empty() ? 0.0 : *std::min_element(gaps.begin(), gaps.end());
+    return SC_OK;
+}
+
+int sc_last_sample_stats(sc_ctx *ctx, uint64_t out[5]) {
+    SC_REQUIRE(ctx, "ctx");
+    SC_REQUIRE(out, "out");
+    for (int i = 0; i < 5; i++) out[i] = ctx->last_sample[i];
     return SC_OK;
 }
 
@@ -783,6 +778,21 @@ int sc_probability_checksum(sc_state *st, int elem, double out[2]) {
     if (rc) return rc;
     if ((rc = materialize_basis(st))) return rc;
     return launch_checksum(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), out);
+}
+
+int sc_state_max_deviation(sc_state *st, int elem, uint64_t index, const double a[2], const double b[2],
+                           double *out) {
+    SC_REQUIRE(st, "state");
+    SC_REQUIRE(a, "a");
+    SC_REQUIRE(b, "b");
+    SC_REQUIRE(out, "out");
+    if (elem < 0 || elem >= st->batch) return fail(SC_E_ARG, "batch element out of range");
+    if (index >= st->dim()) return fail(SC_E_ARG, "basis index out of range");
+    int rc = set_device(st->ctx);
+    if (rc) return rc;
+    if ((rc = materialize_basis(st))) return rc;
+    return launch_max_deviation(st->ctx, st->amp + ((uint64_t)elem << st->n), st->dim(), index,
+                                make_double2(a[0], a[1]), make_double2(b[0], b[1]), out);
 }
 
 // ---- noise ------------------------------------------------------------------------

@@ -79,6 +79,8 @@ struct Context {
     PlanCache *plan_cache = nullptr;  // owned
     // plan statistics of the last sc_apply_ops
     uint64_t last_sweeps = 0, last_launches = 0, last_blocks = 0;
+    // exact-sum statistics of the last sc_sample: fast / serial / zero segments, gather rounds, rescaled
+    uint64_t last_sample[5] = {};
 };
 
 struct State {
@@ -182,16 +184,10 @@ int launch_sweep(Context *ctx, State *st, const SweepPass *pass_dev, const Sweep
 int launch_probabilities_full(Context *ctx, State *st, int elem, double *out_dev);
 int launch_probabilities_marginal(Context *ctx, State *st, int elem, const int32_t *measured, int m,
                                   double *out_dev);
-int device_sum(Context *ctx, const double *x, uint64_t count, double *result_host);
-int launch_scale(Context *ctx, double *x, uint64_t count, double factor);
-int launch_cumulative(Context *ctx, double *p, uint64_t count);
-// Shot sampling over all qubits in index order straight from the state (no probability array).
-int sample_full_state(Context *ctx, State *st, int elem, const double *u_dev, int64_t shots, int64_t *out_dev,
-                      double *gap_dev);
-int launch_search(Context *ctx, const double *cum, uint64_t count, const double *u_dev, int64_t shots,
-                  int64_t *out_dev, double *gap_dev);
 int launch_norm_sq(Context *ctx, const double2 *amp, uint64_t count, double *result_host);
 int launch_checksum(Context *ctx, const double2 *amp, uint64_t count, double *out_host);
+int launch_max_deviation(Context *ctx, const double2 *amp, uint64_t count, uint64_t index, double2 a, double2 b,
+                         double *out_host);
 int launch_kraus_weights(Context *ctx, const double2 *amp, int n, int target, const double *kraus, int k,
                          double *weights_host);
 int launch_scale_state(Context *ctx, double2 *amp, uint64_t count, double factor);

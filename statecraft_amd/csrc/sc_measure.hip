@@ -1,5 +1,5 @@
-// sc_measure.hip -- probabilities, deterministic reductions, inclusive scan,
-// inverse-CDF shot search, norms, checksums and Kraus branch weights.
+// sc_measure.hip -- probabilities, deterministic reductions, norms, checksums and Kraus branch weights.
+// (The exact cumulative sums and the shot search are in sc_seqsum.hip.)
 //
 // Every reduction is two-level with a fixed grid and a fixed LDS tree, so a
 // result is bit-identical from run to run on one device (no atomics).
@@ -52,25 +52,6 @@ __device__ __forceinline__ uint64_t pdep64(uint64_t x, uint64_t mask) {
     return r;
 }
 
-// Marginal, few outcomes: one block per (outcome, slice); lanes walk the complement index.
-// partial[slice * outcomes + o] = sum over the slice's complement indices.
-__global__ void __launch_bounds__(kBlock) k_prob_marginal_wide(const double2 *__restrict__ amp, uint64_t mmask,
-                                                               uint64_t cmask, int cbits, int log_slices, int m,
-                                                               double *__restrict__ partial) {
-    const uint64_t o = blockIdx.x;
-    const uint64_t slice = blockIdx.y;
-    const uint64_t per_slice = 1ULL << (cbits - log_slices);
-    const uint64_t obase = pdep64(o, mmask);
-    double acc[1] = {0.0};
-    for (uint64_t c = threadIdx.x; c < per_slice; c += kBlock) {
-        uint64_t cc = slice * per_slice + c;
-        acc[0] += prob_nofma(amp[obase | pdep64(cc, cmask)]);
-    }
-    double r[1];
-    block_reduce<1>(acc, r);
-    if (threadIdx.x == 0) partial[(slice << m) + o] = r[0];
-}
-
 // Marginal, many outcomes: one thread per outcome walks all complement indices in increasing order.
 __global__ void __launch_bounds__(kBlock) k_prob_marginal_narrow(const double2 *__restrict__ amp, uint64_t mmask,
                                                                  uint64_t cmask, uint64_t outcomes,
@@ -87,25 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_prob_marginal_narrow(const double2 *
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_sum_slices(const double *__restrict__ partial, int log_slices, int m,
-                                                       double *__restrict__ out) {
-    const uint64_t outcomes = 1ULL << m;
-    for (uint64_t o = blockIdx.x * (uint64_t)kBlock + threadIdx.x; o < outcomes; o += (uint64_t)gridDim.x * kBlock) {
-        double acc = 0.0;
-        for (uint64_t s = 0; s < (1ULL << log_slices); s++) acc += partial[(s << m) + o];
-        out[o] = acc;
-    }
-}
-
 // ---- generic reductions -----------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_sum_stage1(const double *__restrict__ x, uint64_t count,
-                                                       double *__restrict__ partial) {
-    double acc[1] = {0.0};
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock)
-        acc[0] += x[i];
-    block_reduce<1>(acc, partial + blockIdx.x);
-}
-
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_sum_final(const double *__restrict__ partial, int n_partial,
                                                       double *__restrict__ out) {
@@ -186,11 +149,6 @@ __global__ void __launch_bounds__(kBlock) k_kraus_stage1(const double2 *__restri
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scale(double *__restrict__ x, uint64_t count, double f) {
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock)
-        x[i] *= f;
-}
-
 __global__ void __launch_bounds__(kBlock) k_scale_state(double2 *__restrict__ x, uint64_t count, double f) {
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock) {
         double2 a = x[i];
@@ -198,219 +156,46 @@ __global__ void __launch_bounds__(kBlock) k_scale_state(double2 *__restrict__ x,
     }
 }
 
-// ---- inclusive scan: tiles of kScanTile values ------------------------------
-constexpr int kScanPer = 16;
-constexpr int kScanTile = kBlock * kScanPer;  // 4096
-
-// Sum of each tile.
-__global__ void __launch_bounds__(kBlock) k_tile_sums(const double *__restrict__ x, uint64_t count,
-                                                      double *__restrict__ sums) {
-    uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    double acc[1] = {0.0};
-    // coalesced: lane t sums elements t, t + 256, ... of the tile (fixed order)
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        uint64_t i = base + threadIdx.x + (uint64_t)j * kBlock;
-        if (i < count) acc[0] += x[i];
-    }
-    block_reduce<1>(acc, sums + blockIdx.x);
-}
-
-// In-place inclusive scan of each tile, plus offsets[blockIdx] (exclusive prefix of tile sums) when non-null.
-// The tile is loaded and stored coalesced through LDS (one pad slot per 16 values keeps the per-thread
-// runs of 16 on distinct banks); each thread scans 16 consecutive values, then the thread totals are scanned.
-__global__ void __launch_bounds__(kBlock) k_tile_scan(double *__restrict__ x, uint64_t count,
-                                                      const double *__restrict__ offsets) {
+// Largest |amp[i] - ref_i| with ref_i = a at i == index and b elsewhere (a two-level reference state), per block.
+__global__ void __launch_bounds__(kBlock) k_maxdev_stage1(const double2 *__restrict__ amp, uint64_t count,
+                                                          uint64_t index, double2 a, double2 b,
+                                                          double *__restrict__ partial) {
     __shared__ double sh[kBlock];
-    __shared__ double tile[kScanTile + kScanTile / kScanPer];
-    uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        int l = threadIdx.x + j * kBlock;
-        uint64_t i = base + l;
-        tile[l + l / kScanPer] = i < count ? x[i] : 0.0;
+    double m = 0.0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock) {
+        const double2 x = amp[i];
+        const double2 r = i == index ? a : b;
+        const double dr = x.x - r.x, di = x.y - r.y;
+        const double d = sqrt(dr * dr + di * di);
+        m = (d > m || d != d) ? d : m;  // NaN propagates
     }
+    sh[threadIdx.x] = m;
     __syncthreads();
-    double v[kScanPer];
-    double run = 0.0;
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        int l = threadIdx.x * kScanPer + j;
-        run += tile[l + l / kScanPer];
-        v[j] = run;
-    }
-    // Hillis-Steele scan of per-thread totals (fixed order).
-    sh[threadIdx.x] = run;
-    __syncthreads();
-    for (int s = 1; s < kBlock; s <<= 1) {
-        double add = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0.0;
-        __syncthreads();
-        sh[threadIdx.x] += add;
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = sh[threadIdx.x + s];
+            if (o > sh[threadIdx.x] || o != o) sh[threadIdx.x] = o;
+        }
         __syncthreads();
     }
-    double prefix = (threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0);
-    if (offsets) prefix += (blockIdx.x > 0 ? offsets[blockIdx.x - 1] : 0.0);
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        int l = threadIdx.x * kScanPer + j;
-        tile[l + l / kScanPer] = prefix + v[j];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        int l = threadIdx.x + j * kBlock;
-        uint64_t i = base + l;
-        if (i < count) x[i] = tile[l + l / kScanPer];
-    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-__global__ void k_set_last_one(double *x, uint64_t count) { x[count - 1] = 1.0; }
-
-// ---- full-register sampling without a probability array ------------------------------------------------
-// k_tile_scan's in-tile order applied to the probabilities of tile t of the state (|a|^2, times scale):
-// the tile goes to LDS (padded as there), each thread accumulates its 16 consecutive values into v[], and
-// the thread totals are scanned in sh[].  Returns the thread's exclusive prefix within the tile; the tile
-// total is sh[kBlock - 1].
-__device__ double state_tile_scan(const double2 *__restrict__ amp, uint64_t base, uint64_t count, double scale,
-                                  double *tile, double *sh, double (&v)[kScanPer]) {
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        const int l = threadIdx.x + j * kBlock;
-        const uint64_t i = base + l;
-        double p = i < count ? prob_nofma(amp[i]) : 0.0;
-        if (scale != 1.0) p *= scale;
-        tile[l + l / kScanPer] = p;
-    }
-    __syncthreads();
-    double run = 0.0;
-#pragma unroll
-    for (int j = 0; j < kScanPer; j++) {
-        const int l = threadIdx.x * kScanPer + j;
-        run += tile[l + l / kScanPer];
-        v[j] = run;
-    }
-    sh[threadIdx.x] = run;
-    __syncthreads();
-    for (int s = 1; s < kBlock; s <<= 1) {
-        const double add = (int)threadIdx.x >= s ? sh[threadIdx.x - s] : 0.0;
-        __syncthreads();
-        sh[threadIdx.x] += add;
-        __syncthreads();
-    }
-    return threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0;
-}
-
-// totals[t]: the last in-tile cumulative value of tile t
-__global__ void __launch_bounds__(kBlock) k_state_tile_totals(const double2 *__restrict__ amp, uint64_t count,
-                                                              double scale, double *__restrict__ totals) {
+__global__ void __launch_bounds__(kBlock) k_max_final(const double *__restrict__ partial, int n_partial,
+                                                      double *__restrict__ out) {
     __shared__ double sh[kBlock];
-    __shared__ double tile[kScanTile + kScanTile / kScanPer];
-    double v[kScanPer];
-    (void)state_tile_scan(amp, (uint64_t)blockIdx.x * kScanTile, count, scale, tile, sh, v);
-    if (threadIdx.x == kBlock - 1) totals[blockIdx.x] = sh[kBlock - 1];
-}
-
-// One workgroup per shot: the tile whose cumulative end (ends = inclusive scan of the tile totals, last
-// forced to 1.0) first reaches u, then inside it the smallest index whose cumulative value reaches u -- the
-// result of java.util.Arrays.binarySearch + StatevectorEngine.sampleOutcome (:325-338) on the cumulative
-// array sc_sample otherwise materialises.
-__global__ void __launch_bounds__(kBlock) k_search_state_tiles(const double2 *__restrict__ amp, uint64_t count,
-                                                               double scale, const double *__restrict__ ends,
-                                                               uint64_t ntiles, const double *__restrict__ u,
-                                                               int64_t shots, int64_t *__restrict__ out,
-                                                               double *__restrict__ gap) {
-    __shared__ double sh[kBlock];
-    __shared__ double tile[kScanTile + kScanTile / kScanPer];
-    __shared__ uint64_t best[kBlock];
-    for (int64_t s = blockIdx.x; s < shots; s += gridDim.x) {
-        const double value = u[s];
-        uint64_t t = ntiles - 1;
-        if (value < 1.0) {
-            uint64_t lo = 0, hi = ntiles - 1;  // smallest t with ends[t] >= value (ends[ntiles - 1] = 1.0)
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                if (ends[mid] >= value) hi = mid;
-                else lo = mid + 1;
-            }
-            t = lo;
-        }
-        const uint64_t base = t * (uint64_t)kScanTile;
-        double v[kScanPer];
-        const double prefix = state_tile_scan(amp, base, count, scale, tile, sh, v) + (t > 0 ? ends[t - 1] : 0.0);
-        uint64_t cand = ~0ULL;
-#pragma unroll
-        for (int j = 0; j < kScanPer; j++) {
-            const int l = threadIdx.x * kScanPer + j;
-            const uint64_t i = base + l;
-            const double c = i == count - 1 ? 1.0 : prefix + v[j];
-            tile[l + l / kScanPer] = c;
-            if (i < count && cand == ~0ULL && (value >= 1.0 ? i == count - 1 : c >= value)) cand = i;
-        }
-        best[threadIdx.x] = cand;
-        __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) best[threadIdx.x] = best[threadIdx.x] < best[threadIdx.x + w] ? best[threadIdx.x]
-                                                                                                  : best[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            uint64_t r = best[0];
-            if (r == ~0ULL) r = base + kScanTile < count ? base + kScanTile : count - 1;  // rounding at a tile end
-            out[s] = (int64_t)r;
-            if (gap) {
-                const uint64_t l = r - base;
-                double g = fabs(value - (r < base + kScanTile ? tile[l + l / kScanPer] : ends[t]));
-                if (r > base && r < base + kScanTile) g = fmin(g, fabs(value - tile[(l - 1) + (l - 1) / kScanPer]));
-                else if (r == base && t > 0) g = fmin(g, fabs(value - ends[t - 1]));
-                gap[s] = g;
-            }
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n_partial; i += kBlock) m = (partial[i] > m || partial[i] != partial[i]) ? partial[i] : m;
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = sh[threadIdx.x + s];
+            if (o > sh[threadIdx.x] || o != o) sh[threadIdx.x] = o;
         }
         __syncthreads();
     }
-}
-
-// ---- shot search: java.util.Arrays.binarySearch + StatevectorEngine.sampleOutcome (:325-338) -------------
-__global__ void __launch_bounds__(kBlock) k_search(const double *__restrict__ cum, int64_t len,
-                                                   const double *__restrict__ u, int64_t shots,
-                                                   int64_t *__restrict__ out, double *__restrict__ gap) {
-    for (int64_t s = blockIdx.x * (int64_t)kBlock + threadIdx.x; s < shots; s += (int64_t)gridDim.x * kBlock) {
-        double value = u[s];
-        int64_t result;
-        if (value >= 1.0) {
-            result = len - 1;
-        } else {
-            int64_t low = 0, high = len - 1, found = -1;
-            while (low <= high) {
-                int64_t mid = (int64_t)(((uint64_t)low + (uint64_t)high) >> 1);
-                double midVal = cum[mid];
-                if (midVal < value) {
-                    low = mid + 1;
-                } else if (midVal > value) {
-                    high = mid - 1;
-                } else {
-                    long long mb = __double_as_longlong(midVal), kb = __double_as_longlong(value);
-                    if (mb == kb) {
-                        found = mid;
-                        break;
-                    }
-                    if (mb < kb) low = mid + 1;
-                    else high = mid - 1;
-                }
-            }
-            if (found >= 0) {
-                while (found > 0 && cum[found - 1] >= value) found--;
-                result = found;
-            } else {
-                result = low;
-            }
-        }
-        out[s] = result;
-        if (gap) {
-            double g = fabs(value - cum[result]);
-            if (result > 0) g = fmin(g, fabs(value - cum[result - 1]));
-            gap[s] = g;
-        }
-    }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 unsigned grid_for(Context *ctx, uint64_t total) {
@@ -449,42 +234,10 @@ int launch_probabilities_marginal(Context *ctx, State *st, int elem, const int32
     for (int k = 0; k < m; k++) mmask |= 1ULL << measured[k];
     const uint64_t full = st->dim() - 1ULL;
     const uint64_t cmask = full & ~mmask;
-    const int cbits = st->n - m;
     const double2 *amp = st->amp + ((uint64_t)elem << st->n);
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)st->dim());
-    if (cbits >= 8 && m <= 12) {
-        // slices so that outcomes * slices >= ~4096 blocks
-        int log_slices = 0;
-        while (m + log_slices < 12 && log_slices < cbits - 8) log_slices++;
-        size_t need = ((size_t)1 << (m + log_slices)) * sizeof(double);
-        int rc = ensure_scratch(ctx, need);
-        if (rc) return rc;
-        double *partial = (double *)ctx->scratch;
-        hipLaunchKernelGGL(k_prob_marginal_wide, dim3(1u << m, 1u << log_slices), dim3(kBlock), 0, ctx->stream, amp,
-                           mmask, cmask, cbits, log_slices, m, partial);
-        SC_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_slices, dim3(grid_for(ctx, 1ULL << m)), dim3(kBlock), 0, ctx->stream, partial,
-                           log_slices, m, out_dev);
-        SC_HIP_CHECK(hipGetLastError());
-    } else {
-        hipLaunchKernelGGL(k_prob_marginal_narrow, dim3(grid_for(ctx, 1ULL << m)), dim3(kBlock), 0, ctx->stream, amp,
-                           mmask, cmask, 1ULL << m, out_dev);
-        SC_HIP_CHECK(hipGetLastError());
-    }
-    return SC_OK;
-}
-
-int device_sum(Context *ctx, const double *x, uint64_t count, double *result_host) {
-    int rc = ensure_scratch(ctx, (size_t)(kRedGrid + 1) * sizeof(double));
-    if (rc) return rc;
-    double *partial = (double *)ctx->scratch;
-    hipLaunchKernelGGL(k_sum_stage1, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, x, count, partial);
-    SC_HIP_CHECK(hipGetLastError());
-    return finish_reduce<1>(ctx, partial, kRedGrid, result_host);
-}
-
-int launch_scale(Context *ctx, double *x, uint64_t count, double factor) {
-    hipLaunchKernelGGL(k_scale, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx->stream, x, count, factor);
+    hipLaunchKernelGGL(k_prob_marginal_narrow, dim3(grid_for(ctx, 1ULL << m)), dim3(kBlock), 0, ctx->stream, amp,
+                       mmask, cmask, 1ULL << m, out_dev);
     SC_HIP_CHECK(hipGetLastError());
     return SC_OK;
 }
@@ -493,87 +246,6 @@ int launch_scale_state(Context *ctx, double2 *amp, uint64_t count, double factor
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 32.0 * (double)count);
     hipLaunchKernelGGL(k_scale_state, dim3(grid_for(ctx, count)), dim3(kBlock), 0, ctx->stream, amp, count, factor);
     SC_HIP_CHECK(hipGetLastError());
-    return SC_OK;
-}
-
-// Recursive tiled inclusive scan; scratch holds the tile sums of every level.
-static int scan_level(Context *ctx, double *x, uint64_t count, double *tmp) {
-    if (count <= (uint64_t)kScanTile) {
-        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kBlock), 0, ctx->stream, x, count, (const double *)nullptr);
-        SC_HIP_CHECK(hipGetLastError());
-        return SC_OK;
-    }
-    uint64_t tiles = (count + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, x, count, tmp);
-    SC_HIP_CHECK(hipGetLastError());
-    int rc = scan_level(ctx, tmp, tiles, tmp + tiles);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)tiles), dim3(kBlock), 0, ctx->stream, x, count, tmp);
-    SC_HIP_CHECK(hipGetLastError());
-    return SC_OK;
-}
-
-int launch_cumulative(Context *ctx, double *p, uint64_t count) {
-    uint64_t need = 0, c = count;
-    while (c > (uint64_t)kScanTile) {
-        c = (c + kScanTile - 1) / kScanTile;
-        need += c;
-    }
-    int rc = ensure_scratch(ctx, (size_t)(need + 1) * sizeof(double));
-    if (rc) return rc;
-    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 24.0 * (double)count);
-    rc = scan_level(ctx, p, count, (double *)ctx->scratch);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_set_last_one, dim3(1), dim3(1), 0, ctx->stream, p, count);
-    SC_HIP_CHECK(hipGetLastError());
-    return SC_OK;
-}
-
-int launch_search(Context *ctx, const double *cum, uint64_t count, const double *u_dev, int64_t shots,
-                  int64_t *out_dev, double *gap_dev) {
-    hipLaunchKernelGGL(k_search, dim3(grid_for(ctx, (uint64_t)shots)), dim3(kBlock), 0, ctx->stream, cum,
-                       (int64_t)count, u_dev, shots, out_dev, gap_dev);
-    SC_HIP_CHECK(hipGetLastError());
-    return SC_OK;
-}
-
-// Sampling over every qubit in index order: no 2^n probability array.  One read of the state gives the
-// tile totals (k_tile_scan's order), whose sum is the normalisation (StatevectorEngine.normalize :298-312:
-// rescaled only when off by more than 1e-12, then the totals are read again with the factor); the totals'
-// inclusive scan, last forced to 1.0, gives the tile ends, and each shot rescans one tile.
-int sample_full_state(Context *ctx, State *st, int elem, const double *u_dev, int64_t shots, int64_t *out_dev,
-                      double *gap_dev) {
-    const uint64_t count = st->dim();
-    const uint64_t ntiles = (count + kScanTile - 1) / kScanTile;
-    const double2 *amp = st->amp + ((uint64_t)elem << st->n);
-    DevBuf ends_buf;
-    int rc = ends_buf.alloc(ntiles * sizeof(double));
-    if (rc) return rc;
-    double *ends = (double *)ends_buf.p;
-    {
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)count);
-        hipLaunchKernelGGL(k_state_tile_totals, dim3((unsigned)ntiles), dim3(kBlock), 0, ctx->stream, amp, count, 1.0,
-                           ends);
-        SC_HIP_CHECK(hipGetLastError());
-    }
-    double sum = 0.0;
-    if ((rc = device_sum(ctx, ends, ntiles, &sum))) return rc;
-    if (sum == 0.0) return fail(SC_E_STATE, "State collapsed to zero probability amplitudes");
-    double scale = 1.0;
-    if (std::fabs(sum - 1.0) > 1e-12) {
-        scale = 1.0 / sum;
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)count);
-        hipLaunchKernelGGL(k_state_tile_totals, dim3((unsigned)ntiles), dim3(kBlock), 0, ctx->stream, amp, count,
-                           scale, ends);
-        SC_HIP_CHECK(hipGetLastError());
-    }
-    if ((rc = launch_cumulative(ctx, ends, ntiles))) return rc;
-    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
-    const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)shots, (uint64_t)ctx->cu_count * 8);
-    hipLaunchKernelGGL(k_search_state_tiles, dim3(grid), dim3(kBlock), 0, ctx->stream, amp, count, scale,
-                       (const double *)ends, ntiles, u_dev, shots, out_dev, gap_dev);
-    SC_HIP_CHECK(hipGetLastError());
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // ends_buf is released on return
     return SC_OK;
 }
 
@@ -637,6 +309,24 @@ int launch_kraus_weights(Context *ctx, const double2 *amp, int n, int target, co
     case 3: return finish_reduce<3>(ctx, partial, kRedGrid, weights_host);
     default: return finish_reduce<4>(ctx, partial, kRedGrid, weights_host);
     }
+}
+
+int launch_max_deviation(Context *ctx, const double2 *amp, uint64_t count, uint64_t index, double2 a, double2 b,
+                         double *out_host) {
+    int rc = ensure_scratch(ctx, (size_t)(kRedGrid + 1) * sizeof(double));
+    if (rc) return rc;
+    double *partial = (double *)ctx->scratch;
+    {
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)count);
+        hipLaunchKernelGGL(k_maxdev_stage1, dim3(kRedGrid), dim3(kBlock), 0, ctx->stream, amp, count, index, a, b,
+                           partial);
+        SC_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, ctx->stream, partial, kRedGrid, partial + kRedGrid);
+    SC_HIP_CHECK(hipGetLastError());
+    SC_HIP_CHECK(hipMemcpyAsync(out_host, partial + kRedGrid, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
 }
 
 }  // namespace sc
