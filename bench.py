@@ -209,6 +209,8 @@ def bench_c2(args):
         "measure_ms_per_step": m_ms / args.steps,
         "checksum": list(cs),
         "sample_boundary_gap": gap,
+        # the exact sequential CDF: 4096-value segments summed as integers / serially / all zero
+        "sample_exact_sums": ctx.sample_stats(),
         "device": {"hbm_total_gib": total / 2**30, "cus": cus},
     }
     if not args.no_cpu_baseline:
@@ -309,23 +311,10 @@ def bench_c3(args):
 
 def bench_c4(args):
     """Batched QAOA MaxCut rings (instances of 12..22 qubits, p = 4) and depolarising GHZ-20 trajectories."""
-    rng = SplittableRandom(4)
-    n_inst, p = args.instances, 4
-    sizes = (rng.next_longs(n_inst) % np.uint64(11)).astype(np.int64) + 12
+    n_inst = args.instances
     ctx = make_context(args)
-    batches = []
-    for n in range(12, 23):
-        idx = np.nonzero(sizes == n)[0]
-        if len(idx) == 0:
-            continue
-        B = len(idx)
-        g = rng.next_doubles(B * p).reshape(B, p) * math.pi
-        b = rng.next_doubles(B * p).reshape(B, p) * math.pi
-        w = 0.5 + rng.next_doubles(B * n).reshape(B, n)
-        ops, _ = lower(workloads.qaoa_ring(n, p, g[0], b[0], w[0]))
-        mats = np.stack([lower(workloads.qaoa_ring(n, p, g[k], b[k], w[k]))[1] for k in range(B)])
-        zm = np.array([(1 << i) | (1 << ((i + 1) % n)) for i in range(n)], dtype=np.uint64)
-        batches.append((n, B, ops, mats, w, zm))
+    batches = [(n, len(idx), ops, mats, w, workloads.ring_zz_masks(n))
+               for n, idx, ops, mats, w in workloads.qaoa_batches(n_inst, 4, 4)]
 
     # the instance states stay allocated in HBM across steps (47.7 GiB for 4096 instances); a step starts
     # every instance from |0..0>
@@ -344,10 +333,7 @@ def bench_c4(args):
 
     # GHZ-20 trajectories with depolarising p = 0.01 on both operands after every CNOT
     nt, nq = args.trajectories, 20
-    gops, gmats = lower(workloads.ghz(nq))
-    noise_after = (gops["kind"] == capi.OP_CNOT).astype(np.uint8)
-    draws = 2 * int(noise_after.sum())
-    u = np.stack([SplittableRandom(1000 + t).next_doubles(draws) for t in range(nt)])
+    gops, gmats, noise_after, u = workloads.ghz_trajectory_uniforms(nt, nq)
 
     tstate = capi.State(ctx, nq, nt)
 

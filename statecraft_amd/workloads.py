@@ -165,3 +165,45 @@ def hea_observable(n: int, a, b):
 
 def with_params(circuit_builder, thetas):
     return circuit_builder(thetas)
+
+
+def qaoa_batches(instances: int = 4096, p: int = 4, seed: int = 4):
+    """Config 4's QAOA instances grouped by size: sizes n ~ U{12..22} (nextLong() % 11 + 12 per instance), then
+    per size in increasing order gamma, beta ~ U[0, pi) (B x p each) and ring weights ~ U[0.5, 1.5) (B x n), all
+    from one SplittableRandom(seed).  Returns [(n, instance indices, ops, per-instance matrices, weights)]."""
+    from .circuit import lower
+
+    rng = SplittableRandom(seed)
+    sizes = (rng.next_longs(instances) % np.uint64(11)).astype(np.int64) + 12
+    out = []
+    for n in range(12, 23):
+        idx = np.nonzero(sizes == n)[0]
+        if len(idx) == 0:
+            continue
+        B = len(idx)
+        g = rng.next_doubles(B * p).reshape(B, p) * math.pi
+        b = rng.next_doubles(B * p).reshape(B, p) * math.pi
+        w = 0.5 + rng.next_doubles(B * n).reshape(B, n)
+        ops, _ = lower(qaoa_ring(n, p, g[0], b[0], w[0]))
+        mats = np.stack([lower(qaoa_ring(n, p, g[k], b[k], w[k]))[1] for k in range(B)])
+        out.append((n, idx, ops, mats, w))
+    return out
+
+
+def ring_zz_masks(n: int) -> np.ndarray:
+    """Z-masks of the ring edges (i, i+1 mod n)."""
+    return np.array([(1 << i) | (1 << ((i + 1) % n)) for i in range(n)], dtype=np.uint64)
+
+
+def ghz_trajectory_uniforms(trajectories: int = 1024, n: int = 20, first_seed: int = 1000):
+    """Config 4's noisy GHZ-n: the circuit's ops, matrices, the noise flag per op (depolarising after every
+    CNOT, on control then target) and, per trajectory t, the nextDouble() stream of SplittableRandom(first_seed
+    + t), one value per scheduled channel."""
+    from .circuit import lower
+
+    ops, mats = lower(ghz(n))
+    noise_after = (ops["kind"] == capi.OP_CNOT).astype(np.uint8)
+    draws = 2 * int(noise_after.sum())
+    u = np.stack([SplittableRandom(first_seed + t).next_doubles(draws) for t in range(trajectories)])
+    return ops, mats, noise_after, u
+
