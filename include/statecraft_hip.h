@@ -206,6 +206,16 @@ int sc_expectation_pauli(sc_state *st, const uint64_t *xmask, const uint64_t *zm
  * 16 per read), so a diagonal observable of many terms costs one pass per 16 terms. */
 int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
                                double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): sc_expectation_pauli_terms with its partial
+ * sums in a fixed 32 MiB device array that the library holds on every GPU it runs on until the process ends,
+ * instead of the context's growable scratch.  Calls no hipMalloc or hipFree.  The call fails with SC_E_ARG
+ * when n_terms * batch exceeds 2^21 (the partial sums of one term row per element no longer fit); it uses
+ * fewer partial sums per row than sc_expectation_pauli_terms when the default count does not fit, which can
+ * change the last bits of a result.  All calls on one GPU share its array: calls in one process take it in
+ * turn, each overwrites what the previous one left, and every value a call reads it wrote first. */
+int sc_expectation_pauli_terms_noalloc(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
+                                       double *out);
 /* Adjoint gradient of E(theta) = <psi(theta)|H|psi(theta)>.  On entry `st` holds the initial states; the
  * call runs the circuit forward (leaving psi_b = U(theta_b)|init_b> in `st`), then sweeps the bra
  * lambda = H psi and the ket psi backwards through U^dagger with one inner-product reduction per

@@ -82,6 +82,7 @@ EXPORTED = [
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
     "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
+    "sc_expectation_pauli_terms_noalloc",
 ]
 
 _lib = None
@@ -137,6 +138,7 @@ def _sig(lib):
         "sc_adjoint_plan_host": (i32, [i32, P, u64, DP, u64, i32, i32, DP, DP, DP]),
         "sc_last_sample_stats": (i32, [P, U64P]),
         "sc_state_max_deviation": (i32, [P, i32, u64, DP, DP, DP]),
+        "sc_expectation_pauli_terms_noalloc": (i32, [P, U64P, U64P, i32, DP]),
         "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
         "sc_sample_host": (i32, [i32, DP, I32P, i32, DP, i64, I64P, DP, U64P]),
     }
@@ -437,6 +439,16 @@ class State:
         out = np.empty((self.batch, len(zm)))
         check(lib().sc_expectation_pauli_terms(self.h, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P), len(zm),
                                                dptr(out)))
+        return out
+
+    def expectation_terms_noalloc(self, xmask, zmask) -> np.ndarray:
+        """expectation_terms through the entry point that allocates no device memory (fixed 32 MiB scratch)."""
+        xm = np.ascontiguousarray(xmask, np.uint64)
+        zm = np.ascontiguousarray(zmask, np.uint64)
+        assert xm.shape == zm.shape
+        out = np.empty((self.batch, len(zm)))
+        check(lib().sc_expectation_pauli_terms_noalloc(self.h, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P),
+                                                       len(zm), dptr(out)))
         return out
 
     def adjoint_gradient(self, ops, mats, n_params: int, xmask, zmask, coeff, checkpoint_every: int = 0,

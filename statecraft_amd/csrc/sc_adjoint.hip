@@ -13,6 +13,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "sc_internal.hpp"
@@ -372,49 +373,57 @@ int finish_parts(Context *ctx, double *part, uint64_t count, double *host) {
     return SC_OK;
 }
 
-// <P_t>_b for every term t and element b: vals[b * n_terms + t], each term a Hermitian Pauli string
-// i^{|x & z|} X^x Z^z.  Terms are grouped by x mask; each launch evaluates up to kTermsPerLaunch terms of one
-// group in one read of the state, and one synchronisation ends the call.
-int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
-                double *vals) {
-    const uint64_t B = (uint64_t)st->batch;
-    const int n = st->n;
-    const uint64_t rows = (uint64_t)n_terms;
-    // partial sums per row: enough workgroups to fill the GPU, at least 8 amplitudes per thread
+// Partial sums per term row for k_pauli_terms: enough workgroups to fill the GPU, at least 8 amplitudes per
+// thread, at most 32 Mi partials; a power of two (the kernel's index split).
+uint64_t term_parts(const Context *ctx, uint64_t B, int n, uint64_t rows) {
     const uint64_t half = n > 0 ? (1ULL << (n - 1)) : 1ULL;
     uint64_t parts = std::max<uint64_t>(1, (8ULL * ctx->cu_count + B - 1) / B);
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, half / (8ULL * kBlock)));
     parts = std::min<uint64_t>(parts, std::max<uint64_t>(1, (32ULL << 20) / (rows * B)));
     parts = std::min<uint64_t>(parts, 4096);
-    parts = 1ULL << (63 - __builtin_clzll(parts));  // a power of two (the kernel's index split)
+    return 1ULL << (63 - __builtin_clzll(parts));
+}
+
+// Device bytes pauli_terms_in needs for `parts` partials per row: the partials, then the row sums.
+size_t term_scratch_bytes(uint64_t B, uint64_t rows, uint64_t parts) { return (size_t)(B * rows * (parts + 1)) * 8; }
+
+// <P_t>_b for every term t and element b: vals[b * n_terms + t], each term a Hermitian Pauli string
+// i^{|x & z|} X^x Z^z.  Terms are grouped by x mask (groups in order of first appearance); each launch
+// evaluates up to kTermsPerLaunch terms of one group in one read of the state, and one synchronisation ends
+// the call.  The partial sums live in `scratch` (scratch_bytes of device memory); fewer partials per row are
+// used when the default count does not fit, and SC_E_ARG is returned when even one per row does not.
+int pauli_terms_in(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
+                   double *vals, void *scratch, size_t scratch_bytes) {
+    const uint64_t B = (uint64_t)st->batch;
+    const int n = st->n;
+    const uint64_t rows = (uint64_t)n_terms;
+    uint64_t parts = term_parts(ctx, B, n, rows);
+    while (parts > 1 && term_scratch_bytes(B, rows, parts) > scratch_bytes) parts >>= 1;
+    if (term_scratch_bytes(B, rows, parts) > scratch_bytes)
+        return fail(SC_E_ARG, "expectation of " + std::to_string(n_terms) + " terms over " + std::to_string(B) +
+                                  " states needs " + std::to_string(term_scratch_bytes(B, rows, 1)) +
+                                  " bytes of scratch, more than the " + std::to_string(scratch_bytes) + " available");
     const size_t part_bytes = (size_t)(B * rows * parts) * sizeof(double);
-    int rc = ensure_scratch(ctx, part_bytes + (size_t)(B * rows) * sizeof(double));
-    if (rc) return rc;
-    double *part = (double *)ctx->scratch;
-    double *out = (double *)((char *)ctx->scratch + part_bytes);
-    // groups in order of first appearance
-    std::vector<uint64_t> gx;
-    std::vector<std::vector<int>> gterms;
-    for (int t = 0; t < n_terms; t++) {
-        size_t k = 0;
-        while (k < gx.size() && gx[k] != xm[t]) k++;
-        if (k == gx.size()) {
-            gx.push_back(xm[t]);
-            gterms.emplace_back();
-        }
-        gterms[k].push_back(t);
-    }
-    for (size_t k = 0; k < gx.size(); k++) {
+    double *part = (double *)scratch;
+    double *out = (double *)((char *)scratch + part_bytes);
+    for (int t0 = 0; t0 < n_terms; t0++) {
+        bool seen = false;  // x mask of an earlier term: its group was done there
+        for (int t = 0; t < t0 && !seen; t++) seen = xm[t] == xm[t0];
+        if (seen) continue;
+        size_t members = 0;
+        for (int t = t0; t < n_terms; t++) members += xm[t] == xm[t0];
         // launches of equal size, at most kTermsPerLaunch terms each
-        const size_t nl = (gterms[k].size() + kTermsPerLaunch - 1) / kTermsPerLaunch;
-        const size_t per = (gterms[k].size() + nl - 1) / nl;
-        for (size_t c0 = 0; c0 < gterms[k].size(); c0 += per) {
+        const size_t nl = (members + kTermsPerLaunch - 1) / kTermsPerLaunch;
+        const size_t per = (members + nl - 1) / nl;
+        int next = t0;  // next member term not yet launched
+        for (size_t c0 = 0; c0 < members; c0 += per) {
             TermGroup g;
             std::memset(&g, 0, sizeof(g));
-            g.x = gx[k];
-            g.nt = (int)std::min<size_t>(per, gterms[k].size() - c0);
+            g.x = xm[t0];
+            g.nt = (int)std::min<size_t>(per, members - c0);
             for (int j = 0; j < g.nt; j++) {
-                const int t = gterms[k][c0 + j];
+                while (xm[next] != xm[t0]) next++;
+                const int t = next++;
                 g.z[j] = zm[t];
                 g.row[j] = t;
                 if (__builtin_popcountll(xm[t] & zm[t]) & 1) g.imag |= 1u << j;
@@ -455,6 +464,30 @@ int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm,
     }
     return SC_OK;
 }
+
+// The same with the context's scratch, grown to the default number of partials.
+int pauli_terms(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
+                double *vals) {
+    const uint64_t B = (uint64_t)st->batch, rows = (uint64_t)n_terms;
+    int rc = ensure_scratch(ctx, term_scratch_bytes(B, rows, term_parts(ctx, B, st->n, rows)));
+    if (rc) return rc;
+    return pauli_terms_in(ctx, st, amp, xm, zm, n_terms, vals, ctx->scratch, ctx->scratch_bytes);
+}
+
+// ---- the fixed scratch array of sc_expectation_pauli_terms_noalloc ------------------------------------
+// 32 MiB of partial sums that every GPU the library runs on holds until the process ends.  In host code the
+// name `scratch` is only a stand-in for the symbol: its device address, one per GPU, comes from
+// hipGetSymbolAddress with that GPU current (init_noalloc_scratch, called by sc_ctx_create) and is the only
+// form in which host code passes the array on.
+constexpr size_t kNoAllocScratchBytes = 32ULL << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+struct NoAllocSlot {
+    std::mutex mu;
+    void *ptr = nullptr;  // device address of `scratch` on this GPU
+};
+constexpr int kNoAllocDevices = 64;
+NoAllocSlot g_noalloc[kNoAllocDevices];
 
 // <psi_b|H|psi_b> for all b: out[b]
 int expectation(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, const double *c,
@@ -630,6 +663,19 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
 
 }  // namespace
 
+// Takes the device address of `scratch` on `device` (which must be current) once per device.
+int init_noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocDevices)
+        return fail(SC_E_UNSUPPORTED, "device index " + std::to_string(device) + " beyond the library's table");
+    NoAllocSlot &slot = g_noalloc[device];
+    std::lock_guard<std::mutex> lock(slot.mu);
+    if (slot.ptr) return SC_OK;
+    void *p = nullptr;
+    SC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    slot.ptr = p;
+    return SC_OK;
+}
+
 int adjoint_plan_host(int n, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                       int n_params, int low_qubits, SweepPlan &plan, std::vector<double> &inv, std::vector<int> &slot) {
     if (n_ops > 0 && !ops) return fail(SC_E_NULL, "ops is null");
@@ -685,6 +731,21 @@ int sc_expectation_pauli_terms(sc_state *st, const uint64_t *xmask, const uint64
     if ((rc = set_device(st->ctx))) return rc;
     if ((rc = materialize_basis(st))) return rc;
     return pauli_terms(st->ctx, st, st->amp, xmask, zmask, n_terms, out);
+}
+
+// No counterpart in the reference: sc_expectation_pauli_terms for callers that cannot allocate device memory.
+int sc_expectation_pauli_terms_noalloc(sc_state *st, const uint64_t *xmask, const uint64_t *zmask, int n_terms,
+                                       double *out) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (!out) return fail(SC_E_NULL, "out is null");
+    int rc = check_terms(st, xmask, zmask, nullptr, n_terms, false);
+    if (rc) return rc;
+    if ((rc = set_device(st->ctx))) return rc;
+    if ((rc = init_noalloc_scratch(st->ctx->device))) return rc;  // done by sc_ctx_create already
+    NoAllocSlot &slot = g_noalloc[st->ctx->device];
+    std::lock_guard<std::mutex> lock(slot.mu);  // calls on one device take the array in turn
+    if ((rc = materialize_basis(st))) return rc;
+    return pauli_terms_in(st->ctx, st, st->amp, xmask, zmask, n_terms, out, slot.ptr, kNoAllocScratchBytes);
 }
 
 int sc_adjoint_plan_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,

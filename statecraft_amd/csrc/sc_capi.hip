@@ -430,6 +430,11 @@ int sc_ctx_create(int device, sc_ctx **out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->cu_count = cus;
+    int rc = init_noalloc_scratch(device);
+    if (rc) {
+        (void)hipStreamDestroy(ctx->stream);
+        return rc;
+    }
     *out = ctx.release();
     return SC_OK;
 }

@@ -98,6 +98,8 @@ struct State {
 int materialize_basis(State *st);
 
 int ensure_scratch(Context *ctx, size_t bytes);
+// Device address of sc_expectation_pauli_terms_noalloc's fixed scratch array on `device` (current), once per device.
+int init_noalloc_scratch(int device);
 int ensure_host_scratch(Context *ctx, size_t bytes);
 int drain_profiler(Context *ctx);
 
