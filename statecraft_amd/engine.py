@@ -246,16 +246,26 @@ class HipStatevectorEngine:
         rng = SplittableRandom(seed if seed is not None else int.from_bytes(os.urandom(8), "little"))
         n = circuit.qubit_count
         seen_measure = False
+        pending = 0  # ops [pending, i) are applied together, before the next op that schedules a channel
         for i, op in enumerate(circuit.operations):
             if op.kind == capi.OP_MEASURE:
+                if i > pending:
+                    st.apply(ops[pending:i], mats)
+                pending = i + 1
                 seen_measure = True
                 continue
             if seen_measure:
                 raise UnsupportedOperationException(
                     "Unitary operations cannot follow measurement operations in the circuit")
-            st.apply(ops[i:i + 1], mats)
-            for sch in noise.scheduled_channels_after(op):
+            scheduled = noise.scheduled_channels_after(op)
+            if not scheduled:
+                continue
+            st.apply(ops[pending:i + 1], mats)
+            pending = i + 1
+            for sch in scheduled:
                 self._apply_channel(st, sch.channel, sch.target_qubits, rng, n)
+        if len(ops) > pending:
+            st.apply(ops[pending:], mats)
 
     def _apply_channel(self, st, channel, mapped, rng, n):
         if isinstance(channel, CompositeChannel):

@@ -537,15 +537,24 @@ SimulationResult HipStatevectorEngine::simulate(const SimulationRequest &req) {
     if (noise && noise->hasNoise()) {
         SplittableRandom rng(req.noiseSeed() ? *req.noiseSeed() : (int64_t)random_seed());
         bool measured = false;
+        size_t pending = 0;  // ops [pending, i) are applied together, before the next op that schedules a channel
+        auto flush = [&](size_t end) {
+            if (end > pending) check_status(sc_apply_ops(st.s, &ops[pending], end - pending, mats.data(), mats.size() / 2));
+            pending = end;
+        };
         for (size_t i = 0; i < ops.size(); i++) {
             if (ops[i].kind == SC_OP_MEASURE) {
+                flush(i);
+                pending = i + 1;
                 measured = true;
                 continue;
             }
             if (measured)
                 throw UnsupportedOperationException("Unitary operations cannot follow measurement operations in the circuit");
-            check_status(sc_apply_ops(st.s, &ops[i], 1, mats.data(), mats.size() / 2));
-            for (auto &sch : noise->scheduledChannelsAfter(c.operations()[i])) {
+            const auto scheduled = noise->scheduledChannelsAfter(c.operations()[i]);
+            if (scheduled.empty()) continue;
+            flush(i + 1);
+            for (auto &sch : scheduled) {
                 // applyErrorChannel (StatevectorEngine.java:382-407), composite recursion included
                 struct Apply {
                     sc_state *s;
@@ -594,6 +603,7 @@ SimulationResult HipStatevectorEngine::simulate(const SimulationRequest &req) {
                 apply(*sch.channel, &sch.targetQubits);
             }
         }
+        flush(ops.size());
     } else {
         check_status(sc_apply_ops(st.s, ops.data(), ops.size(), mats.data(), mats.size() / 2));
     }

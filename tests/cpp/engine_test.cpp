@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <functional>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -314,6 +315,144 @@ int main() {
         }
         CHECK(threw, "ISE zero probability");
         CHECK(SimulatorEngines::require("statevector-hip")->id() == "statevector-hip", "registry");
+    });
+    // ---- StatevectorEngineTest measurement and limit cases (StatevectorEngineTest.java:282-380) ----
+    test("partialMeasurementTargetsSpecifiedQubits", [&] {
+        QuantumCircuit c = QuantumCircuit(3).append(Hadamard(), 0).append(Hadamard(), 1).appendCnot(0, 2);
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withMeasurement(
+            MeasurementInstruction::counts(5000, {2}).withSeed(0xDEAD)));
+        auto &m = *r.measurement;
+        CHECK(m.type == MeasurementResult::Type::Histogram, "histogram");
+        CHECK(m.measuredQubits == std::vector<int>{2}, "measured qubits");
+        CHECK(m.counts.size() == 2, "two outcomes");
+        const int zeros = m.counts.count(0) ? m.counts.at(0) : 0, ones = m.counts.count(1) ? m.counts.at(1) : 0;
+        CHECK(zeros + ones == 5000 && zeros > 0 && ones > 0, "counts");
+    });
+    test("measurementOperationsDefineDefaultTargets", [&] {
+        QuantumCircuit c = QuantumCircuit(3).append(Hadamard(), 0).append(PauliX(), 2).measure({0, 2});
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withMeasurement(
+            MeasurementInstruction::countsAll(1000).withSeed(42)));
+        auto &m = *r.measurement;
+        CHECK((m.measuredQubits == std::vector<int>{0, 2}), "targets from MeasureOperation");
+        int total = 0;
+        for (auto &kv : m.counts) {
+            total += kv.second;
+            CHECK((kv.first & 2) == 2, "qubit 2 is |1>");
+        }
+        CHECK(total == 1000, "shots");
+    });
+    test("shotSamplingSupportsRawOutcomes", [&] {
+        QuantumCircuit c = QuantumCircuit(2).append(Hadamard(), 0).appendCnot(0, 1);
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withMeasurement(
+            MeasurementInstruction::samplesAll(16).withSeed(123)));
+        auto &m = *r.measurement;
+        CHECK(m.type == MeasurementResult::Type::Samples && m.shots == 16 && m.samples.size() == 16, "samples");
+        for (int64_t o : m.samples) CHECK(o == 0 || o == 3, "bell outcome");
+    });
+    test("unitaryAfterMeasurementThrows", [&] {
+        bool threw = false;
+        try {
+            QuantumCircuit c = QuantumCircuit(1).append(Hadamard(), 0).measure({0}).append(PauliX(), 0);
+            engine.simulate(SimulationRequest::zeroState(c).withMeasurement(MeasurementInstruction::countsAll(32)));
+        } catch (const UnsupportedOperationException &) {
+            threw = true;
+        }
+        CHECK(threw, "UOE");
+    });
+    test("largeCircuitNearTargetLimitRemainsNumericallySane", [&] {
+        const int n = 22;
+        QuantumCircuit c = QuantumCircuit(n).append(PauliX(), 0).append(PauliX(), 5).append(PauliX(), 21)
+                               .appendCnot(21, 1).appendCnot(0, 2).appendSwap(5, 3);
+        auto st = *engine.simulate(SimulationRequest::zeroState(c)).finalState;
+        const uint64_t expected = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 3) | (1u << 21);
+        CHECK(std::fabs(st.amplitude(expected).real() - 1.0) <= EPS && std::fabs(st.amplitude(expected).imag()) <= EPS,
+              "basis amplitude");
+        double ns = 0.0;
+        for (uint64_t i = 0; i < (1ULL << n); i++) ns += std::norm(st.amplitude(i));
+        CHECK(std::fabs(ns - 1.0) <= EPS, "normalised");
+    });
+    test("rejectsQubitCountAboveDenseLimit", [&] {
+        bool threw = false;
+        try {
+            engine.simulate(SimulationRequest::zeroState(QuantumCircuit(35)));
+        } catch (const UnsupportedOperationException &e) {
+            threw = std::string(e.what()).find("supports up to 34 qubits") != std::string::npos;
+        }
+        CHECK(threw, "UOE above 34 qubits");
+    });
+    test("wideMeasurementIsBitstringHistogramMsbLeft", [&] {
+        // 32 measured qubits (beyond Histogram's Integer keys): the BitstringHistogram of TensorNetworkEngine
+        // (MSB-left strings); a basis state gives one deterministic outcome
+        const int n = 32;
+        QuantumCircuit c(n);
+        for (int q : {0, 3, 17, 30, 31}) c = c.append(PauliX(), q);
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withMeasurement(
+            MeasurementInstruction::countsAll(64).withSeed(9), false));
+        auto &m = *r.measurement;
+        std::string want(n, '0');
+        for (int q : {0, 3, 17, 30, 31}) want[n - 1 - q] = '1';
+        CHECK(m.type == MeasurementResult::Type::BitstringHistogram, "bitstring histogram");
+        CHECK(m.bitstringCounts.size() == 1 && m.bitstringCounts.count(want) && m.bitstringCounts.at(want) == 64,
+              "deterministic MSB-left outcome");
+        CHECK(!r.finalState, "no final state requested");
+    });
+    // ---- StatevectorNoiseTest cases (StatevectorNoiseTest.java:84-207) ----
+    test("amplitudeDampingDistributionMatchesProbability", [&] {
+        NoiseModel m;
+        m.afterAllGates(ErrorChannel::amplitudeDamping(0.25, 0));
+        QuantumCircuit c = QuantumCircuit(1).append(PauliX(), 0);
+        int zero = 0;
+        for (int t = 0; t < 1000; t++) {
+            auto st = *engine.simulate(SimulationRequest::zeroState(c).withNoiseModel(m).withNoiseSeed(0xBAD0 + t)).finalState;
+            if (std::norm(st.amplitude(0)) > 0.5) zero++;
+        }
+        CHECK(std::fabs(zero / 1000.0 - 0.25) <= 0.05, "gamma +- 0.05");
+    });
+    test("idleDecoherenceUsesMappedQubitInsteadOfChannelTemplate", [&] {
+        QuantumCircuit c = QuantumCircuit(2).append(Hadamard(), 0);
+        NoiseModel m;
+        m.onQubits(ErrorChannel::amplitudeDamping(1.0, 0), {0, 1});
+        auto r = engine.simulate(SimulationRequest::withInitialState(c, StateVector::fromArray(2, {0, 0, 0, 0, 0, 0, 1, 0}))
+                                     .withNoiseModel(m));
+        const double s = 1.0 / std::sqrt(2.0);
+        CHECK(matches({s, -s, 0.0, 0.0}, *r.finalState), "idle channel on qubit 1");
+    });
+    test("bellDemoAmplitudeDampingSeedThreeKeepsStateNormalized", [&] {
+        NoiseModel m;
+        m.afterAllGates(ErrorChannel::amplitudeDamping(0.9, 0)).afterAllGates(ErrorChannel::amplitudeDamping(0.9, 1));
+        QuantumCircuit c = QuantumCircuit(2).append(Hadamard(), 0).appendCnot(0, 1);
+        auto st = *engine.simulate(SimulationRequest::zeroState(c).withNoiseModel(m).withNoiseSeed(3)).finalState;
+        double ns = 0.0;
+        for (uint64_t i = 0; i < 4; i++) ns += std::norm(st.amplitude(i));
+        CHECK(std::fabs(ns - 1.0) <= EPS, "normalised");
+    });
+    test("amplitudeAndPhaseDampingOnGroundStateSkipZeroBranch", [&] {
+        for (int kind = 0; kind < 2; kind++) {
+            NoiseModel m;
+            m.afterAllGates(kind == 0 ? ErrorChannel::amplitudeDamping(0.9, 1) : ErrorChannel::phaseDamping(0.9, 1));
+            QuantumCircuit c = QuantumCircuit(2).append(Hadamard(), 0);
+            auto r = engine.simulate(SimulationRequest::zeroState(c).withNoiseModel(m).withNoiseSeed(3));
+            const double s = 1.0 / std::sqrt(2.0);
+            CHECK(matches({s, s, 0.0, 0.0}, *r.finalState), kind == 0 ? "amplitude damping" : "phase damping");
+        }
+    });
+    test("scheduledNoisePathsMatchGoldenState", [&] {
+        NoiseModel m;
+        m.afterGate(Operation::Kind::SingleGate, ErrorChannel::phaseFlip(1.0, 0))
+            .onQubits(ErrorChannel::amplitudeDamping(1.0, 0), {1})
+            .afterAllGates(ErrorChannel::phaseFlip(1.0, 0));
+        QuantumCircuit c = QuantumCircuit(2).append(Hadamard(), 0);
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withBasisState({0, 1}).withNoiseModel(m));
+        const double s = 1.0 / std::sqrt(2.0);
+        CHECK(matches({s, -s, 0.0, 0.0}, *r.finalState), "golden");
+    });
+    test("noisyMeasurementPathMatchesGoldenHistogram", [&] {
+        NoiseModel m;
+        m.afterAllGates(ErrorChannel::amplitudeDamping(1.0, 0));
+        QuantumCircuit c = QuantumCircuit(1).append(PauliX(), 0);
+        auto r = engine.simulate(SimulationRequest::zeroState(c).withNoiseModel(m).withMeasurement(
+            MeasurementInstruction::countsAll(16).withSeed(5), false));
+        CHECK(r.measurement->shots == 16 && r.measurement->counts == (std::map<int64_t, int>{{0, 16}}), "{0: 16}");
     });
     std::printf("%d failure(s)\n", failures);
     return failures;

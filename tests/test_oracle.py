@@ -241,3 +241,103 @@ def test_pauli_expectation_matches_dense():
 def test_checksum_weight_definition():
     for i in (0, 1, 2, 1000):
         assert oracle.lib().orc_checksum_weight(i) == SplittableRandom(i * 0x9E3779B97F4A7C15).next_double()
+
+
+# ---- pins for normalize, the probability checksum and the Kraus branch draw -----------------------------------
+
+def _seq_sum(x):
+    s = 0.0
+    for v in np.asarray(x, np.float64).tolist():
+        s = s + v
+    return s
+
+
+@pytest.mark.parametrize("offset", [0.0, 5e-13, -9e-13, 2e-12, -3e-11, 0.25, -0.5])
+def test_normalize_rescales_only_beyond_threshold(offset):
+    # StatevectorEngine.normalize (:298-312): sequential sum; when |sum - 1| > 1e-12 every entry is multiplied by
+    # 1.0 / sum (the reciprocal, rounded once), otherwise the array is left alone bit for bit
+    rng = np.random.default_rng(int(abs(offset) * 1e15) + 3)
+    p = rng.exponential(size=3000)
+    p = p / _seq_sum(p) * (1.0 + offset)
+    want = p.copy()
+    total = _seq_sum(p)
+    if abs(total - 1.0) > 1e-12:
+        want = np.array([v * (1.0 / total) for v in p.tolist()])
+    got = p.copy()
+    assert oracle.lib().orc_normalize(oracle._dp(got), len(got)) == 0
+    assert np.array_equal(got, want)
+    assert (abs(offset) > 1e-12) == (not np.array_equal(got, p))
+    assert oracle.lib().orc_normalize(oracle._dp(np.zeros(4)), 4) != 0  # zero probability: IllegalStateException
+
+
+def test_normalize_then_sample_on_unnormalised_state_brute_force():
+    # smallest i with cum[i] >= u on the rescaled sequential CDF (last forced to 1), by a plain Python search
+    rng = np.random.default_rng(21)
+    n = 7
+    s = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)) * 3.0
+    p = [(a.real * a.real) + (a.imag * a.imag) for a in s.tolist()]
+    total = _seq_sum(p)
+    scale = 1.0 / total
+    q = [v * scale for v in p]
+    cum, run = [], 0.0
+    for v in q:
+        run = run + v
+        cum.append(run)
+    cum[-1] = 1.0
+    u = rng.random(400)
+    want = [next(i for i, c in enumerate(cum) if c >= x) for x in u.tolist()]
+    assert np.array_equal(oracle.sample(s, list(range(n)), u), want)
+
+
+def test_probability_checksum_against_independent_sums():
+    # out[0] = sum p_i, out[1] = sum p_i w(i), w(i) = SplitMix64 of (i + 1) * golden gamma, top 53 bits * 2^-53,
+    # accumulated left to right (Python floats, ints for the mixing)
+    rng = np.random.default_rng(22)
+    n = 9
+    s = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    M = (1 << 64) - 1
+
+    def w(i):
+        z = ((i + 1) * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        return (z >> 11) * 2.0 ** -53
+
+    s0 = s1 = 0.0
+    for i, a in enumerate(s.tolist()):
+        p = (a.real * a.real) + (a.imag * a.imag)
+        s0 = s0 + p
+        s1 = s1 + p * w(i)
+    assert oracle.checksum(s) == (s0, s1)
+
+
+def test_sample_weighted_index_brute_force():
+    # StatevectorEngine.sampleWeightedIndex (:579-588): first i with value < running sum, else the last index
+    rng = np.random.default_rng(23)
+    f = oracle.lib().orc_sample_weighted_index
+    for trial in range(300):
+        k = int(rng.integers(1, 6))
+        w = rng.random(k) * (rng.random(k) > 0.3)
+        total = _seq_sum(w)
+        for value in (0.0, total * rng.random(), total, math.nextafter(total, 0.0), _seq_sum(w[:1])):
+            run, want = 0.0, k - 1
+            for i, x in enumerate(w.tolist()):
+                run = run + x
+                if value < run:
+                    want = i
+                    break
+            arr = np.ascontiguousarray(w)
+            assert f(oracle._dp(arr), k, float(value)) == want, (w, value)
+
+
+def test_amplitude_damping_statistics():
+    # StatevectorNoiseTest.amplitudeDampingDistributionMatchesProbability (:85-106): X then amplitude damping
+    # gamma = 0.25 on |0>, seeds 0xBAD0 + trial, 1000 trials: the fraction relaxed to |0> is 0.25 +- 0.05
+    k = ErrorChannel.amplitude_damping(0.25, 0).kraus
+    zero = 0
+    for trial in range(1000):
+        s = np.array([0, 1], complex)
+        oracle.apply_channel_1q(s, 0, k, [0xBAD0 + trial])
+        zero += abs(s[0]) ** 2 > 0.5
+    assert abs(zero / 1000 - 0.25) <= 0.05
