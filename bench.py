@@ -7,7 +7,8 @@ circuit with the amplitudes it must change (1q / diagonal: 2^n, CNOT: 2^(n-1)).
 
 Also reported: the sweep kernel's achieved HBM bandwidth (HIP events on the engine's stream, bytes =
 32 B x 2^n per sweep) against the MI355X peak, and the CPU oracle (plain C port of the reference
-StatevectorOps, 1 thread) timed on a bounded sample of the same workload.
+StatevectorOps split over the host's threads like StatevectorParallelExecutor) timed on a bounded sample of
+the same circuit family at 29 qubits, the reference's dense limit.
 
   python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--no-cpu-baseline]
 """
@@ -51,25 +52,96 @@ def make_context(args) -> capi.Context:
     return ctx
 
 
-def cpu_baseline(n, ops, mats, seconds: float):
-    """The oracle (C port of StatevectorOps, 1 thread) on the first ops of the same circuit until `seconds`."""
+def host_threads() -> int:
+    """Host threads for the CPU baseline: the process's CPU share (OMP_NUM_THREADS caps it when set)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(env))) if env and env.isdigit() else n
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _next_int_pow2(rng: SplittableRandom, bound: int) -> int:
+    """SplittableRandom.nextInt(bound) for a power-of-two bound: the low bits of mix32(nextSeed()) (the JDK's
+    published algorithm; not checked against a JDK here)."""
+    m64 = (1 << 64) - 1
+    rng.seed = (rng.seed + 0x9E3779B97F4A7C15) & m64
+    z = rng.seed
+    z = ((z ^ (z >> 33)) * 0x62A9D9ED799705F5) & m64
+    r = ((((z ^ (z >> 28)) * 0xCB24D0A5C88C35B3) & m64) >> 32) & 0xFFFFFFFF
+    return r & (bound - 1)
+
+
+def cpu_baseline(depth: int, seed: int, seconds: float):
+    """The reference's CPU path restated (oracle/sv_parallel.c: StatevectorParallelExecutor's leaf split on P
+    threads, the scalar pair kernels), timed on this host: the same random layered family at 29 qubits, the
+    reference's dense limit, op by op for about `seconds`; then the stock StatevectorKernelMicrobenchmark shape
+    (16 qubits, 384 random X/Y/Z/H gates from SplittableRandom(0xBEEF), 64 iterations, serial and parallel)."""
     import oracle
 
+    P = host_threads()
+    n = 29
+    ops, mats = lower(workloads.random_layered(n, depth, seed))
     st = np.zeros(1 << n, dtype=np.complex128)
+    st[:] = 0.0  # touch every page before timing
     st[0] = 1.0
-    credit = 0
-    done = 0
+    credit, done = 0, 0
     t0 = time.perf_counter()
     for i in range(len(ops)):
-        oracle.run_ops_inplace(st, n, ops[i:i + 1], mats)
-        k = int(ops[i]["kind"])
-        credit += (1 << (n - 1)) if k == capi.OP_CNOT else (1 << n)
+        oracle.run_ops_parallel(st, n, ops[i:i + 1], mats, P, 4096)
+        credit += (1 << (n - 1)) if int(ops[i]["kind"]) == capi.OP_CNOT else (1 << n)
         done += 1
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
     del st
-    return credit / dt, done, dt
+    # StatevectorKernelMicrobenchmark (engines/src/test/.../StatevectorKernelMicrobenchmark.java:27-107)
+    from statecraft_amd.circuit import Hadamard, PauliX, PauliY, PauliZ
+
+    rng = SplittableRandom(0xBEEF)
+    cands = [PauliX(), PauliY(), PauliZ(), Hadamard()]
+    gates = []
+    for _ in range(384):
+        g = cands[_next_int_pow2(rng, 4)]
+        t = _next_int_pow2(rng, 16)
+        gates.append((t, np.array([g.element(0, 0), g.element(0, 1), g.element(1, 0), g.element(1, 1)])))
+    u = rng.next_doubles(2 << 16) - 0.5
+    base = (u[0::2] + 1j * u[1::2]).astype(np.complex128)
+    base *= 1.0 / math.sqrt(float(np.sum(base.real * base.real + base.imag * base.imag)))
+
+    targets = np.array([t for t, _ in gates], np.int32)
+    mats16 = np.stack([g for _, g in gates])
+
+    def run(threads, iters):
+        s = base.copy()
+        t1 = time.perf_counter()
+        oracle.gate_loop(s, 16, targets, mats16, iters, threads, 1024)
+        return time.perf_counter() - t1
+
+    run(1, 16)
+    serial_s, par_s = run(1, 64), run(P, 64)
+    micro_updates = (1 << 16) * 384 * 64
+    return {
+        "value": credit / dt, "unit": "amp-updates/s", "cores": P, "kind": "port",
+        "sample": f"first {done} of the 1490 ops of the same random layered family at 29 qubits (the reference's "
+                  f"dense limit; 8 GiB state) in {dt:.1f} s on {P} threads of {cpu_model()}: C restatement of "
+                  f"StatevectorOps' pair kernels split like StatevectorParallelExecutor (the Java reference needs "
+                  f"a JDK, absent on this host)",
+        "microbenchmark": {
+            "shape": "16 qubits, 384 random X/Y/Z/H gates x 64 iterations (StatevectorKernelMicrobenchmark)",
+            "serial_ms": 1e3 * serial_s, "parallel_ms": 1e3 * par_s, "threads": P,
+            "serial_amp_updates_per_s": micro_updates / serial_s,
+            "parallel_amp_updates_per_s": micro_updates / par_s,
+        },
+    }
 
 
 def pmc_traffic(fetch_csv: str, write_csv: str, copy_bytes: float):
@@ -213,14 +285,6 @@ def bench_c2(args):
         "sample_exact_sums": ctx.sample_stats(),
         "device": {"hbm_total_gib": total / 2**30, "cus": cus},
     }
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread) ...")
-        rate, done, dt = cpu_baseline(n, ops, mats, args.cpu_seconds)
-        result["cpu_baseline"] = {
-            "value": rate, "unit": "amp-updates/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} ops of the same {n}-qubit circuit in {dt:.1f} s (C port of StatevectorOps, "
-                      f"scalar, 1 thread; the Java reference needs a JDK, absent on this box)",
-        }
     st.close()
     ctx.close()
     return result
@@ -441,6 +505,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", default="sweep", choices=list(MODES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="c2: skip the adjoint-gradient block of the metric")
     ap.add_argument("--no-copy-reference", action="store_true")
     ap.add_argument("--low-qubits", type=int, default=None,
                     help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")
@@ -472,6 +537,19 @@ def main(argv=None) -> int:
               file=sys.stderr)
         return 2
     res = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+    if args.workload == "c2" and not args.no_adjoint:
+        # the metric's second half: adjoint gradients of the 24-qubit, 384-parameter HEA against parameter shift
+        log("adjoint block (24-qubit HEA, 384 parameters, batch 64) ...")
+        a = argparse.Namespace(**vars(args))
+        a.chunk_bits, a.low_qubits = 13, 4
+        c5 = bench_c5(a)
+        res["adjoint"] = {k: c5[k] for k in ("value", "unit", "ms_per_step", "param_shift_gradients_per_s",
+                                             "speedup_vs_param_shift", "max_abs_diff_vs_param_shift",
+                                             "param_shift_sample")}
+        res["adjoint"]["config"] = c5["config"]
+    if args.workload == "c2" and not args.no_cpu_baseline:
+        log("cpu baseline (reference path restated in C, all host threads, 29 qubits) ...")
+        res["cpu_baseline"] = cpu_baseline(args.depth, args.seed, args.cpu_seconds)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -62,6 +62,9 @@ def lib():
             "orc_expect_pauli": (d, [DP, i, U64P, U64P, DP, i]),
             "orc_checksum_weight": (d, [u64]),
             "orc_probability_checksum": (None, [DP, i, DP]),
+            "orc_run_ops_parallel": (i, [DP, i, c, i64, DP, i, i64]),
+            "orc_apply_single_parallel": (None, [DP, i, i, DP, i, i64]),
+            "orc_gate_loop": (None, [DP, i, I32P, DP, i, i, i, i64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -190,3 +193,28 @@ def checksum(state):
     out = np.empty(2)
     lib().orc_probability_checksum(_dp(_f64(state)), n, _dp(out))
     return float(out[0]), float(out[1])
+
+
+def run_ops_parallel(state: np.ndarray, n: int, ops, mats, threads: int, min_work: int = 4096) -> None:
+    """Single-qubit and CNOT ops on `threads` threads (StatevectorParallelExecutor's split), in place."""
+    ops = np.ascontiguousarray(ops)
+    m = np.ascontiguousarray(mats if len(mats) else np.zeros(1), dtype=np.complex128)
+    rc = lib().orc_run_ops_parallel(_dp(state.view(np.float64)), n, ops.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                    _dp(m.view(np.float64)), int(threads), int(min_work))
+    if rc != 0:
+        raise RuntimeError(f"oracle run_ops_parallel failed with status {rc}")
+
+
+def apply_single_parallel(state: np.ndarray, n: int, target: int, g, threads: int, min_work: int = 4096) -> None:
+    gg = np.ascontiguousarray(g, np.complex128).reshape(4)
+    lib().orc_apply_single_parallel(_dp(state.view(np.float64)), n, int(target), _dp(gg.view(np.float64)),
+                                    int(threads), int(min_work))
+
+
+def gate_loop(state: np.ndarray, n: int, targets, gates, iterations: int, threads: int, min_work: int) -> None:
+    """`iterations` passes over single-qubit gates (targets, 4 complex each) on `threads` threads, in place."""
+    t = np.ascontiguousarray(targets, np.int32)
+    g = np.ascontiguousarray(gates, np.complex128).reshape(-1, 4)
+    lib().orc_gate_loop(_dp(state.view(np.float64)), n, t.ctypes.data_as(I32P), _dp(g.view(np.float64)), len(t),
+                        int(iterations), int(threads), int(min_work))
+

@@ -589,7 +589,10 @@ int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *ma
     int n = st->n;
     uint64_t stride = (uint64_t)mat_stride;
     void *args[] = {&amp, &out, &n, &log_chunks, &total, &mats, &stride, &elem_begin, &basis};
-    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)elem_count << st->n));
+    // algorithmic bytes: a read and a write of every amplitude, or the write alone when the pass generates
+    // its basis-state input instead of loading it
+    const double per_amp = basis >= 0 ? 16.0 : 32.0;
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, per_amp * (double)((uint64_t)elem_count << st->n));
     SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, threads, 1, 1, 0, ctx->stream, args, nullptr));
     return SC_OK;
 }

@@ -341,3 +341,21 @@ def test_amplitude_damping_statistics():
         oracle.apply_channel_1q(s, 0, k, [0xBAD0 + trial])
         zero += abs(s[0]) ** 2 > 0.5
     assert abs(zero / 1000 - 0.25) <= 0.05
+
+
+def test_parallel_restatement_equals_serial_oracle():
+    # the timed CPU baseline (oracle/sv_parallel.c, StatevectorParallelExecutor's split) computes exactly what the
+    # serial oracle computes: the same per-pair arithmetic, any thread count and leaf size
+    n = 13
+    ops, mats = lower(workloads.random_layered(n, 6, 11))
+    want = oracle.run_ops(n, ops, mats)
+    for threads, min_work in ((1, 4096), (3, 64), (8, 1)):
+        got = oracle.zero_state(n)
+        oracle.run_ops_parallel(got, n, ops, mats, threads, min_work)
+        assert np.array_equal(got, want), (threads, min_work)
+    s0 = random_state(10, np.random.default_rng(3))
+    g = np.array([1, 1, 1, -1], complex) / math.sqrt(2)
+    a, b = s0.copy(), s0.copy()
+    oracle.gate_loop(a, 10, [3, 0, 9], np.stack([g, g, g]), 2, 1, 1024)
+    oracle.gate_loop(b, 10, [3, 0, 9], np.stack([g, g, g]), 2, 4, 16)
+    assert np.array_equal(a, b)
