@@ -47,6 +47,7 @@ def make_context(args) -> capi.Context:
     ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
     ctx.set_option(capi.OPT_JIT, args.jit)
     ctx.set_option(capi.OPT_SWEEP_PASS_CAP, args.pass_cap)
+    ctx.set_option(capi.OPT_DIRECT_IO, 0 if args.no_direct_io else 1)
     if args.jit:
         ctx.set_option(capi.OPT_CHUNK_BITS, args.chunk_bits)
     return ctx
@@ -506,6 +507,8 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", default="sweep", choices=list(MODES))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c2: skip the adjoint-gradient block of the metric")
+    ap.add_argument("--no-direct-io", action="store_true",
+                    help="SC_OPT_DIRECT_IO 0: every pass goes through LDS at both ends")
     ap.add_argument("--no-copy-reference", action="store_true")
     ap.add_argument("--low-qubits", type=int, default=None,
                     help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")

@@ -108,6 +108,9 @@ typedef struct sc_op {
 #define SC_OPT_CHUNK_BITS 10    /* local qubits per sweep chunk with runtime-compiled passes: 12 (64 KiB of LDS, two
                                    256-thread workgroups per CU) or 13 (128 KiB, one 512-thread workgroup per CU);
                                    default 12.  The compiled sweep kernel always uses 12. */
+#define SC_OPT_DIRECT_IO 11     /* 1: a runtime-compiled pass whose first (last) stage keeps the chunk's low positions
+                                   on lane bits reads HBM straight into (writes it straight from) that stage's
+                                   registers, skipping an LDS round trip; 0: always through LDS; default 1 */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
@@ -255,6 +258,13 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
                         uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, int chunk_bits,
                         uint64_t *passes, uint64_t *stages, uint64_t *fused_blocks, double *fp64_ops_per_amp,
                         double *state_inout);
+/* Host-only (no device needed): per pass of the same plan (at most max_passes entries), its stages, the stage
+ * exchanges that need a workgroup barrier, its FP64 operations per amplitude and its direct-HBM flags (bit 0:
+ * loads into the first stage, bit 1: stores from the last; SC_OPT_DIRECT_IO); n_passes receives the count. */
+int sc_plan_pass_stats_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                            uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits, int max_passes,
+                            int32_t *stages, int32_t *barriers, double *fp64_ops_per_amp, int32_t *direct,
+                            int *n_passes);
 /* Host-only (no device needed): the reverse sweep of an adjoint gradient as sweep passes (the plan the GPU's
  * runtime-compiled adjoint passes run), executed on the host: psi_inout holds the circuit's output state,
  * lam_inout H applied to it (2^n interleaved complex values each, one batch element); on return they hold

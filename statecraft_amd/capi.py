@@ -24,6 +24,7 @@ OP_SINGLE, OP_CNOT, OP_TWO, OP_DIAG2, OP_SWAP, OP_MCU, OP_MEASURE = range(1, 8)
 GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
 (OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP,
  OPT_SWEEP_BLOCKS_PER_CU, OPT_JIT, OPT_SWEEP_PASS_CAP, OPT_CHUNK_BITS) = range(1, 11)
+OPT_DIRECT_IO = 11
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
 KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 
@@ -82,7 +83,7 @@ EXPORTED = [
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
     "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
-    "sc_expectation_pauli_terms_noalloc",
+    "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host",
 ]
 
 _lib = None
@@ -139,6 +140,8 @@ def _sig(lib):
         "sc_last_sample_stats": (i32, [P, U64P]),
         "sc_state_max_deviation": (i32, [P, i32, u64, DP, DP, DP]),
         "sc_expectation_pauli_terms_noalloc": (i32, [P, U64P, U64P, i32, DP]),
+        "sc_plan_pass_stats_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, I32P, I32P, DP, I32P,
+                                          ctypes.POINTER(i32)]),
         "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
         "sc_sample_host": (i32, [i32, DP, I32P, i32, DP, i64, I64P, DP, U64P]),
     }
@@ -198,6 +201,23 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
                                     1 if remap else 0, 1 if jit else 0, int(pass_cap), int(chunk_bits),
                                     ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(f), ptr))
     return {"passes": a.value, "stages": b.value, "fused_blocks": c.value, "fp64_ops_per_amp": f.value}, out
+
+
+def plan_pass_stats_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = False, jit: bool = True,
+                         chunk_bits: int = 12):
+    """Host-only: per sweep pass of a circuit's plan, (stages, barrier exchanges, FP64 operations per amplitude,
+    direct-HBM flags)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+    cap = 4096
+    st, br, fp, dr = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap), np.zeros(cap, np.int32)
+    cnt = ctypes.c_int()
+    check(lib().sc_plan_pass_stats_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
+                                        int(low_qubits), 1 if remap else 0, 1 if jit else 0, int(chunk_bits), cap,
+                                        st.ctypes.data_as(I32P), br.ctypes.data_as(I32P), dptr(fp),
+                                        dr.ctypes.data_as(I32P), ctypes.byref(cnt)))
+    k = min(cnt.value, cap)
+    return [(int(st[i]), int(br[i]), float(fp[i]), int(dr[i])) for i in range(k)]
 
 
 def adjoint_plan_host(n: int, ops, mats, n_params: int, psi, lam, low_qubits: int = 4):

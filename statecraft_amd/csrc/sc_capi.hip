@@ -200,9 +200,10 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
+    cm.direct_io = cm.direct_io && ctx->direct_io;
     const int cb = (ctx->jit && ctx->chunk_bits == kMaxChunkBits && st->n >= kMaxChunkBits) ? kMaxChunkBits : kChunkBits;
-    const std::string key =
-        plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse, ctx->jit, cm.pass_cap, cb);
+    const std::string key = plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
+                                           ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
@@ -501,6 +502,9 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value != kChunkBits && value != kMaxChunkBits) return fail(SC_E_ARG, "chunk bits must be 12 or 13");
         ctx->chunk_bits = (int)value;
         return SC_OK;
+    case SC_OPT_DIRECT_IO:
+        ctx->direct_io = value != 0;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -519,6 +523,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_JIT: *value = ctx->jit; return SC_OK;
     case SC_OPT_SWEEP_PASS_CAP: *value = ctx->pass_cap; return SC_OK;
     case SC_OPT_CHUNK_BITS: *value = ctx->chunk_bits; return SC_OK;
+    case SC_OPT_DIRECT_IO: *value = ctx->direct_io; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -893,6 +898,42 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
         std::vector<double> pool(2 * std::max<uint64_t>(plan.pool_complex, 1));
         build_pool(plan, mats, pool.data());
         execute_plan_host(num_qubits, plan, pool.data(), state_inout);
+    }
+    return SC_OK;
+}
+
+int sc_plan_pass_stats_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                            uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits, int max_passes,
+                            int32_t *stages, int32_t *barriers, double *fp64_ops_per_amp, int32_t *direct,
+                            int *n_passes) {
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    SC_REQUIRE(n_passes, "n_passes");
+    if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
+    if (chunk_bits != kChunkBits && !(chunk_bits == kMaxChunkBits && jit))
+        return fail(SC_E_ARG, "chunk bits must be 12, or 13 with runtime-compiled passes");
+    if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
+    if (rc) return rc;
+    SweepPlan plan;
+    if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits,
+                          jit ? kCompiledPassCost : kInterpretedCost, chunk_bits)))
+        return rc;
+    *n_passes = (int)plan.passes.size();
+    for (int p = 0; p < (int)plan.passes.size() && p < max_passes; p++) {
+        const SweepPass &ps = plan.passes[p];
+        int nb = 0;
+        double f = 0.0;
+        for (int st = 0; st < ps.nstages; st++) {
+            nb += ps.stages[st].wave_local ? 0 : 1;
+            for (int o = ps.stages[st].op_begin; o < ps.stages[st].op_end; o++) f += sweep_op_fp64(plan.ops[o]);
+        }
+        if (stages) stages[p] = ps.nstages;
+        if (barriers) barriers[p] = nb;
+        if (fp64_ops_per_amp) fp64_ops_per_amp[p] = f;
+        if (direct) direct[p] = ps.direct;
     }
     return SC_OK;
 }

@@ -65,7 +65,8 @@ struct Context {
     int sweep_blocks_per_cu = 0;
     int jit = 0;
     int pass_cap = -1;
-    int chunk_bits = 12;  // SC_OPT_CHUNK_BITS  // SC_OPT_SWEEP_PASS_CAP
+    int chunk_bits = 12;  // SC_OPT_CHUNK_BITS
+    int direct_io = 1;    // SC_OPT_DIRECT_IO
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)
@@ -160,6 +161,17 @@ struct SweepPass {
     // LDS slot of the local qubit the pass moves there.  Equal to io_* when the pass does not remap qubits.
     uint32_t st_thread_slot[kMaxThreadBits];
     uint32_t st_reg_slot[kRegs];
+    // Runtime-compiled passes only.  bit 0: the HBM loads go straight into the first stage's registers (its
+    // layout keeps the chunk's low positions on lane bits 0-3, so the loads stay coalesced) and the load
+    // phase's LDS round trip is skipped; bit 1: the last stage's registers go straight to HBM and the store
+    // phase's round trip is skipped.  Global offsets of thread-index bit i / register index r in the first
+    // stage's layout (ld_*) and in the last stage's layout (sd_*).
+    int32_t direct;
+    int32_t pad2;
+    uint64_t ld_thread[kMaxThreadBits];
+    uint64_t ld_reg[kRegs];
+    uint64_t sd_thread[kMaxThreadBits];
+    uint64_t sd_reg[kRegs];
     SweepStage stages[kMaxStages];
 };
 

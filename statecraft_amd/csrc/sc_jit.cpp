@@ -129,18 +129,24 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
 // (sc_sweep.hip) with every table read replaced by its value.  With `prefetch` the HBM loads of a
 // workgroup's next chunk are issued before the stages of the current one, into a third register set, so
 // that they arrive while the chunk is computed; otherwise they are issued after the stages, as in k_sweep.
+// A pass with direct loads (SweepPass::direct bit 0) reads HBM straight into its first stage's registers,
+// one with direct stores (bit 1) writes its last stage's registers straight to HBM; the LDS round trips of
+// the load and store phases are then skipped.
 std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch) {
     std::ostringstream s;
     const int TB = plan.chunk_bits - kRegBits;  // thread-index bits: 8 (256 threads) or 9 (512)
+    const int S = ps.nstages;
+    const bool DL = S > 0 && (ps.direct & 1) != 0, DS = S > 0 && (ps.direct & 2) != 0;
     s << "extern \"C\" __global__ void __launch_bounds__(" << (1 << TB)
       << ") __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
          // amp and out may be the same buffer: every chunk is read once and written once, by one workgroup,
          // so no access of one pointer can observe an access of the other
          "@K@(sc::v2d *__restrict__ amp, sc::v2d *__restrict__ out, int n, int log_chunks, uint64_t total_chunks,\n"
          "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin, long long basis) {\n"
-         "  using namespace sc;\n"
-         "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n"
-         "  CV2 *mats = (CV2 *)mats_arg;\n"
+         "  using namespace sc;\n";
+    const bool uses_lds = !(DL && DS && S == 1);
+    if (uses_lds) s << "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n";
+    s << "  CV2 *mats = (CV2 *)mats_arg;\n"
          "  const uint32_t tid = threadIdx.x;\n"
          // with one matrix pool per element (mat_stride != 0) a workgroup takes a contiguous range of chunks,
          // so that consecutive chunks share their element's matrices in the scalar cache; otherwise chunks
@@ -153,23 +159,30 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
          "  auto cbase = [](uint64_t x) -> uint64_t { return "
       << deposit_expr("x", ps.outer_mask) << "; };\n";
-    s << "  const uint64_t gthread = " << lane_expr("tid", ps.glob_thread, TB, "+", true) << ";\n";
+    // global offsets of the load side (first stage's layout when direct, else the load phase's) and of the
+    // store side (last stage's layout when direct, else the store phase's)
+    const uint64_t *l_thread = DL ? ps.ld_thread : ps.glob_thread, *l_reg = DL ? ps.ld_reg : ps.glob_reg;
+    const uint64_t *s_thread = DS ? ps.sd_thread : ps.glob_thread, *s_reg = DS ? ps.sd_reg : ps.glob_reg;
+    s << "  const uint64_t gload = " << lane_expr("tid", l_thread, TB, "+", true) << ";\n";
+    s << "  const uint64_t gstore = " << lane_expr("tid", s_thread, TB, "+", true) << ";\n";
     // LDS addresses are byte offsets (slot * 16), so that each access costs one XOR
-    s << "  char *const ldsb = (char *)lds;\n";
-    s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, TB, "^", false, 16) << ";\n";
-    s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, TB, "^", false, 16) << ";\n";
+    if (uses_lds) {
+        s << "  char *const ldsb = (char *)lds;\n";
+        s << "  const uint32_t sthread = " << lane_expr("tid", ps.io_thread_slot, TB, "^", false, 16) << ";\n";
+        s << "  const uint32_t sout = " << lane_expr("tid", ps.st_thread_slot, TB, "^", false, 16) << ";\n";
+    }
     auto regs_load = [&](const char *dst, const char *srcp) {
-        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << ps.glob_reg[r] << "];\n";
+        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << srcp << "[" << l_reg[r] << "];\n";
     };
     // basis >= 0: the input of every element is the basis state |basis>, generated instead of loaded
     auto regs_basis = [&](const char *dst, const char *chunk_base) {
-        s << "    const uint64_t i0 = " << chunk_base << " + gthread;\n";
+        s << "    const uint64_t i0 = " << chunk_base << " + gload;\n";
         for (int r = 0; r < kRegs; r++)
-            s << "    " << dst << "[" << r << "] = (i0 + " << ps.glob_reg[r]
+            s << "    " << dst << "[" << r << "] = (i0 + " << l_reg[r]
               << "ULL == (uint64_t)basis) ? make_v2d(1.0, 0.0) : make_v2d(0.0, 0.0);\n";
     };
     auto regs_store = [&](const char *dstp, const char *src) {
-        for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << ps.glob_reg[r] << "] = " << src << "[" << r << "];\n";
+        for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << s_reg[r] << "] = " << src << "[" << r << "];\n";
     };
     auto lds_write = [&](const char *base, const uint32_t *off, const char *src) {
         for (int r = 0; r < kRegs; r++)
@@ -179,22 +192,21 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         for (int r = 0; r < kRegs; r++)
             s << "    " << dst << "[" << r << "] = *(const v2d *)(ldsb + (" << base << " ^ " << off[r] * 16u << "u));\n";
     };
+    auto copy_regs = [&](const char *dst, const char *src) {
+        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = " << src << "[" << r << "];\n";
+    };
     // Exchanges in which every wavefront reads back only the LDS slots it wrote need no workgroup barrier.
-    const int S = ps.nstages;
     auto sync = [&](bool wave_local) { s << (wave_local ? "    wave_sync();\n" : "    __syncthreads();\n"); };
-    const bool into_first = S > 0 ? ps.stages[0].wave_local != 0 : (ps.wave_local & 1) != 0;
+    // the chunk's input: one register set; the prefetching variant loads the next chunk into a third one
+    const char *in0 = prefetch ? "q" : "p";
     s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
     s << "  uint64_t cb = cbase(c & chunk_mask);\n"
          "  if (basis >= 0) {\n";
-    regs_basis("p", "cb");
+    regs_basis(in0, "cb");
     s << "  } else {\n"
-         "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gthread);\n";
-    regs_load("p", "src");
+         "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gload);\n";
+    regs_load(in0, "src");
     s << "  }\n";
-    s << "  {\n";
-    lds_write("sthread", ps.io_reg_slot, "p");
-    s << "  }\n";
-    sync(into_first);
     s << "  for (;;) {\n"
          "    const uint64_t elem = (uint64_t)elem_begin + (c >> log_chunks);\n"
          "    CV2 *emats = mats + elem * mat_stride;\n"
@@ -204,46 +216,81 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         s << "    if (next < c_end && basis >= 0) {\n";
         regs_basis(dst, "ncb");
         s << "    } else if (next < c_end) {\n"
-             "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gthread);\n";
+             "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gload);\n";
         regs_load(dst, "src");
         s << "    }\n";
     };
-    if (prefetch) next_loads("q");
-    for (int st = 0; st < ps.nstages; st++) {
+    // The first LDS write of a chunk follows the previous chunk's last LDS read: a barrier unless both use the
+    // same wave-index positions (SweepPass::wave_local bit 1).
+    const bool war_local = (ps.wave_local & 2) != 0;
+    bool first_write_done = false;
+    auto before_first_write = [&]() {
+        if (!first_write_done) sync(war_local);
+        first_write_done = true;
+    };
+    const char *cur = nullptr;  // register set holding the current values
+    if (!DL) {
+        before_first_write();
+        s << "    {\n";
+        lds_write("sthread", ps.io_reg_slot, in0);
+        s << "    }\n";
+        if (prefetch) next_loads("q");
+        sync(S > 0 ? ps.stages[0].wave_local != 0 : (ps.wave_local & 1) != 0);
+    } else {
+        if (prefetch) {
+            s << "    {\n";
+            copy_regs("p", "q");
+            s << "    }\n";
+            next_loads("q");
+        }
+        cur = "p";
+    }
+    for (int st = 0; st < S; st++) {
         const SweepStage &stg = ps.stages[st];
         s << "    {\n"
-             "    const uint32_t t = opaque(tid);\n"
-             "    const uint32_t ts = "
-          << lane_expr("t", stg.thread_slot, TB, "^", false, 16) << ";\n";
-        lds_read("a", "ts", stg.lds_off);
-        bool in_a = true;
+             "    const uint32_t t = opaque(tid);\n";
+        const bool from_lds = !(st == 0 && DL);
+        const bool to_lds = !(st == S - 1 && DS);
+        if (from_lds || to_lds)
+            s << "    const uint32_t ts = " << lane_expr("t", stg.thread_slot, TB, "^", false, 16) << ";\n";
+        bool in_a;
+        if (from_lds) {
+            lds_read("a", "ts", stg.lds_off);
+            in_a = true;
+        } else {
+            in_a = std::string(cur) == "a";
+        }
         for (int o = stg.op_begin; o < stg.op_end; o++) {
             s << "    " << op_call(plan.ops[o], in_a ? "a" : "p", in_a ? "p" : "a") << "\n";
             in_a = !in_a;
         }
-        s << "    const uint32_t tw = opaque(ts);\n";
-        lds_write("tw", stg.lds_off, in_a ? "a" : "p");
-        sync(st + 1 < S ? ps.stages[st + 1].wave_local != 0 : (ps.wave_local & 1) != 0);
+        cur = in_a ? "a" : "p";
+        if (to_lds) {
+            if (st == 0 && DL) before_first_write();
+            s << "    const uint32_t tw = opaque(ts);\n";
+            lds_write("tw", stg.lds_off, cur);
+            sync(st + 1 < S ? ps.stages[st + 1].wave_local != 0 : (ps.wave_local & 1) != 0);
+        }
         s << "    }\n";
     }
+    if (!DS) {
+        s << "    {\n"
+             "    const uint32_t sl = opaque(sout);\n";
+        lds_read("a", "sl", ps.st_reg_slot);
+        s << "    }\n";
+        cur = "a";
+    }
+    const std::string fin = cur ? cur : "a";
+    // without prefetch the next chunk's loads go into p: after the store when p holds the results
+    if (!prefetch && fin != "p") next_loads("p");
     s << "    {\n"
-         "    const uint32_t sl = opaque(sout);\n";
-    lds_read("a", "sl", ps.st_reg_slot);
+         "    v2d *dst = out + (elem << n) + cb + opaque(gstore);\n";
+    regs_store("dst", fin.c_str());
     s << "    }\n";
-    if (!prefetch) next_loads("p");
-    s << "    {\n"
-         "    v2d *dst = out + (elem << n) + cb + opaque(gthread);\n";
-    regs_store("dst", "a");
-    s << "    }\n"
-         "    if (next >= c_end) break;\n"
+    if (!prefetch && fin == "p") next_loads("p");
+    s << "    if (next >= c_end) break;\n"
          "    c = next;\n"
          "    cb = ncb;\n";
-    sync((ps.wave_local & 2) != 0);  // every wavefront has read its store-phase slots
-    s << "    {\n"
-         "    const uint32_t sl = opaque(sthread);\n";
-    lds_write("sl", ps.io_reg_slot, prefetch ? "q" : "p");
-    s << "    }\n";
-    sync(into_first);
     s << "  }\n"
          "}\n";
     return s.str();

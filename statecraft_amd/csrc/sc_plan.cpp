@@ -603,6 +603,8 @@ bool register_internal(const SweepOp &op) {
 
 }  // namespace
 
+double sweep_op_fp64(const SweepOp &op) { return op_fp64(op); }
+
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
                 int max_fuse_bits, const SweepCostModel &cm, int chunk_bits) {
     plan = SweepPlan();
@@ -918,6 +920,20 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         // the exchange needs no workgroup barrier.  A dynamic programme over the stages picks the ordered
         // pair (a, b) per stage (both outside the stage's register positions) with the fewest changes,
         // starting from the load phase's pair (6, 7) and ending at the store phase's (pi[6], pi[7]).
+        // ---- direct HBM access at the pass's ends (runtime-compiled passes): when the chunk's four lowest
+        // local positions are physical positions 0-3 and a first / last stage keeps them off its register
+        // bits (and its wave-index bits), its layout puts them on lane bits 0-3, and HBM can be read into or
+        // written from that stage's registers with 256-byte runs per 16 lanes: no LDS round trip there.
+        bool direct_load = false, direct_store = false;
+        {
+            const int S = (int)stages.size();
+            bool low_ok = cm.direct_io && S > 0 && CB >= 4;
+            for (int k = 0; k < 4 && low_ok; k++) low_ok = lq[k] == k;
+            bool pi_identity = true;
+            for (int k = 0; k < CB; k++) pi_identity = pi_identity && pi[k] == k;
+            direct_load = low_ok && (stage_rp[0] & 0xFu) == 0;
+            direct_store = low_ok && pi_identity && (stage_rp[S - 1] & 0xFu) == 0;
+        }
         {
             const int S = (int)stages.size();
             const int kPairs = CB * CB;
@@ -925,11 +941,14 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             std::vector<std::vector<int>> cost(S, std::vector<int>(kPairs, INF)), from(S, std::vector<int>(kPairs, -1));
             auto ok = [&](int s, int w) {
                 const int a = w / CB, b = w % CB;
-                return a != b && !((stage_rp[s] >> a) & 1u) && !((stage_rp[s] >> b) & 1u);
+                if (a == b || ((stage_rp[s] >> a) & 1u) || ((stage_rp[s] >> b) & 1u)) return false;
+                // a stage that touches HBM directly keeps the low positions on lane bits
+                if (((s == 0 && direct_load) || (s == S - 1 && direct_store)) && (a < 4 || b < 4)) return false;
+                return true;
             };
             const int w_load = 6 * CB + 7, w_store = pi[6] * CB + pi[7];
             for (int w = 0; w < kPairs; w++)
-                if (S > 0 && ok(0, w)) cost[0][w] = (w != w_load);
+                if (S > 0 && ok(0, w)) cost[0][w] = direct_load ? 0 : (w != w_load);
             for (int s = 1; s < S; s++)
                 for (int w = 0; w < kPairs; w++) {
                     if (!ok(s, w)) continue;
@@ -945,10 +964,10 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             std::vector<int> wsel(S, -1);
             if (S > 0) {
                 int best = -1;
+                auto end_cost = [&](int w) { return cost[S - 1][w] + (direct_store ? 0 : (w != w_store)); };
                 for (int w = 0; w < kPairs; w++) {
                     if (cost[S - 1][w] >= INF) continue;
-                    const int c = cost[S - 1][w] + (w != w_store);
-                    if (best < 0 || c < cost[S - 1][best] + (best != w_store)) best = w;
+                    if (best < 0 || end_cost(w) < end_cost(best)) best = w;
                 }
                 for (int s = S - 1; s >= 0; s--) {
                     wsel[s] = best;
@@ -976,7 +995,11 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             pass_wave_local = 0;
             if (TB == 8) {
                 if (S > 0 ? wsel[S - 1] == w_store : w_load == w_store) pass_wave_local |= 1;
-                if (w_store == w_load) pass_wave_local |= 2;
+                // a chunk's last LDS read and the next chunk's first LDS write use the same wave-index
+                // positions (or one of them does not exist): no wavefront overwrites slots another still reads
+                const int first_w = !direct_load ? w_load : (S > 1 || !direct_store) ? wsel[0] : -1;
+                const int last_r = !direct_store ? w_store : (S > 1 || !direct_load) ? wsel[S - 1] : -1;
+                if (first_w < 0 || last_r < 0 || first_w == last_r) pass_wave_local |= 2;
             }
             pass.wave_local = pass_wave_local;
         }
@@ -1037,6 +1060,19 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             pass.glob_reg[r] = g;
             pass.io_reg_slot[r] = slot_of(l);
         }
+        // direct loads / stores: global offsets of the first / last stage's thread and register bits
+        pass.direct = (direct_load ? 1 : 0) | (direct_store ? 2 : 0);
+        auto stage_offsets = [&](const StageLayout &lay, uint64_t *th, uint64_t *rg) {
+            for (int i = 0; i < TB; i++) th[i] = 1ULL << lq[lay.tpos[i]];
+            for (int r = 0; r < kRegs; r++) {
+                uint64_t g = 0;
+                for (int j = 0; j < kRegBits; j++)
+                    if ((r >> j) & 1) g |= 1ULL << lq[lay.rpos[j]];
+                rg[r] = g;
+            }
+        };
+        if (direct_load) stage_offsets(layouts[0], pass.ld_thread, pass.ld_reg);
+        if (direct_store) stage_offsets(layouts[stages.size() - 1], pass.sd_thread, pass.sd_reg);
         // store phase: chunk position k takes the value of local position pi[k]
         for (int i = 0; i < TB; i++) pass.st_thread_slot[i] = slot_of(1u << pi[i]);
         for (int r = 0; r < kRegs; r++) {
@@ -1495,7 +1531,17 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                 c >>= 1;
                 om &= om - 1ULL;
             }
-            for (int k = 0; k < ns; k++)
+            const bool dl = (pass.direct & 1) != 0, ds = (pass.direct & 2) != 0;
+            const int S = pass.nstages;
+            // a direct first stage reads from HBM and a direct last stage writes to HBM: each chunk reads and
+            // writes only its own addresses, so the in-place update below is safe
+            auto direct_offset = [&](const uint64_t *th, uint32_t tid) {
+                uint64_t g = 0;
+                for (int i = 0; i < TB; i++)
+                    if ((tid >> i) & 1u) g += th[i];
+                return g;
+            };
+            for (int k = 0; k < ns && !dl; k++)
                 for (uint32_t tid = 0; tid < threads; tid++) {
                     uint64_t g = 0;
                     uint32_t s = 0;
@@ -1508,7 +1554,10 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                         lds[k][2 * sl + 1] = states[k][2 * gi + 1];
                     }
                 }
-            for (int st = 0; st < pass.nstages; st++) {
+            // a stage's reads of every thread precede its writes (the kernel's barrier): buffer the writes of
+            // a direct last stage until the stage is done
+            std::vector<std::pair<uint64_t, double>> pending;
+            for (int st = 0; st < S; st++) {
                 const SweepStage &stg = pass.stages[st];
                 for (uint32_t tid = 0; tid < threads; tid++) {
                     uint32_t ts = 0;
@@ -1516,6 +1565,11 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                         if ((tid >> i) & 1u) ts ^= stg.thread_slot[i];
                     for (int k = 0; k < ns; k++)
                         for (int r = 0; r < kRegs; r++) {
+                            if (st == 0 && dl) {
+                                const uint64_t gi = chunk_base + direct_offset(pass.ld_thread, tid) + pass.ld_reg[r];
+                                a[k][2 * r] = states[k][2 * gi], a[k][2 * r + 1] = states[k][2 * gi + 1];
+                                continue;
+                            }
                             uint32_t sl = ts ^ stg.lds_off[r];
                             a[k][2 * r] = lds[k][2 * sl], a[k][2 * r + 1] = lds[k][2 * sl + 1];
                         }
@@ -1529,12 +1583,19 @@ void execute_plan_host(int n, const SweepPlan &plan, const double *pool, double 
                     }
                     for (int k = 0; k < ns; k++)
                         for (int r = 0; r < kRegs; r++) {
+                            if (st == S - 1 && ds) {
+                                const uint64_t gi = chunk_base + direct_offset(pass.sd_thread, tid) + pass.sd_reg[r];
+                                pending.push_back({2 * gi, a[k][2 * r]});
+                                pending.push_back({2 * gi + 1, a[k][2 * r + 1]});
+                                continue;
+                            }
                             uint32_t sl = ts ^ stg.lds_off[r];
                             lds[k][2 * sl] = a[k][2 * r], lds[k][2 * sl + 1] = a[k][2 * r + 1];
                         }
                 }
             }
-            for (int k = 0; k < ns; k++)
+            for (const auto &pw : pending) states[0][pw.first] = pw.second;
+            for (int k = 0; k < ns && !ds; k++)
                 for (uint32_t tid = 0; tid < threads; tid++) {
                     uint64_t g = 0;
                     uint32_t s = 0;

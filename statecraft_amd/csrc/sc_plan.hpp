@@ -101,19 +101,24 @@ struct SweepPlan {
 // prod(pivots)^2 of every probe of a plan for one matrix array: (slot, factor) pairs.
 void probe_factors(const SweepPlan &plan, const double *mats, std::vector<std::pair<uint64_t, double>> &out);
 
+// FP64 operations per amplitude of one register op (an FMA or a MUL counts one).
+double sweep_op_fp64(const SweepOp &op);
+
 // Cost model of the planner's fusion choices, in FP64 operations per amplitude.
 struct SweepCostModel {
     double op_overhead;  // fixed cost of every register op
     bool const_ops;      // ops are compile-time constants (runtime-compiled pass kernels)
     double pass_cap;     // most estimated FP64 operations per amplitude a pass takes on (0: no limit)
     bool unit_1q;        // single-qubit ops with a unit-pivot form use it (SW_U1U) and fold their pivots
+    bool direct_io;      // passes may load into their first stage's and store from their last stage's registers
 };
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
-constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false};
-constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true};
+constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false, false};
+constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true, true};
 // Adjoint passes: as the runtime-compiled passes; a pass with gradient probes applies its unit-pivot factor
-// by a scaling op at its end, and each probe's sum is corrected on the host (SweepPlan::probe_pivots).
-constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true};
+// by a scaling op at its end, and each probe's sum is corrected on the host (SweepPlan::probe_pivots).  Their
+// kernels always go through LDS at both ends.
+constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true, false};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).

@@ -298,3 +298,23 @@ def test_thirteen_qubit_chunks_need_runtime_compiled_passes():
     ops, mats = random_circuit(14, 10, np.random.default_rng(2))
     with pytest.raises(capi.IllegalArgumentException):
         capi.plan_sweeps_host(14, ops, mats, 4, None, False, False, chunk_bits=13)
+
+
+@pytest.mark.parametrize("n,chunk_bits", [(12, 12), (14, 12), (16, 12), (14, 13), (16, 13)])
+@pytest.mark.parametrize("family", ["random", "layered", "grover"])
+def test_direct_hbm_passes_match_oracle(n, chunk_bits, family):
+    # runtime-compiled plans load straight into a pass's first stage / store from its last stage when their
+    # layouts keep the chunk's low positions on lane bits; the host executor replays exactly that movement
+    rng = np.random.default_rng(7 * n + chunk_bits)
+    if family == "random":
+        ops, mats = random_circuit(n, 300, rng)
+    elif family == "layered":
+        ops, mats = lower(workloads.random_layered(n, 8, n))
+    else:
+        ops, mats = lower(workloads.grover(n, 2, n)[0])
+    init = random_state(n, rng)
+    per_pass = capi.plan_pass_stats_host(n, ops, mats, 4, False, True, chunk_bits)
+    _, out = capi.plan_sweeps_host(n, ops, mats, 4, init, False, True, -1, chunk_bits)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+    if family != "random" and n > chunk_bits:
+        assert any(d & 1 for *_, d in per_pass) and any(d & 2 for *_, d in per_pass), per_pass
