@@ -230,6 +230,31 @@ int sc_expectation_pauli_terms_noalloc(sc_state *st, const uint64_t *xmask, cons
 int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
                         int mats_per_element, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
                         int n_terms, int n_params, int checkpoint_every, double *grads, double *energies);
+/* ---- batched jobs in one graph launch (no reference counterpart) ---------- */
+/* One job: every element of `state` starts from the basis state |basis>, runs the circuit (ops; element b's
+ * matrices at mats + 2 * b * mat_stride doubles, as sc_apply_ops_batched), and out[b * n_terms + t] receives
+ * <psi_b|P_t|psi_b> (as sc_expectation_pauli_terms). */
+typedef struct sc_job {
+    sc_state *state;
+    uint64_t basis;
+    const sc_op *ops;
+    uint64_t n_ops;
+    const double *mats;
+    uint64_t mat_stride;
+    const uint64_t *xmask;
+    const uint64_t *zmask;
+    int32_t n_terms;
+    int32_t reserved;
+    double *out;
+} sc_job;
+/* Runs the jobs in order with results identical to the separate calls.  With runtime-compiled sweep passes
+ * (SC_OPT_JIT > 0, every state of at least 12 qubits) the first call of a job structure (shapes, circuit
+ * structure and matrix classes, bases, terms) records pool uploads, every pass launch, the term kernels and
+ * the result copies as one HIP graph; every later call rebuilds only the matrix pools and launches that graph
+ * once (*graph_launched = 1).  Otherwise the jobs run one after another (*graph_launched = 0).  The graph
+ * keeps the states' device addresses: destroying a state drops nothing but makes its graph unusable, and a
+ * new graph is recorded for new states.  The context keeps its 8 most recently used graphs. */
+int sc_run_jobs(sc_ctx *ctx, const sc_job *jobs, int n_jobs, int *graph_launched);
 /* Statistics of the last sc_adjoint_gradient on this context: re-anchors performed, max drift seen. */
 int sc_adjoint_stats(sc_ctx *ctx, uint64_t *reanchors, double *max_drift);
 

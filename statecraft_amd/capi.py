@@ -83,7 +83,7 @@ EXPORTED = [
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
     "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
-    "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host",
+    "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host", "sc_run_jobs",
 ]
 
 _lib = None
@@ -140,6 +140,7 @@ def _sig(lib):
         "sc_last_sample_stats": (i32, [P, U64P]),
         "sc_state_max_deviation": (i32, [P, i32, u64, DP, DP, DP]),
         "sc_expectation_pauli_terms_noalloc": (i32, [P, U64P, U64P, i32, DP]),
+        "sc_run_jobs": (i32, [P, P, i32, ctypes.POINTER(i32)]),
         "sc_plan_pass_stats_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, I32P, I32P, DP, I32P,
                                           ctypes.POINTER(i32)]),
         "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
@@ -243,6 +244,35 @@ def jit_compile_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = True,
                                     ctypes.byref(sec),
                                     ctypes.byref(size), dump_dir.encode() if dump_dir else None))
     return {"kernels": k.value, "seconds": sec.value, "code_bytes": size.value}
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("state", P), ("basis", ctypes.c_uint64), ("ops", P), ("n_ops", ctypes.c_uint64), ("mats", DP),
+                ("mat_stride", ctypes.c_uint64), ("xmask", U64P), ("zmask", U64P), ("n_terms", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("out", DP)]
+
+
+def run_jobs(ctx: "Context", jobs):
+    """sc_run_jobs: each job (state, basis, ops, per-element matrices [batch, m], xmask, zmask) runs its circuit on
+    every element from |basis> and yields the per-term expectations [batch, n_terms].  Returns (outs, whether
+    one graph launch ran them)."""
+    keep, arr = [], (_Job * len(jobs))()
+    outs = []
+    for k, (st, basis, ops, mats, xm, zm) in enumerate(jobs):
+        o = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = np.ascontiguousarray(mats, dtype=np.complex128)
+        assert m.ndim == 2 and m.shape[0] == st.batch
+        mv = m.view(np.float64)
+        x = np.ascontiguousarray(xm, np.uint64)
+        z = np.ascontiguousarray(zm, np.uint64)
+        out = np.empty((st.batch, len(z)))
+        keep += [o, m, x, z]
+        outs.append(out)
+        arr[k] = _Job(st.h, int(basis), o.ctypes.data_as(P), len(o), dptr(mv), m.shape[1], x.ctypes.data_as(U64P),
+                      z.ctypes.data_as(U64P), len(z), 0, dptr(out))
+    used = ctypes.c_int()
+    check(lib().sc_run_jobs(ctx.h, ctypes.cast(arr, P), len(jobs), ctypes.byref(used)))
+    return outs, bool(used.value)
 
 
 SAMPLE_STATS = ("fast_segments", "serial_segments", "zero_segments", "gather_rounds", "rescaled")

@@ -373,6 +373,8 @@ int finish_parts(Context *ctx, double *part, uint64_t count, double *host) {
     return SC_OK;
 }
 
+}  // namespace
+
 // Partial sums per term row for k_pauli_terms: enough workgroups to fill the GPU, at least 8 amplitudes per
 // thread, at most 32 Mi partials; a power of two (the kernel's index split).
 uint64_t term_parts(const Context *ctx, uint64_t B, int n, uint64_t rows) {
@@ -392,8 +394,8 @@ size_t term_scratch_bytes(uint64_t B, uint64_t rows, uint64_t parts) { return (s
 // evaluates up to kTermsPerLaunch terms of one group in one read of the state, and one synchronisation ends
 // the call.  The partial sums live in `scratch` (scratch_bytes of device memory); fewer partials per row are
 // used when the default count does not fit, and SC_E_ARG is returned when even one per row does not.
-int pauli_terms_in(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
-                   double *vals, void *scratch, size_t scratch_bytes) {
+int pauli_terms_enqueue(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm,
+                        int n_terms, void *scratch, size_t scratch_bytes, double *out) {
     const uint64_t B = (uint64_t)st->batch;
     const int n = st->n;
     const uint64_t rows = (uint64_t)n_terms;
@@ -403,9 +405,8 @@ int pauli_terms_in(Context *ctx, State *st, const double2 *amp, const uint64_t *
         return fail(SC_E_ARG, "expectation of " + std::to_string(n_terms) + " terms over " + std::to_string(B) +
                                   " states needs " + std::to_string(term_scratch_bytes(B, rows, 1)) +
                                   " bytes of scratch, more than the " + std::to_string(scratch_bytes) + " available");
-    const size_t part_bytes = (size_t)(B * rows * parts) * sizeof(double);
     double *part = (double *)scratch;
-    double *out = (double *)((char *)scratch + part_bytes);
+    if (!out) out = (double *)((char *)scratch + (size_t)(B * rows * parts) * sizeof(double));
     for (int t0 = 0; t0 < n_terms; t0++) {
         bool seen = false;  // x mask of an earlier term: its group was done there
         for (int t = 0; t < t0 && !seen; t++) seen = xm[t] == xm[t0];
@@ -455,13 +456,32 @@ int pauli_terms_in(Context *ctx, State *st, const double2 *amp, const uint64_t *
     hipLaunchKernelGGL(k_rows_final, dim3((unsigned)((total_rows + 255) / 256)), dim3(256), 0, ctx->stream, part,
                        total_rows, (int)parts, out);
     SC_HIP_CHECK(hipGetLastError());
-    SC_HIP_CHECK(hipMemcpyAsync(vals, out, total_rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+// Host side of pauli_terms_enqueue's rows: the factor 2 of x-paired terms and the sign of i^{|x & z|}.
+void pauli_terms_finish(const uint64_t *xm, const uint64_t *zm, int n_terms, uint64_t B, double *vals) {
     for (int t = 0; t < n_terms; t++) {
         const int ny = __builtin_popcountll(xm[t] & zm[t]) & 3;
         const double f = (xm[t] != 0 ? 2.0 : 1.0) * ((ny == 1 || ny == 2) ? -1.0 : 1.0);
-        for (uint64_t b = 0; b < B; b++) vals[b * rows + t] *= f;
+        for (uint64_t b = 0; b < B; b++) vals[b * (uint64_t)n_terms + t] *= f;
     }
+}
+
+namespace {
+
+// Enqueue, copy the rows to the host, synchronise, finish.
+int pauli_terms_in(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm, int n_terms,
+                   double *vals, void *scratch, size_t scratch_bytes) {
+    const uint64_t B = (uint64_t)st->batch, rows = (uint64_t)n_terms;
+    uint64_t parts = term_parts(ctx, B, st->n, rows);
+    while (parts > 1 && term_scratch_bytes(B, rows, parts) > scratch_bytes) parts >>= 1;
+    double *out = (double *)((char *)scratch + (size_t)(B * rows * parts) * sizeof(double));
+    int rc = pauli_terms_enqueue(ctx, st, amp, xm, zm, n_terms, scratch, scratch_bytes, out);
+    if (rc) return rc;
+    SC_HIP_CHECK(hipMemcpyAsync(vals, out, B * rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    pauli_terms_finish(xm, zm, n_terms, B, vals);
     return SC_OK;
 }
 

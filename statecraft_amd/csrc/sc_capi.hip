@@ -190,30 +190,44 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
     return SC_OK;
 }
 
-// mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
-// src / dst (runtime-compiled passes only): the first pass reads the states from src and the last one
-// writes them to dst; passes in between work in st->amp.  nullptr means st->amp.
-int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems,
-               const double2 *src = nullptr, double2 *dst = nullptr) {
+}  // namespace
+
+int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCache::Entry **out) {
     // ---- the plan (and its runtime-compiled kernels) of this circuit structure, cached per context
     const int max_fuse = std::min(ctx->max_fused, kRegBits);
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
     cm.direct_io = cm.direct_io && ctx->direct_io;
-    const int cb = (ctx->jit && ctx->chunk_bits == kMaxChunkBits && st->n >= kMaxChunkBits) ? kMaxChunkBits : kChunkBits;
-    const std::string key = plan_cache_key(st->n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
+    const int cb = (ctx->jit && ctx->chunk_bits == kMaxChunkBits && n >= kMaxChunkBits) ? kMaxChunkBits : kChunkBits;
+    const std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
                                            ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
         e->key = key;
-        int rc = plan_sweeps(st->n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm, cb);
+        int rc = plan_sweeps(n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm, cb);
         if (rc) return rc;
         if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
         ent = ctx->plan_cache->insert(std::move(e));
     }
     ent->used = ++ctx->plan_cache->tick;
+    *out = ent;
+    return SC_OK;
+}
+
+namespace {
+
+// mats_per_elem: one matrix array per batch element (size batch) or a single shared array.
+// src / dst (runtime-compiled passes only): the first pass reads the states from src and the last one
+// writes them to dst; passes in between work in st->amp.  nullptr means st->amp.
+int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems,
+               const double2 *src = nullptr, double2 *dst = nullptr) {
+    PlanCache::Entry *ent = nullptr;
+    {
+        int rc = sweep_plan_entry(ctx, st->n, ops, &ent);
+        if (rc) return rc;
+    }
     const SweepPlan &plan = ent->plan;
     const std::vector<hipFunction_t> &fns = ent->fns;
     if (!src) src = st->amp;
@@ -452,6 +466,7 @@ int sc_ctx_destroy(sc_ctx *ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->scratch_host) (void)hipHostFree(ctx->scratch_host);
     if (ctx->plan_dev) (void)hipFree(ctx->plan_dev);
+    destroy_job_graphs(ctx);
     delete ctx->plan_cache;
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -53,7 +53,8 @@ struct ProfScope {
     ~ProfScope();
 };
 
-struct PlanCache;  // sc_plan.hpp
+struct PlanCache;      // sc_plan.hpp
+struct JobGraphCache;  // sc_graph.hip
 
 struct Context {
     int device = 0;
@@ -82,6 +83,7 @@ struct Context {
     uint64_t last_sweeps = 0, last_launches = 0, last_blocks = 0;
     // exact-sum statistics of the last sc_sample: fast / serial / zero segments, gather rounds, rescaled
     uint64_t last_sample[5] = {};
+    JobGraphCache *job_graphs = nullptr;  // recorded sc_run_jobs graphs (owned)
 };
 
 struct State {
@@ -99,6 +101,9 @@ struct State {
 int materialize_basis(State *st);
 
 int ensure_scratch(Context *ctx, size_t bytes);
+// sc_run_jobs (sc_graph.hip); destroy_job_graphs releases a context's recorded graphs.
+int run_jobs(Context *ctx, const sc_job *jobs, int n_jobs, int *used_graph);
+void destroy_job_graphs(Context *ctx);
 // Device address of sc_expectation_pauli_terms_noalloc's fixed scratch array on `device` (current), once per device.
 int init_noalloc_scratch(int device);
 int ensure_host_scratch(Context *ctx, size_t bytes);
@@ -205,6 +210,16 @@ int launch_max_deviation(Context *ctx, const double2 *amp, uint64_t count, uint6
 int launch_kraus_weights(Context *ctx, const double2 *amp, int n, int target, const double *kraus, int k,
                          double *weights_host);
 int launch_scale_state(Context *ctx, double2 *amp, uint64_t count, double factor);
+
+// Per-term Pauli expectations of every batch element (sc_adjoint.hip), in two halves so that the device part
+// can be captured into a graph: pauli_terms_enqueue launches the term kernels into `scratch` (partial sums,
+// term_scratch_bytes of them for term_parts partials per row) and leaves the raw row sums in out_dev
+// ([batch][n_terms]); pauli_terms_finish applies the host-side factors to those rows once copied back.
+uint64_t term_parts(const Context *ctx, uint64_t B, int n, uint64_t rows);
+size_t term_scratch_bytes(uint64_t B, uint64_t rows, uint64_t parts);
+int pauli_terms_enqueue(Context *ctx, State *st, const double2 *amp, const uint64_t *xm, const uint64_t *zm,
+                        int n_terms, void *scratch, size_t scratch_bytes, double *out_dev);
+void pauli_terms_finish(const uint64_t *xm, const uint64_t *zm, int n_terms, uint64_t B, double *vals);
 
 
 

@@ -144,6 +144,11 @@ struct PlanCache {
 std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, int max_fuse_bits,
                            int jit, double pass_cap, int chunk_bits = kChunkBits);
 
+// The plan (and runtime-compiled pass kernels) of an analysed circuit on n qubits under the context's options,
+// from the context's plan cache (planned and compiled on a miss).  The entry stays valid until 16 other
+// structures have been planned after it.
+int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCache::Entry **out);
+
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
 void build_pool(const SweepPlan &plan, const double *mats, double *out);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS
