@@ -41,16 +41,27 @@ def log(msg: str) -> None:
 
 
 def make_context(args) -> capi.Context:
+    """A context with the library's defaults (runtime-compiled passes, chunk and low qubits chosen per circuit
+    structure), changed only by the options given on the command line."""
     ctx = capi.Context(0, MODES[args.mode])
-    ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
-    ctx.set_option(capi.OPT_REMAP, 1 if args.remap else 0)
+    if args.low_qubits is not None:
+        ctx.set_option(capi.OPT_LOW_QUBITS, args.low_qubits)
+    if args.remap:
+        ctx.set_option(capi.OPT_REMAP, 1)
     ctx.set_option(capi.OPT_SWEEP_BLOCKS_PER_CU, args.blocks_per_cu)
-    ctx.set_option(capi.OPT_JIT, args.jit)
+    if args.jit is not None:
+        ctx.set_option(capi.OPT_JIT, args.jit)
     ctx.set_option(capi.OPT_SWEEP_PASS_CAP, args.pass_cap)
     ctx.set_option(capi.OPT_DIRECT_IO, 0 if args.no_direct_io else 1)
-    if args.jit:
+    if args.chunk_bits is not None:
         ctx.set_option(capi.OPT_CHUNK_BITS, args.chunk_bits)
     return ctx
+
+
+def context_options(ctx) -> dict:
+    return {"jit": ctx.get_option(capi.OPT_JIT), "remap": bool(ctx.get_option(capi.OPT_REMAP)),
+            "low_qubits_option": ctx.get_option(capi.OPT_LOW_QUBITS),
+            "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS)}
 
 
 def host_threads() -> int:
@@ -194,6 +205,8 @@ def bench_c2(args):
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
     plan = ctx.plan_stats()
+    geom = ctx.plan_geometry()  # the geometry the library picked for this circuit
+    opts = context_options(ctx)
     ctx.synchronize()
     ctx.profile_reset()
     ctx.set_option(capi.OPT_PROFILE, 1)
@@ -230,9 +243,9 @@ def bench_c2(args):
         c_n, c_ms, c_bytes = ctx.profile_read(capi.KFAM_COPY)
         copy_gbs = (c_bytes / c_n) / (c_ms / c_n * 1e-3) / 1e9 if c_n else None
         dst.close()
-    fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, args.low_qubits, None, bool(args.remap), bool(args.jit),
-                                           args.pass_cap, args.chunk_bits if args.jit else 12)[0]["fp64_ops_per_amp"]
-                     if args.mode == "sweep" else None)
+    fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, geom["low_qubits"], None, opts["remap"], opts["jit"] != 0,
+                                           args.pass_cap, geom["chunk_bits"])[0]["fp64_ops_per_amp"]
+                     if args.mode == "sweep" and args.warmup > 0 else None)
     traffic = None
     if args.traffic_csv:
         traffic = pmc_traffic(args.traffic_csv[0], args.traffic_csv[1], 32.0 * (1 << n))
@@ -254,8 +267,10 @@ def bench_c2(args):
             "qubits": n, "depth": depth, "ops": int(len(ops)), "shots": args.shots, "exec_mode": args.mode,
             "sweeps_per_step": sweeps, "kernel_launches_per_step": plan["launches"],
             "fused_16x16_blocks": plan["fused_blocks"], "parallelism": "1 GPU",
-            "low_qubits": args.low_qubits, "remap": bool(args.remap), "blocks_per_cu": args.blocks_per_cu,
-            "jit": args.jit, "pass_cap": args.pass_cap, "chunk_bits": args.chunk_bits if args.jit else 12,
+            "blocks_per_cu": args.blocks_per_cu, "pass_cap": args.pass_cap, **opts,
+            # chosen by the library's geometry autotuner unless fixed by the options above
+            "chunk_bits": geom["chunk_bits"], "low_qubits": geom["low_qubits"],
+            "geometries_planned": geom["candidates"], "predicted_gate_ms": geom["ns_per_amp"] * (1 << n) * 1e-6,
         },
         "roofline": {
             "bound": "hbm",
@@ -264,7 +279,7 @@ def bench_c2(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": {"sweep": "sc_pass_*" if args.jit else "k_sweep", "fused": "k_dense",
+            "kernel": {"sweep": "sc_pass_*" if opts["jit"] else "k_sweep", "fused": "k_dense",
                        "direct": "k_single"}[args.mode],
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
@@ -347,6 +362,7 @@ def bench_c3(args):
 
     for _ in range(args.warmup):
         step()
+    geom = ctx.plan_geometry()
     ctx.profile_reset()
     ctx.set_option(capi.OPT_PROFILE, 1)
     t0 = time.perf_counter()
@@ -363,11 +379,12 @@ def bench_c3(args):
     res = _line(f"{n}q grover amp-updates/s", credit * args.steps / el, "amp-updates/s", args,
                 1e3 * el / args.steps,
                 {"workload": f"grover-{n}q-8-iterations", "qubits": n, "ops": len(ops), "marked": marked,
-                 "plan": ctx.plan_stats(), "chunk_bits": args.chunk_bits if args.jit else 12,
-                 "low_qubits": args.low_qubits},
+                 "plan": ctx.plan_stats(), **context_options(ctx), "chunk_bits": geom["chunk_bits"],
+                 "low_qubits": geom["low_qubits"], "geometries_planned": geom["candidates"]},
                 marked_probability=p, closed_form=want, rel_error=abs(p - want) / want,
                 roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK_GBS, "kernel": "sc_pass_*" if args.jit else "k_sweep"},
+                          "frac": achieved / HBM_PEAK_GBS,
+                          "kernel": "sc_pass_*" if ctx.get_option(capi.OPT_JIT) else "k_sweep"},
                 shots_1024_distinct=int(len(np.unique(out))))
     st.close()
     ctx.close()
@@ -425,11 +442,14 @@ def bench_c4(args):
     t_traj = (time.perf_counter() - t1) / args.steps
     for st in states + [tstate]:
         st.close()
+    opts = context_options(ctx)
+    geoms = {str(n): list(capi.plan_geometry_host(n, ops, mats[0], opts["low_qubits_option"], opts["remap"],
+                                                  opts["jit"], opts["chunk_bits_option"])[:2])
+             for n, B, ops, mats, w, zm in batches} if opts["jit"] else None
     return _line("batched QAOA instances/s (12-22 q, p=4) + GHZ-20 depolarising trajectories/s",
                  n_inst / t_qaoa, "instances/s", args, 1e3 * (t_qaoa + t_traj),
                  {"workload": "qaoa-ring-batched+ghz20-trajectories", "instances": n_inst,
-                  "trajectories": nt, "chunk_bits": args.chunk_bits if args.jit else 12,
-                  "low_qubits": args.low_qubits},
+                  "trajectories": nt, **opts, "chunk_and_low_qubits_by_size": geoms},
                  qaoa_seconds=t_qaoa, mean_cut_expectation=float(np.mean(mean_c)),
                  qaoa_one_graph_launch=bool(graph_used and all(graph_used)),
                  trajectories_per_s=nt / t_traj, trajectory_seconds=t_traj,
@@ -450,6 +470,7 @@ def bench_c5(args):
     st = capi.State(ctx, n, batch)
     # warm-up with the timed call's segmentation: plans and pass kernels of every checkpoint segment
     st.adjoint_gradient(ops, mats[:1].repeat(batch, 0), P, *obs, checkpoint_every=args.checkpoint, per_element=True)
+    geom = ctx.plan_geometry()  # of the forward run (the reverse sweep's passes use 12-qubit chunks)
     st.set_basis(0)
     t0 = time.perf_counter()
     grads, energies = st.adjoint_gradient(ops, mats, P, *obs, checkpoint_every=args.checkpoint, per_element=True)
@@ -486,7 +507,8 @@ def bench_c5(args):
     err = float(np.max(np.abs(np.array(ps) - grads[0, sample])))
     return _line("adjoint gradients/s, 24q HEA 384 params", adj_rate, "gradients/s", args, 1e3 * t_adj,
                  {"workload": "hea-24q-8-layers-384-params", "param_sets": batch,
-                  "checkpoint_every": args.checkpoint},
+                  "checkpoint_every": args.checkpoint, **context_options(ctx),
+                  "forward_chunk_bits": geom["chunk_bits"], "forward_low_qubits": geom["low_qubits"]},
                  param_shift_gradients_per_s=ps_rate, speedup_vs_param_shift=adj_rate / ps_rate,
                  param_shift_sample=len(sample), max_abs_diff_vs_param_shift=err,
                  adjoint_stats=ctx.adjoint_stats(), energy_0=float(energies[0]))
@@ -514,10 +536,11 @@ def main(argv=None) -> int:
                     help="SC_OPT_DIRECT_IO 0: every pass goes through LDS at both ends")
     ap.add_argument("--no-copy-reference", action="store_true")
     ap.add_argument("--low-qubits", type=int, default=None,
-                    help="coalesced low positions every pass keeps local (default 4; 3 with 13-qubit chunks)")
-    ap.add_argument("--chunk-bits", type=int, default=None, choices=[12, 13],
-                    help="SC_OPT_CHUNK_BITS: local qubits per chunk of the runtime-compiled passes (default 12; 13 "
-                         "for c3, whose 33-qubit layers fit two 13-qubit passes, and for c4 and c5)")
+                    help="SC_OPT_LOW_QUBITS: coalesced low positions every pass keeps local (default: the library "
+                         "picks per circuit)")
+    ap.add_argument("--chunk-bits", type=int, default=None, choices=[0, 12, 13],
+                    help="SC_OPT_CHUNK_BITS: local qubits per chunk of the runtime-compiled passes (default: the "
+                         "library picks per circuit from its measured pass-time model)")
     ap.add_argument("--remap", action="store_true", help="sweep passes relabel qubits in place (SC_OPT_REMAP)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident sweep workgroups per CU (0: all that fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -526,18 +549,10 @@ def main(argv=None) -> int:
                          "workload: fills roofline.traffic")
     ap.add_argument("--pass-cap", type=int, default=-1,
                     help="SC_OPT_SWEEP_PASS_CAP: FP64 operations per amplitude one pass takes on (-1 default, 0 none)")
-    ap.add_argument("--jit", type=int, default=2, choices=[0, 1, 2],
+    ap.add_argument("--jit", type=int, default=None, choices=[0, 1, 2],
                     help="SC_OPT_JIT: 0 compiled sweep kernel, 1 runtime-compiled pass kernels, 2 the same with "
-                         "next-chunk prefetch (kernels are compiled during warmup)")
+                         "next-chunk prefetch (the library default; kernels are compiled during warmup)")
     args = ap.parse_args(argv)
-    # per-workload sweep geometry (measured): 13-qubit chunks cut the passes of Grover's 33-qubit layers (with 3
-    # low positions), of the batched 20-22 qubit circuits and of the 24-qubit forward runs; the 30-qubit layered
-    # circuit takes the same time either way (adjoint passes always use 12)
-    chunk_default, low_default = {"c3": (13, 3), "c4": (13, 4), "c5": (13, 4)}.get(args.workload, (12, 4))
-    if args.chunk_bits is None:
-        args.chunk_bits = chunk_default
-    if args.low_qubits is None:
-        args.low_qubits = low_default if args.chunk_bits == chunk_default else (3 if args.chunk_bits == 13 else 4)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",
               file=sys.stderr)
@@ -546,9 +561,7 @@ def main(argv=None) -> int:
     if args.workload == "c2" and not args.no_adjoint:
         # the metric's second half: adjoint gradients of the 24-qubit, 384-parameter HEA against parameter shift
         log("adjoint block (24-qubit HEA, 384 parameters, batch 64) ...")
-        a = argparse.Namespace(**vars(args))
-        a.chunk_bits, a.low_qubits = 13, 4
-        c5 = bench_c5(a)
+        c5 = bench_c5(args)
         res["adjoint"] = {k: c5[k] for k in ("value", "unit", "ms_per_step", "param_shift_gradients_per_s",
                                              "speedup_vs_param_shift", "max_abs_diff_vs_param_shift",
                                              "param_shift_sample")}

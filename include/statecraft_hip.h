@@ -95,19 +95,25 @@ typedef struct sc_op {
 #define SC_OPT_MAX_FUSED 2      /* max qubits of a dense fused block (0..5; sweeps fuse at most 4, 0 = no sweep
                                    fusion, fused mode at least 1), default 4 */
 #define SC_OPT_PROFILE 3        /* 1: time every sweep/gate launch with HIP events */
-#define SC_OPT_LOW_QUBITS 4     /* qubits always resident in a sweep chunk for coalescing (0..8), default 4 */
+#define SC_OPT_LOW_QUBITS 4     /* qubits always resident in a sweep chunk for coalescing (0..8); -1: chosen per
+                                   circuit structure with SC_OPT_CHUNK_BITS (4 for the compiled sweep kernel),
+                                   default -1 */
 #define SC_OPT_TEST_DRIFT 5     /* test hook: adds value*1e-12 to psi[0] at every adjoint checkpoint, default 0 */
-#define SC_OPT_REMAP 6          /* 1: sweep passes relabel qubits in place to keep coalesced low positions useful, default 1 */
+#define SC_OPT_REMAP 6          /* 1: sweep passes relabel qubits in place to keep coalesced low positions useful, default 0 */
 #define SC_OPT_SWEEP_BLOCKS_PER_CU 7 /* resident sweep workgroups per CU (0: as many as fit), default 0 */
 #define SC_OPT_JIT 8            /* 1: sweep passes run as kernels compiled at run time (hipRTC) with the pass
                                    constant-folded; compiled once per circuit structure and cached; 2: the same
                                    with the next chunk's HBM loads issued before the current chunk's stages;
-                                   default 0 */
+                                   0: the precompiled sweep kernel; default 2 */
 #define SC_OPT_SWEEP_PASS_CAP 9 /* most estimated FP64 operations per amplitude one sweep pass takes on (0: no
                                    limit; -1: the default of the sweep kernel kind), default -1 */
 #define SC_OPT_CHUNK_BITS 10    /* local qubits per sweep chunk with runtime-compiled passes: 12 (64 KiB of LDS, two
                                    256-thread workgroups per CU) or 13 (128 KiB, one 512-thread workgroup per CU);
-                                   default 12.  The compiled sweep kernel always uses 12. */
+                                   0: chosen per circuit structure (with SC_OPT_LOW_QUBITS -1 also the low qubits):
+                                   the circuit is planned under 12/4, 13/4 and 13/3 chunk/low qubits and the plan
+                                   with the smallest pass time predicted by a model fitted to measured MI355X pass
+                                   times is kept and cached (sc_last_plan_geometry); default 0.  The compiled
+                                   sweep kernel always uses 12. */
 #define SC_OPT_DIRECT_IO 11     /* 1: a runtime-compiled pass whose first (last) stage keeps the chunk's low positions
                                    on lane bits reads HBM straight into (writes it straight from) that stage's
                                    registers, skipping an LDS round trip; 0: always through LDS; default 1 */
@@ -273,6 +279,11 @@ int sc_profile_reset(sc_ctx *ctx);
 int sc_profile_read(sc_ctx *ctx, int family, uint64_t *launches, double *total_ms, double *bytes);
 /* Plan statistics of the last sc_apply_ops: number of full-state sweeps executed and ops credited. */
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks);
+/* Geometry of the last sweep plan the context used (planned or from its cache): local qubits per chunk, low
+ * qubits, how many geometries were planned to choose it (1 when both options fix it) and its predicted pass
+ * time in ns per amplitude summed over its passes.  Any output may be NULL.  No counterpart in the reference:
+ * it reports the SC_OPT_CHUNK_BITS / SC_OPT_LOW_QUBITS autotuner's choice. */
+int sc_last_plan_geometry(sc_ctx *ctx, int *chunk_bits, int *low_qubits, int *candidates, double *ns_per_amp);
 /* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
  * (n >= 12) as a context with the given low-qubit, remap, jit, pass-cap and chunk-bit (12, or 13 with jit)
  * options would, and reports its passes,
@@ -283,6 +294,11 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
                         uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, int chunk_bits,
                         uint64_t *passes, uint64_t *stages, uint64_t *fused_blocks, double *fp64_ops_per_amp,
                         double *state_inout);
+/* Host-only (no device needed): the geometry a context with these options picks for the circuit (chunk_bits 0
+ * and / or low_qubits -1 leave it to the autotuner; see SC_OPT_CHUNK_BITS), as sc_last_plan_geometry reports. */
+int sc_plan_geometry_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                          uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits,
+                          int *chunk_bits_out, int *low_qubits_out, int *candidates, double *ns_per_amp);
 /* Host-only (no device needed): per pass of the same plan (at most max_passes entries), its stages, the stage
  * exchanges that need a workgroup barrier, its FP64 operations per amplitude and its direct-HBM flags (bit 0:
  * loads into the first stage, bit 1: stores from the last; SC_OPT_DIRECT_IO); n_passes receives the count. */

@@ -83,7 +83,8 @@ EXPORTED = [
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
     "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
-    "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host", "sc_run_jobs",
+    "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host", "sc_run_jobs", "sc_last_plan_geometry",
+    "sc_plan_geometry_host",
 ]
 
 _lib = None
@@ -143,6 +144,9 @@ def _sig(lib):
         "sc_run_jobs": (i32, [P, P, i32, ctypes.POINTER(i32)]),
         "sc_plan_pass_stats_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, i32, I32P, I32P, DP, I32P,
                                           ctypes.POINTER(i32)]),
+        "sc_last_plan_geometry": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32), DP]),
+        "sc_plan_geometry_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, ctypes.POINTER(i32),
+                                        ctypes.POINTER(i32), ctypes.POINTER(i32), DP]),
         "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
         "sc_sample_host": (i32, [i32, DP, I32P, i32, DP, i64, I64P, DP, U64P]),
     }
@@ -219,6 +223,19 @@ def plan_pass_stats_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = F
                                         dr.ctypes.data_as(I32P), ctypes.byref(cnt)))
     k = min(cnt.value, cap)
     return [(int(st[i]), int(br[i]), float(fp[i]), int(dr[i])) for i in range(k)]
+
+
+def plan_geometry_host(n: int, ops, mats, low_qubits: int = -1, remap: bool = False, jit: int = 2,
+                       chunk_bits: int = 0):
+    """Host-only: the sweep geometry a context with these options picks for the circuit: (chunk bits, low
+    qubits, geometries planned, predicted ns per amplitude)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+    cb, lq, nc, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    check(lib().sc_plan_geometry_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
+                                      int(low_qubits), 1 if remap else 0, int(jit), int(chunk_bits), ctypes.byref(cb),
+                                      ctypes.byref(lq), ctypes.byref(nc), ctypes.byref(ns)))
+    return cb.value, lq.value, nc.value, ns.value
 
 
 def adjoint_plan_host(n: int, ops, mats, n_params: int, psi, lam, low_qubits: int = 4):
@@ -340,6 +357,13 @@ class Context:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         check(lib().sc_last_plan_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return {"sweeps": a.value, "launches": b.value, "fused_blocks": c.value}
+
+    def plan_geometry(self):
+        """Geometry of the last sweep plan this context used."""
+        cb, lq, nc, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().sc_last_plan_geometry(self.h, ctypes.byref(cb), ctypes.byref(lq), ctypes.byref(nc),
+                                          ctypes.byref(ns)))
+        return {"chunk_bits": cb.value, "low_qubits": lq.value, "candidates": nc.value, "ns_per_amp": ns.value}
 
     def sample_stats(self):
         """Exact-sum statistics of the last sample() on this context."""

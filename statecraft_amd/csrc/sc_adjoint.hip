@@ -628,12 +628,13 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
         if ((rc = adjoint_sequence(n, ops, begin, end, slot, inv_elems, n_mat_complex, seq))) return rc;
         // plan and kernels of this segment's structure, cached like forward plans
         const int max_fuse = std::min(ctx->max_fused, 2);  // both states' registers stay within the budget
-        const std::string key = plan_cache_key(n, seq, ctx->low_qubits, false, max_fuse, ctx->jit, 0.0) + "adjoint";
+        const int low = ctx->low_qubits < 0 ? kDefaultLowQubits : ctx->low_qubits;
+        const std::string key = plan_cache_key(n, seq, low, false, max_fuse, ctx->jit, 0.0) + "adjoint";
         PlanCache::Entry *ent = ctx->plan_cache->find(key);
         if (!ent) {
             std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
             e->key = key;
-            if ((rc = plan_sweeps(n, seq, ctx->low_qubits, false, e->plan, max_fuse, kAdjointPassCost))) return rc;
+            if ((rc = plan_sweeps(n, seq, low, false, e->plan, max_fuse, kAdjointPassCost))) return rc;
             if (!e->plan.passes.empty() && (rc = jit_adjoint_kernels(ctx, e->plan, e->fns))) return rc;
             ent = ctx->plan_cache->insert(std::move(e));
         }

@@ -199,19 +199,26 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCa
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
     cm.direct_io = cm.direct_io && ctx->direct_io;
-    const int cb = (ctx->jit && ctx->chunk_bits == kMaxChunkBits && n >= kMaxChunkBits) ? kMaxChunkBits : kChunkBits;
+    // a fixed 13-bit chunk applies from 13 qubits with runtime-compiled passes; 0 leaves the choice to the planner
+    int cb = ctx->chunk_bits;
+    if (!ctx->jit || (cb == kMaxChunkBits && n < kMaxChunkBits)) cb = kChunkBits;
     const std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
                                            ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
         e->key = key;
-        int rc = plan_sweeps(n, ops, ctx->low_qubits, ctx->remap != 0, e->plan, max_fuse, cm, cb);
+        int rc = plan_sweeps_geometry(n, ops, ctx->low_qubits, cb, ctx->remap != 0, ctx->jit != 0, e->plan, max_fuse,
+                                      cm, &e->pick);
         if (rc) return rc;
         if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
         ent = ctx->plan_cache->insert(std::move(e));
     }
     ent->used = ++ctx->plan_cache->tick;
+    ctx->last_chunk_bits = ent->pick.chunk_bits;
+    ctx->last_low_qubits = ent->pick.low_qubits;
+    ctx->last_candidates = ent->pick.candidates;
+    ctx->last_ns_per_amp = ent->pick.ns_per_amp;
     *out = ent;
     return SC_OK;
 }
@@ -491,7 +498,7 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         return SC_OK;
     }
     case SC_OPT_LOW_QUBITS:
-        if (value < 0 || value > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
+        if (value < -1 || value > 8) return fail(SC_E_ARG, "low qubits must be 0..8, or -1 (chosen per circuit)");
         ctx->low_qubits = (int)value;
         return SC_OK;
     case SC_OPT_TEST_DRIFT:
@@ -514,7 +521,8 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         ctx->pass_cap = (int)value;
         return SC_OK;
     case SC_OPT_CHUNK_BITS:
-        if (value != kChunkBits && value != kMaxChunkBits) return fail(SC_E_ARG, "chunk bits must be 12 or 13");
+        if (value != 0 && value != kChunkBits && value != kMaxChunkBits)
+            return fail(SC_E_ARG, "chunk bits must be 12 or 13, or 0 (chosen per circuit)");
         ctx->chunk_bits = (int)value;
         return SC_OK;
     case SC_OPT_DIRECT_IO:
@@ -991,6 +999,43 @@ int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches,
     if (sweeps) *sweeps = ctx->last_sweeps;
     if (kernel_launches) *kernel_launches = ctx->last_launches;
     if (fused_blocks) *fused_blocks = ctx->last_blocks;
+    return SC_OK;
+}
+
+int sc_last_plan_geometry(sc_ctx *ctx, int *chunk_bits, int *low_qubits, int *candidates, double *ns_per_amp) {
+    SC_REQUIRE(ctx, "ctx");
+    if (chunk_bits) *chunk_bits = ctx->last_chunk_bits;
+    if (low_qubits) *low_qubits = ctx->last_low_qubits;
+    if (candidates) *candidates = ctx->last_candidates;
+    if (ns_per_amp) *ns_per_amp = ctx->last_ns_per_amp;
+    return SC_OK;
+}
+
+int sc_plan_geometry_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                          uint64_t n_mat_complex, int low_qubits, int remap, int jit, int chunk_bits,
+                          int *chunk_bits_out, int *low_qubits_out, int *candidates, double *ns_per_amp) {
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
+    if (chunk_bits != 0 && chunk_bits != kChunkBits && chunk_bits != kMaxChunkBits)
+        return fail(SC_E_ARG, "chunk bits must be 12 or 13, or 0 (chosen per circuit)");
+    if (low_qubits < -1 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8, or -1 (chosen per circuit)");
+    if (jit < 0 || jit > 2) return fail(SC_E_ARG, "jit must be 0, 1 or 2");
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
+    if (rc) return rc;
+    int cb = chunk_bits;
+    if (!jit || (cb == kMaxChunkBits && num_qubits < kMaxChunkBits)) cb = kChunkBits;
+    SweepPlan plan;
+    GeometryPick pick;
+    if ((rc = plan_sweeps_geometry(num_qubits, pops, low_qubits, cb, remap != 0, jit != 0, plan, kRegBits,
+                                   jit ? kCompiledPassCost : kInterpretedCost, &pick)))
+        return rc;
+    if (chunk_bits_out) *chunk_bits_out = pick.chunk_bits;
+    if (low_qubits_out) *low_qubits_out = pick.low_qubits;
+    if (candidates) *candidates = pick.candidates;
+    if (ns_per_amp) *ns_per_amp = pick.ns_per_amp;
     return SC_OK;
 }
 

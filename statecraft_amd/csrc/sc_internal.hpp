@@ -61,12 +61,12 @@ struct Context {
     hipStream_t stream = nullptr;
     int exec_mode = SC_EXEC_SWEEP;
     int max_fused = 4;
-    int low_qubits = 4;
-    int remap = 1;
+    int low_qubits = -1;  // SC_OPT_LOW_QUBITS; -1: chosen per circuit structure (plan_sweeps_geometry)
+    int remap = 0;
     int sweep_blocks_per_cu = 0;
-    int jit = 0;
+    int jit = 2;
     int pass_cap = -1;
-    int chunk_bits = 12;  // SC_OPT_CHUNK_BITS
+    int chunk_bits = 0;   // SC_OPT_CHUNK_BITS; 0: chosen per circuit structure
     int direct_io = 1;    // SC_OPT_DIRECT_IO
     int cu_count = 256;
     Profiler prof;
@@ -81,6 +81,9 @@ struct Context {
     PlanCache *plan_cache = nullptr;  // owned
     // plan statistics of the last sc_apply_ops
     uint64_t last_sweeps = 0, last_launches = 0, last_blocks = 0;
+    // geometry of the last sweep plan acquired (sc_last_plan_geometry)
+    int last_chunk_bits = 0, last_low_qubits = 0, last_candidates = 0;
+    double last_ns_per_amp = 0.0;
     // exact-sum statistics of the last sc_sample: fast / serial / zero segments, gather rounds, rescaled
     uint64_t last_sample[5] = {};
     JobGraphCache *job_graphs = nullptr;  // recorded sc_run_jobs graphs (owned)

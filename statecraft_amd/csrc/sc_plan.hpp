@@ -125,6 +125,30 @@ constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true, false};
 int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool remap, SweepPlan &plan,
                 int max_fuse_bits = kRegBits, const SweepCostModel &cm = kInterpretedCost,
                 int chunk_bits = kChunkBits);
+
+// Predicted time of a runtime-compiled pass (and of a whole plan) on MI355X, in ns per amplitude: the larger
+// of the chunk size's HBM floor and a linear cost in the pass's FP64 operations and LDS round trips, fitted to
+// measured pass times.
+double predicted_pass_ns_per_amp(const SweepPlan &plan, const SweepPass &pass);
+double predicted_plan_ns_per_amp(const SweepPlan &plan);
+
+// Low qubits of a sweep plan when the option leaves them to the library and no prediction applies.
+constexpr int kDefaultLowQubits = 4;
+
+struct GeometryPick {
+    int chunk_bits = kChunkBits;
+    int low_qubits = kDefaultLowQubits;
+    double ns_per_amp = 0.0;  // predicted plan time per amplitude
+    int candidates = 0;       // geometries planned
+};
+// The geometry autotuner.  chunk_bits 0 and / or low_qubits -1 leave that choice to the library: with
+// runtime-compiled passes (compiled) the circuit is planned under each measured geometry that fits the fixed
+// choices ((12, 4), (13, 4), (13, 3) chunk / low qubits; 13-bit chunks from 13 qubits) and the plan with the
+// smallest predicted time is kept (ties: that order).  The compiled sweep kernel always uses (12, 4) or the
+// fixed low count.  Deterministic: the pick depends only on the circuit structure and the options.
+int plan_sweeps_geometry(int n, const std::vector<PlanOp> &ops, int low_qubits, int chunk_bits, bool remap,
+                         bool compiled, SweepPlan &plan, int max_fuse_bits, const SweepCostModel &cm,
+                         GeometryPick *pick);
 // Plans of recently executed circuits.  A plan depends on the op structure (kinds, qubits, controls,
 // matrix offsets and the diagonal / real / exchange classification of every matrix) and the options that
 // shape it, not on matrix values, so circuits that differ only in parameters share one entry.
@@ -133,6 +157,7 @@ struct PlanCache {
         std::string key;
         SweepPlan plan;
         std::vector<hipFunction_t> fns;  // runtime-compiled pass kernels (SC_OPT_JIT), else empty
+        GeometryPick pick;               // the geometry the plan was made with
         uint64_t used = 0;
     };
     static constexpr size_t kMaxEntries = 16;

@@ -21,7 +21,15 @@ def pytest_configure(config):
 def hip_ctx():
     from statecraft_amd import capi
 
+    # The shared context runs the precompiled sweep kernel (jit 0, 4 low qubits, in-place relabelling): the
+    # parity suite's many circuit structures then need no run-time compilation.  The library's defaults
+    # (runtime-compiled passes, geometry chosen per circuit) are exercised by the tests that create their own
+    # contexts and by the engine API tests.
     ctx = capi.Context(0)
+    ctx.set_option(capi.OPT_JIT, 0)
+    ctx.set_option(capi.OPT_LOW_QUBITS, 4)
+    ctx.set_option(capi.OPT_REMAP, 1)
+    ctx.set_option(capi.OPT_CHUNK_BITS, 12)
     yield ctx
     ctx.close()
 

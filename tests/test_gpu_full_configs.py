@@ -40,16 +40,10 @@ def say(msg):
 
 @pytest.fixture(scope="module")
 def big_ctx():
-    ctx = capi.Context(0, capi.EXEC_SWEEP)
-    ctx.set_option(capi.OPT_JIT, 2)
-    ctx.set_option(capi.OPT_REMAP, 0)
+    # library defaults: runtime-compiled passes, chunk and low qubits chosen per circuit structure
+    ctx = capi.Context(0)
     yield ctx
     ctx.close()
-
-
-def geometry(ctx, chunk_bits, low):
-    ctx.set_option(capi.OPT_CHUNK_BITS, chunk_bits)
-    ctx.set_option(capi.OPT_LOW_QUBITS, low)
 
 
 def random_layered_inverse(n, depth, seed):
@@ -69,12 +63,12 @@ def random_layered_inverse(n, depth, seed):
 
 def test_c2_random_layered_30q(big_ctx):
     n, depth, seed = 30, 20, 7
-    geometry(big_ctx, 12, 4)
     ops, mats = lower(workloads.random_layered(n, depth, seed))
     st = capi.State(big_ctx, n)
     try:
         say("c2: 30 qubits, depth 20, 1490 ops")
         st.apply(ops, mats)
+        say(f"c2: geometry {big_ctx.plan_geometry()}")
         cs = st.checksum()
         assert abs(cs[0] - 1.0) <= EPS, cs
         u = SplittableRandom(seed).next_doubles(8192)
@@ -101,7 +95,6 @@ def test_c2_random_layered_30q(big_ctx):
 
 def test_c3_grover_33q(big_ctx):
     n, iters, seed = 33, 8, 33
-    geometry(big_ctx, 13, 3)
     circ, marked = workloads.grover(n, iters, seed)
     ops, mats = lower(circ)
     assert max(bin(int(c)).count("1") for c in ops["controls"]) == 32
@@ -109,6 +102,7 @@ def test_c3_grover_33q(big_ctx):
     try:
         say(f"c3: 33 qubits (128 GiB), Grover, 8 iterations, marked {marked}")
         st.apply(ops, mats)
+        say(f"c3: geometry {big_ctx.plan_geometry()}")
         # the diffusion H X MCZ X H is -(2|s><s| - I): each iteration adds a factor -1, (-1)^8 = 1
         th = math.asin(2.0 ** (-n / 2.0))
         a = math.sin((2 * iters + 1) * th)
@@ -132,7 +126,6 @@ def test_c3_grover_33q(big_ctx):
 
 
 def test_c4_qaoa_batch_and_ghz_trajectories(big_ctx):
-    geometry(big_ctx, 13, 4)
     batches = workloads.qaoa_batches(4096, 4, 4)
     assert sum(len(idx) for _, idx, _, _, _ in batches) == 4096
     say(f"c4: {len(batches)} QAOA size batches")
@@ -176,7 +169,6 @@ def test_c4_qaoa_batch_and_ghz_trajectories(big_ctx):
 
 def test_c5_adjoint_hea_24q(big_ctx):
     n, layers, batch = 24, 8, 64
-    geometry(big_ctx, 13, 4)
     P = 2 * n * layers
     rng = np.random.default_rng(5)
     thetas = SplittableRandom(5).next_doubles(batch * P).reshape(batch, P) * 2 * math.pi

@@ -318,3 +318,47 @@ def test_direct_hbm_passes_match_oracle(n, chunk_bits, family):
     assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
     if family != "random" and n > chunk_bits:
         assert any(d & 1 for *_, d in per_pass) and any(d & 2 for *_, d in per_pass), per_pass
+
+
+def test_geometry_autotuner_picks_the_fastest_predicted_plan():
+    """chunk_bits 0 / low_qubits -1: every measured geometry is planned and the smallest predicted time is kept."""
+    n = 33
+    ops, mats = lower(workloads.grover(n, 8, 33)[0])
+    fixed = {(cb, lq): capi.plan_geometry_host(n, ops, mats, lq, False, 2, cb) for cb, lq in ((12, 4), (13, 4), (13, 3))}
+    for (cb, lq), (gcb, glq, cands, ns) in fixed.items():
+        assert (gcb, glq, cands) == (cb, lq, 1) and ns > 0
+    pick = capi.plan_geometry_host(n, ops, mats)
+    best = min(fixed, key=lambda g: fixed[g][3])
+    assert pick[:3] == (best[0], best[1], 3) and pick[3] == fixed[best][3]
+    # on the Grover circuit the 13-bit chunk with 3 low qubits needs a third fewer passes: it wins
+    assert best == (13, 3)
+    assert capi.plan_geometry_host(n, ops, mats) == pick  # deterministic
+    # fixing one option leaves the measured geometries that agree with it
+    assert capi.plan_geometry_host(n, ops, mats, -1, False, 2, 12)[:3] == (12, 4, 1)
+    assert capi.plan_geometry_host(n, ops, mats, -1, False, 2, 13)[:3] == (13, 3, 2)
+    low4 = capi.plan_geometry_host(n, ops, mats, 4, False, 2, 0)
+    assert low4[1:3] == (4, 2) and low4[3] == min(fixed[(12, 4)][3], fixed[(13, 4)][3])
+
+
+def test_geometry_autotuner_limits():
+    ops, mats = lower(workloads.ry_qft(12, 3)[0])
+    # 12 qubits hold no 13-bit chunk; the precompiled sweep kernel always uses 12 / 4
+    assert capi.plan_geometry_host(12, ops, mats)[:3] == (12, 4, 1)
+    ops, mats = lower(workloads.random_layered(16, 4, 1))
+    assert capi.plan_geometry_host(16, ops, mats, -1, False, 0, 0)[:3] == (12, 4, 1)
+    assert capi.plan_geometry_host(16, ops, mats, 2, False, 0, 13)[:3] == (12, 2, 1)
+    with pytest.raises(capi.IllegalArgumentException):
+        capi.plan_geometry_host(16, ops, mats, -2)
+    with pytest.raises(capi.IllegalArgumentException):
+        capi.plan_geometry_host(16, ops, mats, -1, False, 2, 11)
+
+
+@pytest.mark.parametrize("n", [13, 17])
+def test_autotuned_plan_matches_oracle(n):
+    """The plan a default context runs (geometry from the autotuner) moves the data exactly as the oracle."""
+    ops, mats = lower(workloads.random_layered(n, 6, n))
+    cb, lq, _, _ = capi.plan_geometry_host(n, ops, mats)
+    s0 = random_state(n, np.random.default_rng(n))
+    _, got = capi.plan_sweeps_host(n, ops, mats, low_qubits=lq, state=s0, remap=False, jit=True, chunk_bits=cb)
+    want = oracle.run_ops(n, ops, mats, s0.copy())
+    assert np.max(np.abs(got - want)) <= 1e-12
