@@ -204,9 +204,6 @@ def bench_c2(args):
     for i in range(args.warmup):
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
-    plan = ctx.plan_stats()
-    geom = ctx.plan_geometry()  # the geometry the library picked for this circuit
-    opts = context_options(ctx)
     ctx.synchronize()
     ctx.profile_reset()
     ctx.set_option(capi.OPT_PROFILE, 1)
@@ -217,6 +214,9 @@ def bench_c2(args):
     ctx.synchronize()
     t1 = time.perf_counter()
     ctx.set_option(capi.OPT_PROFILE, 0)
+    plan = ctx.plan_stats()
+    geom = ctx.plan_geometry()  # the geometry the library picked for this circuit
+    opts = context_options(ctx)
     launches, ms, nbytes = ctx.profile_read(capi.KFAM_SWEEP if args.mode == "sweep" else
                                             capi.KFAM_FUSED if args.mode == "fused" else capi.KFAM_GATE)
     g_launches, g_ms, _ = ctx.profile_read(capi.KFAM_GATE)
@@ -245,7 +245,7 @@ def bench_c2(args):
         dst.close()
     fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, geom["low_qubits"], None, opts["remap"], opts["jit"] != 0,
                                            args.pass_cap, geom["chunk_bits"])[0]["fp64_ops_per_amp"]
-                     if args.mode == "sweep" and args.warmup > 0 else None)
+                     if args.mode == "sweep" else None)
     traffic = None
     if args.traffic_csv:
         traffic = pmc_traffic(args.traffic_csv[0], args.traffic_csv[1], 32.0 * (1 << n))
@@ -362,7 +362,6 @@ def bench_c3(args):
 
     for _ in range(args.warmup):
         step()
-    geom = ctx.plan_geometry()
     ctx.profile_reset()
     ctx.set_option(capi.OPT_PROFILE, 1)
     t0 = time.perf_counter()
@@ -372,6 +371,7 @@ def bench_c3(args):
     ctx.synchronize()
     el = time.perf_counter() - t0
     ctx.set_option(capi.OPT_PROFILE, 0)
+    geom = ctx.plan_geometry()
     launches, ms, nbytes = ctx.profile_read(capi.KFAM_SWEEP)
     achieved = (nbytes / launches) / (ms / launches * 1e-3) / 1e9 if launches else 0.0
     u = SplittableRandom(33).next_doubles(1024)

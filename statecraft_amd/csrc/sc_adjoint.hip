@@ -82,7 +82,7 @@ __device__ __forceinline__ uint64_t expand_mask(uint64_t x, uint64_t mask) {
 // when |x & z_t| is even, 2i s(i) Im q when odd.  The kernel therefore visits each pair once (index bit
 // ctz(x) clear) and accumulates s(i) Re q or s(i) Im q per term (bit t of `imag`); for x = 0 it visits every
 // index with q = |psi_i|^2.  The host applies the factor 2 and the sign of i^{|x & z_t|}.
-constexpr int kTermsPerLaunch = 16;  // more would spill the terms' scalar registers
+constexpr int kTermsPerLaunch = 32;  // a 32-qubit ring's ZZ terms in one read of the state
 struct TermGroup {
     uint64_t x;
     uint64_t z[kTermsPerLaunch];
@@ -445,8 +445,14 @@ int pauli_terms_enqueue(Context *ctx, State *st, const double2 *amp, const uint6
                 else if (g.nt <= 12)
                     hipLaunchKernelGGL(k_pauli_terms<12>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
                                        (int)rows, (int)parts, part);
-                else
+                else if (g.nt <= 16)
                     hipLaunchKernelGGL(k_pauli_terms<16>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
+                                       (int)rows, (int)parts, part);
+                else if (g.nt <= 24)
+                    hipLaunchKernelGGL(k_pauli_terms<24>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
+                                       (int)rows, (int)parts, part);
+                else
+                    hipLaunchKernelGGL(k_pauli_terms<32>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
                                        (int)rows, (int)parts, part);
                 SC_HIP_CHECK(hipGetLastError());
             }

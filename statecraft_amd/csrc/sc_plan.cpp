@@ -652,6 +652,9 @@ int plan_sweeps_geometry(int n, const std::vector<PlanOp> &ops, int low_qubits, 
         if (cand.empty()) cand.push_back({chunk_bits > 0 ? chunk_bits : kChunkBits,
                                           low_qubits >= 0 ? low_qubits : kDefaultLowQubits});
     }
+    // A later candidate replaces the pick only when predicted at least kMargin faster: closer predictions are
+    // within the model's error (6% per run), and the earlier geometry (two workgroups per CU first) is kept.
+    constexpr double kMargin = 0.03;
     double best = 0.0;
     int best_i = -1;
     for (size_t i = 0; i < cand.size(); i++) {
@@ -659,7 +662,7 @@ int plan_sweeps_geometry(int n, const std::vector<PlanOp> &ops, int low_qubits, 
         int rc = plan_sweeps(n, ops, cand[i].second, remap, p, max_fuse_bits, cm, cand[i].first);
         if (rc) return rc;
         const double t = predicted_plan_ns_per_amp(p);
-        if (best_i < 0 || t < best) {
+        if (best_i < 0 || t < best * (1.0 - kMargin)) {
             best = t;
             best_i = (int)i;
             plan = std::move(p);

@@ -144,7 +144,7 @@ struct GeometryPick {
 // The geometry autotuner.  chunk_bits 0 and / or low_qubits -1 leave that choice to the library: with
 // runtime-compiled passes (compiled) the circuit is planned under each measured geometry that fits the fixed
 // choices ((12, 4), (13, 4), (13, 3) chunk / low qubits; 13-bit chunks from 13 qubits) and the plan with the
-// smallest predicted time is kept (ties: that order).  The compiled sweep kernel always uses (12, 4) or the
+// smallest predicted time is kept, where a later candidate must be predicted 3% faster than an earlier one.  The compiled sweep kernel always uses (12, 4) or the
 // fixed low count.  Deterministic: the pick depends only on the circuit structure and the options.
 int plan_sweeps_geometry(int n, const std::vector<PlanOp> &ops, int low_qubits, int chunk_bits, bool remap,
                          bool compiled, SweepPlan &plan, int max_fuse_bits, const SweepCostModel &cm,

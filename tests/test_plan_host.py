@@ -328,7 +328,10 @@ def test_geometry_autotuner_picks_the_fastest_predicted_plan():
     for (cb, lq), (gcb, glq, cands, ns) in fixed.items():
         assert (gcb, glq, cands) == (cb, lq, 1) and ns > 0
     pick = capi.plan_geometry_host(n, ops, mats)
-    best = min(fixed, key=lambda g: fixed[g][3])
+    best = (12, 4)  # a later candidate must be predicted 3% faster than the pick so far
+    for g in ((13, 4), (13, 3)):
+        if fixed[g][3] < fixed[best][3] * 0.97:
+            best = g
     assert pick[:3] == (best[0], best[1], 3) and pick[3] == fixed[best][3]
     # on the Grover circuit the 13-bit chunk with 3 low qubits needs a third fewer passes: it wins
     assert best == (13, 3)
@@ -337,7 +340,11 @@ def test_geometry_autotuner_picks_the_fastest_predicted_plan():
     assert capi.plan_geometry_host(n, ops, mats, -1, False, 2, 12)[:3] == (12, 4, 1)
     assert capi.plan_geometry_host(n, ops, mats, -1, False, 2, 13)[:3] == (13, 3, 2)
     low4 = capi.plan_geometry_host(n, ops, mats, 4, False, 2, 0)
-    assert low4[1:3] == (4, 2) and low4[3] == min(fixed[(12, 4)][3], fixed[(13, 4)][3])
+    want = fixed[(13, 4)] if fixed[(13, 4)][3] < 0.97 * fixed[(12, 4)][3] else fixed[(12, 4)]
+    assert low4[:2] == want[:2] and low4[2] == 2 and low4[3] == want[3]
+    # the 30-qubit layered circuit: 12 / 4 and 13 / 4 are predicted within 2% of each other; 12 / 4 is kept
+    lops, lmats = lower(workloads.random_layered(30, 20, 7))
+    assert capi.plan_geometry_host(30, lops, lmats)[:3] == (12, 4, 3)
 
 
 def test_geometry_autotuner_limits():
