@@ -20,6 +20,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -183,6 +184,23 @@ def pmc_traffic(fetch_csv: str, write_csv: str, copy_bytes: float):
     return f[kern] * f_scale + w[kern] * w_scale
 
 
+def profiled_traffic(workload: str, chunk_bits: int, low_qubits: int):
+    """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled against k_copy)
+    for this workload and geometry, from the summary committed under profiles/ (None when none matches)."""
+    import re
+
+    path = Path(ROOT) / "profiles" / "sc_pass_random_layered_30q.md"
+    try:
+        text = path.read_text()
+    except OSError:
+        return None, None
+    for m in re.finditer(r"^traffic: workload=(\S+) chunk_bits=(\d+) low_qubits=(\d+) bytes_per_launch=(\d+)$", text,
+                         re.M):
+        if (m.group(1), int(m.group(2)), int(m.group(3))) == (workload, chunk_bits, low_qubits):
+            return float(m.group(4)), "profiles/" + path.name
+    return None, None
+
+
 def bench_c2(args):
     n, depth = args.qubits, args.depth
     circ = workloads.random_layered(n, depth, args.seed)
@@ -246,9 +264,13 @@ def bench_c2(args):
     fp64_per_step = (capi.plan_sweeps_host(n, ops, mats, geom["low_qubits"], None, opts["remap"], opts["jit"] != 0,
                                            args.pass_cap, geom["chunk_bits"])[0]["fp64_ops_per_amp"]
                      if args.mode == "sweep" else None)
-    traffic = None
+    traffic, traffic_source = None, None
     if args.traffic_csv:
         traffic = pmc_traffic(args.traffic_csv[0], args.traffic_csv[1], 32.0 * (1 << n))
+        traffic_source = "rocprofv3 --pmc csv files given with --traffic-csv"
+    elif args.mode == "sweep" and opts["jit"]:
+        traffic, traffic_source = profiled_traffic(f"random-layered-{n}q-depth{depth}", geom["chunk_bits"],
+                                                   geom["low_qubits"])
     result = {
         "metric": METRIC,
         "value": value,
@@ -279,6 +301,7 @@ def bench_c2(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "kernel": {"sweep": "sc_pass_*" if opts["jit"] else "k_sweep", "fused": "k_dense",
                        "direct": "k_single"}[args.mode],
             "bytes_per_launch": bytes_per_launch,

@@ -83,25 +83,53 @@ __device__ __forceinline__ uint64_t expand_mask(uint64_t x, uint64_t mask) {
 // ctz(x) clear) and accumulates s(i) Re q or s(i) Im q per term (bit t of `imag`); for x = 0 it visits every
 // index with q = |psi_i|^2.  The host applies the factor 2 and the sign of i^{|x & z_t|}.
 constexpr int kTermsPerLaunch = 32;  // a 32-qubit ring's ZZ terms in one read of the state
+constexpr int kTermJ = 8;            // amplitudes a lane takes per iteration
 struct TermGroup {
     uint64_t x;
     uint64_t z[kTermsPerLaunch];
     int32_t row[kTermsPerLaunch];  // output row of term t
+    uint8_t jm[kTermsPerLaunch];   // term t's sign pattern over the kTermJ amplitudes of an iteration (below)
     uint32_t imag;
     int32_t nt;
 };
 
 // grid (parts, elements), parts a power of two: part[((e0 + elem) * rows + row_t) * parts + block].
-// A thread visits h = h0 + k * S (S = parts * kBlock) with h0 < S, so the index i (h with a zero inserted at
-// bit ctz(x) for pairs) is c | u: c fixed per thread, u the same for every thread of an iteration.  The sign
-// (-1)^parity(i & z_t) is the thread's constant sign times a uniform one, so each term costs one FMA with a
-// scalar +-1 per amplitude and the thread's sign is applied once at the end.  NT >= g.nt terms are unrolled.
-template <int NT>
+// Let spread(h) be h with a zero inserted at bit ctz(x) (pairs; else h) and S = parts * kBlock.  A thread
+// visits h = h0 + k0 + j S for its h0 < S, every k0 that is a multiple of 8 S, and j < 8: the index is
+// i = spread(h0) | spread(k0) | spread(j S), three disjoint bit sets, so term t's sign is the product of a
+// per-thread factor (applied once at the end), a factor uniform over the iteration (parity of
+// spread(k0) & z_t) and (-1)^{<j, jm_t>}, where bit b of jm_t is the parity of spread(2^b S) & z_t.  Each
+// iteration therefore loads 8 values, takes their Walsh-Hadamard transform W_m = sum_j (-1)^{<j, m>} q_j, and
+// adds +-W_{jm_t} to term t: one add per term per 8 amplitudes, and the scalar unit computes one parity per
+// term per iteration.  NT >= g.nt terms are unrolled.
+__device__ __forceinline__ void wht8(double (&q)[kTermJ]) {
+#pragma unroll
+    for (int h = 1; h < kTermJ; h <<= 1)
+#pragma unroll
+        for (int j = 0; j < kTermJ; j++)
+            if (!(j & h)) {
+                const double a = q[j], b = q[j | h];
+                q[j] = a + b;
+                q[j | h] = a - b;
+            }
+}
+
+// q[m] for a uniform m < 8 without indexing registers at run time (that would go through scratch memory):
+// m = 0, the common case, takes a scalar branch; otherwise a select tree on m's three bits.
+__device__ __forceinline__ double pick8(const double (&q)[kTermJ], uint32_t m) {
+    if (m == 0) return q[0];
+    const bool b0 = m & 1u, b1 = m & 2u, b2 = m & 4u;
+    const double l0 = b0 ? q[1] : q[0], l1 = b0 ? q[3] : q[2], l2 = b0 ? q[5] : q[4], l3 = b0 ? q[7] : q[6];
+    const double h0 = b1 ? l1 : l0, h1 = b1 ? l3 : l2;
+    return b2 ? h1 : h0;
+}
+
+template <int NT, bool PAIRS>
 __global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restrict__ amp, int n, uint64_t e0,
                                                         TermGroup g, int rows, int parts, double *__restrict__ part) {
     const uint64_t elem = e0 + blockIdx.y;
     const double2 *base = amp + (elem << n);
-    const bool pairs = g.x != 0;
+    constexpr bool pairs = PAIRS;  // g.x != 0
     const uint64_t count = pairs ? (1ULL << (n - 1)) : (1ULL << n);
     const int pbit = pairs ? __builtin_ctzll(g.x) : 63;
     auto spread = [&](uint64_t h) {  // h with a zero inserted at bit pbit (pairs), else h
@@ -114,15 +142,16 @@ __global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restric
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = 0.0;
-    for (uint64_t k0 = 0; k0 < count; k0 += 4 * S) {
-        double qr[4], qi[4];
+    for (uint64_t k0 = 0; k0 < count; k0 += kTermJ * S) {
+        double qr[kTermJ], qi[kTermJ] = {};
+        const uint64_t ck = c | spread(k0);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < kTermJ; j++) {
             const uint64_t hk = k0 + (uint64_t)j * S;
             qr[j] = 0.0;
             qi[j] = 0.0;
             if (hk + h0 < count) {
-                const uint64_t i = c | spread(hk);
+                const uint64_t i = ck | spread((uint64_t)j * S);
                 if (pairs) {
                     const double2 a = base[i], b = base[i ^ g.x];
                     qr[j] = fma(b.x, a.x, b.y * a.y);
@@ -133,14 +162,15 @@ __global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restric
                 }
             }
         }
+        wht8(qr);
+        if constexpr (PAIRS) wht8(qi);
+        const uint64_t uk = spread(k0);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t u = spread(k0 + (uint64_t)j * S);
-#pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const double v = ((g.imag >> t) & 1u) ? qi[j] : qr[j];
-                acc[t] = fma((__popcll(u & g.z[t]) & 1) ? -1.0 : 1.0, v, acc[t]);
-            }
+        for (int t = 0; t < NT; t++) {
+            uint32_t m = g.jm[t] & (kTermJ - 1);
+            asm volatile("" : "+s"(m));  // derive the select masks here, not once per term before the loop
+            const double w = (PAIRS && ((g.imag >> t) & 1u)) ? pick8(qi, m) : pick8(qr, m);
+            acc[t] = (__popcll(uk & g.z[t]) & 1) ? acc[t] - w : acc[t] + w;
         }
     }
 #pragma unroll
@@ -162,6 +192,29 @@ __global__ void __launch_bounds__(kBlock) k_pauli_terms(const double2 *__restric
         for (int w = 0; w < kBlock / 64; w++) v += sh[w][threadIdx.x];
         part[((elem * (uint64_t)rows) + (uint64_t)g.row[threadIdx.x]) * (uint64_t)parts + blockIdx.x] = v;
     }
+}
+
+template <int NT>
+void launch_terms_nt(const TermGroup &g, dim3 grid, hipStream_t stream, const double2 *amp, int n, uint64_t e0,
+                     int rows, int parts, double *part) {
+    if (g.x != 0)
+        hipLaunchKernelGGL((k_pauli_terms<NT, true>), grid, dim3(kBlock), 0, stream, amp, n, e0, g, rows, parts, part);
+    else
+        hipLaunchKernelGGL((k_pauli_terms<NT, false>), grid, dim3(kBlock), 0, stream, amp, n, e0, g, rows, parts, part);
+}
+
+// The k_pauli_terms instance for the group's term count and x mask (pairs, or one index per amplitude).
+int launch_terms(const TermGroup &g, dim3 grid, hipStream_t stream, const double2 *amp, int n, uint64_t e0, int rows,
+                 int parts, double *part) {
+    if (g.nt <= 1) launch_terms_nt<1>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else if (g.nt <= 4) launch_terms_nt<4>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else if (g.nt <= 8) launch_terms_nt<8>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else if (g.nt <= 12) launch_terms_nt<12>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else if (g.nt <= 16) launch_terms_nt<16>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else if (g.nt <= 24) launch_terms_nt<24>(g, grid, stream, amp, n, e0, rows, parts, part);
+    else launch_terms_nt<32>(g, grid, stream, amp, n, e0, rows, parts, part);
+    SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
 }
 
 // out[r] = sum of the parts partial sums of row r
@@ -422,38 +475,27 @@ int pauli_terms_enqueue(Context *ctx, State *st, const double2 *amp, const uint6
             std::memset(&g, 0, sizeof(g));
             g.x = xm[t0];
             g.nt = (int)std::min<size_t>(per, members - c0);
+            // the index bits of spread(2^b S), b < 3, that the kernel's iterations step through
+            const int pbit = g.x ? __builtin_ctzll(g.x) : 63;
+            auto spread = [&](uint64_t h) {
+                const uint64_t lo = pbit < 63 ? (h & ((1ULL << pbit) - 1ULL)) : h;
+                return g.x ? (((h ^ lo) << 1) | lo) : h;
+            };
             for (int j = 0; j < g.nt; j++) {
                 while (xm[next] != xm[t0]) next++;
                 const int t = next++;
                 g.z[j] = zm[t];
                 g.row[j] = t;
                 if (__builtin_popcountll(xm[t] & zm[t]) & 1) g.imag |= 1u << j;
+                for (int b = 0; b < 3; b++)
+                    if (__builtin_popcountll(spread((uint64_t)parts * kBlock << b) & zm[t]) & 1) g.jm[j] |= (uint8_t)(1u << b);
             }
             for (uint64_t e0 = 0; e0 < B; e0 += 65535) {
                 const unsigned cnt = (unsigned)std::min<uint64_t>(65535, B - e0);
                 ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_ADJOINT, 16.0 * (double)((uint64_t)cnt << n));
                 const dim3 grid((unsigned)parts, cnt);
-                if (g.nt <= 1)
-                    hipLaunchKernelGGL(k_pauli_terms<1>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
-                                       (int)parts, part);
-                else if (g.nt <= 4)
-                    hipLaunchKernelGGL(k_pauli_terms<4>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
-                                       (int)parts, part);
-                else if (g.nt <= 8)
-                    hipLaunchKernelGGL(k_pauli_terms<8>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g, (int)rows,
-                                       (int)parts, part);
-                else if (g.nt <= 12)
-                    hipLaunchKernelGGL(k_pauli_terms<12>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
-                                       (int)rows, (int)parts, part);
-                else if (g.nt <= 16)
-                    hipLaunchKernelGGL(k_pauli_terms<16>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
-                                       (int)rows, (int)parts, part);
-                else if (g.nt <= 24)
-                    hipLaunchKernelGGL(k_pauli_terms<24>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
-                                       (int)rows, (int)parts, part);
-                else
-                    hipLaunchKernelGGL(k_pauli_terms<32>, grid, dim3(kBlock), 0, ctx->stream, amp, n, e0, g,
-                                       (int)rows, (int)parts, part);
+                const int rc = launch_terms(g, grid, ctx->stream, amp, n, e0, (int)rows, (int)parts, part);
+                if (rc) return rc;
                 SC_HIP_CHECK(hipGetLastError());
             }
         }

@@ -371,6 +371,30 @@ int main() {
         for (uint64_t i = 0; i < (1ULL << n); i++) ns += std::norm(st.amplitude(i));
         CHECK(std::fabs(ns - 1.0) <= EPS, "normalised");
     });
+    test("twentyQubitRyQftWithDefaultOptionsMatchesReference", [&] {
+        // the 20-qubit Ry layer + QFT workload (thetas 2 pi u from SplittableRandom(42)) through the engine with
+        // its default options (runtime-compiled passes, geometry chosen per circuit), 4096 shots, seed 42
+        const int n = 20;
+        SplittableRandom rng(42);
+        QuantumCircuit c(n);
+        for (int q = 0; q < n; q++) c = c.append(rotationY(2.0 * M_PI * rng.nextDouble()), q);
+        for (int t = n - 1; t >= 0; t--) {
+            c = c.append(Hadamard(), t);
+            for (int k = t - 1; k >= 0; k--) c = c.appendControlledPhase(M_PI / double(1 << (t - k)), k, t);
+        }
+        for (int q = 0; q < n / 2; q++) c = c.appendSwap(q, n - 1 - q);
+        std::vector<Complex> init(size_t(1) << n, Complex(0.0, 0.0));
+        init[0] = 1.0;
+        auto req = SimulationRequest::zeroState(c).withMeasurement(MeasurementInstruction::countsAll(4096).withSeed(42));
+        auto r = engine.simulate(req);
+        CHECK(matches(apply_scalar(c, init), *r.finalState), "20-qubit state");
+        const auto &h = r.measurement->counts;
+        int total = 0;
+        for (const auto &kv : h) total += kv.second;
+        CHECK(r.measurement->shots == 4096 && total == 4096, "4096 shots");
+        auto again = engine.simulate(req);
+        CHECK(again.measurement->counts == h, "same seed, same counts");
+    });
     test("rejectsQubitCountAboveDenseLimit", [&] {
         bool threw = false;
         try {
