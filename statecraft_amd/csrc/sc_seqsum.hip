@@ -280,6 +280,15 @@ double serial_run(double S, const double *v, uint64_t count) {
     return S;
 }
 
+// The binade of x >= 0: its unbiased exponent field (ilogb for normal x; -1023 for zero and subnormals, which
+// lie below every binade the chain uses).
+inline int binade_of(double x) {
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    return (int)((b >> 52) & 0x7FF) - 1023;
+}
+constexpr uint64_t kMant = (1ULL << 52) - 1ULL;
+
 }  // namespace
 
 int plan_sequences(Provider &pv, const Source &src, Plan &plan) {
@@ -298,7 +307,7 @@ int plan_sequences(Provider &pv, const Source &src, Plan &plan) {
             const uint64_t g = o * sps + k;
             const double a = approx[g];
             if (a != 0.0 && P > 0.0) {
-                const int e1 = std::ilogb(P), e2 = std::ilogb(P + a);
+                const int e1 = binade_of(P), e2 = binade_of(P + a);
                 if (e1 == e2 && e1 >= kMinBinade && e1 <= kMaxBinade) cand[g] = e1;
             }
             P += a;
@@ -342,14 +351,17 @@ int plan_sequences(Provider &pv, const Source &src, Plan &plan) {
                     continue;
                 }
                 const int e = cand[g];
-                if (e != kNoBinade && ties[g] == 0 && S > 0.0 && std::ilogb(S) == e && isum[g] >= 0.0 &&
+                if (e != kNoBinade && ties[g] == 0 && S > 0.0 && binade_of(S) == e && isum[g] >= 0.0 &&
                     isum[g] <= two53m1) {
-                    const double M = std::ldexp(S, 52 - e);  // exact integer in [2^52, 2^53)
-                    const uint64_t MR = (uint64_t)M + (uint64_t)isum[g];
+                    // S = M 2^(e-52) with the integer M in [2^52, 2^53): its significand
+                    uint64_t bits;
+                    std::memcpy(&bits, &S, 8);
+                    const uint64_t MR = ((bits & kMant) | (1ULL << 52)) + (uint64_t)isum[g];
                     if (MR <= (uint64_t)two53m1) {
                         plan.mode[g] = kFast;
                         plan.binade[g] = e;
-                        S = std::ldexp((double)MR, e - 52);
+                        bits = ((uint64_t)(e + 1023) << 52) | (MR & kMant);  // (M + R) 2^(e-52), exact
+                        std::memcpy(&S, &bits, 8);
                         fast++;
                         continue;
                     }
@@ -391,41 +403,55 @@ struct DeviceProvider : Provider {
     Context *ctx;
     Source src;
     uint64_t nseg;
+    // device workspace in the context's scratch (approx, candidates, integer sums, ties) and pinned staging in
+    // its host scratch: no allocation per call, and transfers at pinned-memory rates
+    double *d_approx = nullptr, *d_isum = nullptr;
+    int32_t *d_cand = nullptr;
+    uint32_t *d_ties = nullptr;
+    char *h = nullptr;
     DeviceProvider(Context *c, const Source &s) : ctx(c), src(s), nseg(s.nseq * s.segs_per_seq) {}
+    int init() {
+        const size_t bytes = (size_t)nseg * kWorkBytesPerSeg;
+        int rc = ensure_scratch(ctx, bytes);
+        if (rc) return rc;
+        if ((rc = ensure_host_scratch(ctx, bytes))) return rc;
+        d_approx = (double *)ctx->scratch;
+        d_isum = d_approx + nseg;
+        d_cand = (int32_t *)(d_isum + nseg);
+        d_ties = (uint32_t *)(d_cand + nseg);
+        h = (char *)ctx->scratch_host;
+        return SC_OK;
+    }
+    static constexpr size_t kWorkBytesPerSeg = 24;
 
     int approx(std::vector<double> &out) override {
-        DevBuf b;
-        int rc = b.alloc(nseg * 8);
+        int rc = init();
         if (rc) return rc;
         {
             ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)(src.nseq * src.len));
-            hipLaunchKernelGGL(k_seq_approx, dim3(grid_blocks(nseg)), dim3(kBlock), 0, ctx->stream, src,
-                               (double *)b.p);
+            hipLaunchKernelGGL(k_seq_approx, dim3(grid_blocks(nseg)), dim3(kBlock), 0, ctx->stream, src, d_approx);
             SC_HIP_CHECK(hipGetLastError());
         }
-        out.resize(nseg);
-        SC_HIP_CHECK(hipMemcpyAsync(out.data(), b.p, nseg * 8, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP_CHECK(hipMemcpyAsync(h, d_approx, nseg * 8, hipMemcpyDeviceToHost, ctx->stream));
         SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        out.assign((const double *)h, (const double *)h + nseg);
         return SC_OK;
     }
     int int_sums(const std::vector<int32_t> &cand, std::vector<double> &isum, std::vector<uint32_t> &ties) override {
-        DevBuf b;
-        int rc = b.alloc(nseg * 16);
-        if (rc) return rc;
-        int32_t *c = (int32_t *)b.p;
-        double *r = (double *)((char *)b.p + nseg * 8);
-        uint32_t *t = (uint32_t *)((char *)b.p + nseg * 4);
-        SC_HIP_CHECK(hipMemcpyAsync(c, cand.data(), nseg * 4, hipMemcpyHostToDevice, ctx->stream));
+        std::memcpy(h, cand.data(), nseg * 4);
+        SC_HIP_CHECK(hipMemcpyAsync(d_cand, h, nseg * 4, hipMemcpyHostToDevice, ctx->stream));
         {
             ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 16.0 * (double)(src.nseq * src.len));
-            hipLaunchKernelGGL(k_seq_int, dim3(grid_blocks(nseg)), dim3(kBlock), 0, ctx->stream, src, c, r, t);
+            hipLaunchKernelGGL(k_seq_int, dim3(grid_blocks(nseg)), dim3(kBlock), 0, ctx->stream, src, d_cand, d_isum,
+                               d_ties);
             SC_HIP_CHECK(hipGetLastError());
         }
-        isum.resize(nseg);
-        ties.resize(nseg);
-        SC_HIP_CHECK(hipMemcpyAsync(isum.data(), r, nseg * 8, hipMemcpyDeviceToHost, ctx->stream));
-        SC_HIP_CHECK(hipMemcpyAsync(ties.data(), t, nseg * 4, hipMemcpyDeviceToHost, ctx->stream));
+        // results into the staging buffer after the candidates' region
+        SC_HIP_CHECK(hipMemcpyAsync(h + nseg * 4, d_isum, nseg * 8, hipMemcpyDeviceToHost, ctx->stream));
+        SC_HIP_CHECK(hipMemcpyAsync(h + nseg * 12, d_ties, nseg * 4, hipMemcpyDeviceToHost, ctx->stream));
         SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        isum.assign((const double *)(h + nseg * 4), (const double *)(h + nseg * 4) + nseg);
+        ties.assign((const uint32_t *)(h + nseg * 12), (const uint32_t *)(h + nseg * 12) + nseg);
         return SC_OK;
     }
     int gather(const std::vector<uint64_t> &segs, std::vector<double> &vals) override {
@@ -625,16 +651,20 @@ int exact_sample(Context *ctx, const double2 *amp, int n, const int32_t *measure
     if (rc) return rc;
     const uint64_t nseg = src.segs_per_seq;
     const std::vector<double> ends = plan_ends(plan, nseg);
-    DevBuf b;
-    if ((rc = b.alloc(nseg * 21 + 64))) return rc;
-    double *d_start = (double *)b.p;
+    // the search tables through the context's pinned staging into its device scratch (the providers are done
+    // with both): start, ends, binade, mode
+    const size_t tbytes = (size_t)nseg * 21 + 64;
+    if ((rc = ensure_scratch(ctx, tbytes)) || (rc = ensure_host_scratch(ctx, tbytes))) return rc;
+    char *h = (char *)ctx->scratch_host;
+    std::memcpy(h, plan.start.data(), nseg * 8);
+    std::memcpy(h + nseg * 8, ends.data(), nseg * 8);
+    std::memcpy(h + nseg * 16, plan.binade.data(), nseg * 4);
+    std::memcpy(h + nseg * 20, plan.mode.data(), nseg);
+    double *d_start = (double *)ctx->scratch;
     double *d_ends = d_start + nseg;
     int32_t *d_bin = (int32_t *)(d_ends + nseg);
     uint8_t *d_mode = (uint8_t *)(d_bin + nseg);
-    SC_HIP_CHECK(hipMemcpyAsync(d_start, plan.start.data(), nseg * 8, hipMemcpyHostToDevice, ctx->stream));
-    SC_HIP_CHECK(hipMemcpyAsync(d_ends, ends.data(), nseg * 8, hipMemcpyHostToDevice, ctx->stream));
-    SC_HIP_CHECK(hipMemcpyAsync(d_bin, plan.binade.data(), nseg * 4, hipMemcpyHostToDevice, ctx->stream));
-    SC_HIP_CHECK(hipMemcpyAsync(d_mode, plan.mode.data(), nseg, hipMemcpyHostToDevice, ctx->stream));
+    SC_HIP_CHECK(hipMemcpyAsync(d_start, h, nseg * 21, hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
         const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)shots, (uint64_t)ctx->cu_count * 8);
@@ -643,7 +673,7 @@ int exact_sample(Context *ctx, const double2 *amp, int n, const int32_t *measure
                            (const uint8_t *)d_mode, u_dev, shots, out_dev, gap_dev);
         SC_HIP_CHECK(hipGetLastError());
     }
-    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // b is released on return
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the tables live in the context's scratch
     if (stats) {
         stats->fast = plan.fast;
         stats->serial = plan.serial;
