@@ -294,6 +294,15 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
                         uint64_t n_mat_complex, int low_qubits, int remap, int jit, int pass_cap, int chunk_bits,
                         uint64_t *passes, uint64_t *stages, uint64_t *fused_blocks, double *fp64_ops_per_amp,
                         double *state_inout);
+/* Host-only (no device needed): the product-state input of runtime-compiled passes from a basis state.  The
+ * circuit's leading single-qubit ops (uncontrolled, before any other op touches their qubit) are taken out
+ * of the plan; the first pass generates prod_q v_q[i_q], v_q those ops applied to bit q of `basis`, and the
+ * rest runs as planned (as a context does when sc_apply_ops starts from sc_state_set_basis).  state_out
+ * (2^n interleaved complex values) receives the result with the kernels' data movement; absorbed (may be
+ * NULL) the number of ops taken out.  No reference counterpart: it validates that path without a GPU. */
+int sc_plan_product_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                         uint64_t n_mat_complex, uint64_t basis, int low_qubits, int chunk_bits, int *absorbed,
+                         double *state_out);
 /* Host-only (no device needed): the geometry a context with these options picks for the circuit (chunk_bits 0
  * and / or low_qubits -1 leave it to the autotuner; see SC_OPT_CHUNK_BITS), as sc_last_plan_geometry reports. */
 int sc_plan_geometry_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
@@ -323,7 +332,7 @@ int sc_sample_host(int num_qubits, const double *state, const int32_t *measured,
                    int64_t shots, int64_t *outcomes, double *boundary_gap, uint64_t *stats);
 /* Host-only (no device needed): generates the runtime-compiled kernels of every pass of that plan (the
  * SC_OPT_JIT path, with chunk_bits local qubits per chunk; jit = 3: the adjoint passes of the circuit's
- * reverse sweep, always 12) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the
+ * reverse sweep, always 12; jit = 4: as 2 for the product-state plan from a basis state) and compiles them with hipRTC on `threads` threads.  Reports the kernel count, the
  * compile wall time and the code object size; when dump_dir is non-NULL each module's source (.hip) and
  * code object (.co) are written there. */
 int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,

@@ -84,7 +84,7 @@ EXPORTED = [
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
     "sc_last_sample_stats", "sc_cumulative_host", "sc_sample_host", "sc_state_max_deviation",
     "sc_expectation_pauli_terms_noalloc", "sc_plan_pass_stats_host", "sc_run_jobs", "sc_last_plan_geometry",
-    "sc_plan_geometry_host",
+    "sc_plan_geometry_host", "sc_plan_product_host",
 ]
 
 _lib = None
@@ -147,6 +147,7 @@ def _sig(lib):
         "sc_last_plan_geometry": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32), DP]),
         "sc_plan_geometry_host": (i32, [i32, P, u64, DP, u64, i32, i32, i32, i32, ctypes.POINTER(i32),
                                         ctypes.POINTER(i32), ctypes.POINTER(i32), DP]),
+        "sc_plan_product_host": (i32, [i32, P, u64, DP, u64, u64, i32, i32, ctypes.POINTER(i32), DP]),
         "sc_cumulative_host": (i32, [DP, u64, DP, U64P]),
         "sc_sample_host": (i32, [i32, DP, I32P, i32, DP, i64, I64P, DP, U64P]),
     }
@@ -236,6 +237,18 @@ def plan_geometry_host(n: int, ops, mats, low_qubits: int = -1, remap: bool = Fa
                                       int(low_qubits), 1 if remap else 0, int(jit), int(chunk_bits), ctypes.byref(cb),
                                       ctypes.byref(lq), ctypes.byref(nc), ctypes.byref(ns)))
     return cb.value, lq.value, nc.value, ns.value
+
+
+def plan_product_host(n: int, ops, mats, basis: int = 0, low_qubits: int = -1, chunk_bits: int = 0):
+    """Host-only: the circuit run from |basis> with its leading single-qubit ops as the first pass's
+    product-state input; returns (state, ops absorbed)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+    out = np.zeros(1 << n, dtype=np.complex128)
+    k = ctypes.c_int()
+    check(lib().sc_plan_product_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(basis),
+                                     int(low_qubits), int(chunk_bits), ctypes.byref(k), dptr(out.view(np.float64))))
+    return out, k.value
 
 
 def adjoint_plan_host(n: int, ops, mats, n_params: int, psi, lam, low_qubits: int = 4):

@@ -192,9 +192,19 @@ int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const do
 
 }  // namespace
 
-int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCache::Entry **out) {
+int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops_in, PlanCache::Entry **out,
+                     bool basis_input) {
     // ---- the plan (and its runtime-compiled kernels) of this circuit structure, cached per context
     const int max_fuse = std::min(ctx->max_fused, kRegBits);
+    // from a basis state, runtime-compiled passes generate the product state of the leading single-qubit ops
+    std::vector<PlanOp> rest;
+    std::vector<std::pair<int, int>> absorbed;
+    if (basis_input && ctx->jit) {
+        rest = ops_in;
+        absorb_leading_ops(n, rest, absorbed);
+        if (absorbed.size() == rest.size()) absorbed.clear();  // nothing else: the plain plan is as good
+    }
+    const std::vector<PlanOp> &ops = absorbed.empty() ? ops_in : rest;
     if (!ctx->plan_cache) ctx->plan_cache = new PlanCache();
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
@@ -202,8 +212,12 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCa
     // a fixed 13-bit chunk applies from 13 qubits with runtime-compiled passes; 0 leaves the choice to the planner
     int cb = ctx->chunk_bits;
     if (!ctx->jit || (cb == kMaxChunkBits && n < kMaxChunkBits)) cb = kChunkBits;
-    const std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
-                                           ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
+    std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
+                                     ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
+    if (!absorbed.empty()) {
+        key += "|product";
+        for (const auto &a : absorbed) key += " " + std::to_string(a.first) + ":" + std::to_string(a.second);
+    }
     PlanCache::Entry *ent = ctx->plan_cache->find(key);
     if (!ent) {
         std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());
@@ -211,6 +225,10 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCa
         int rc = plan_sweeps_geometry(n, ops, ctx->low_qubits, cb, ctx->remap != 0, ctx->jit != 0, e->plan, max_fuse,
                                       cm, &e->pick);
         if (rc) return rc;
+        if (!absorbed.empty()) {
+            if (e->plan.passes.empty()) return sweep_plan_entry(ctx, n, ops_in, out, false);
+            finalize_product(n, e->plan, absorbed);
+        }
         if (ctx->jit && !e->plan.passes.empty() && (rc = jit_pass_kernels(ctx, e->plan, e->fns))) return rc;
         ent = ctx->plan_cache->insert(std::move(e));
     }
@@ -232,7 +250,7 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
                const double2 *src = nullptr, double2 *dst = nullptr) {
     PlanCache::Entry *ent = nullptr;
     {
-        int rc = sweep_plan_entry(ctx, st->n, ops, &ent);
+        int rc = sweep_plan_entry(ctx, st->n, ops, &ent, st->pending_basis >= 0);
         if (rc) return rc;
     }
     const SweepPlan &plan = ent->plan;
@@ -272,7 +290,7 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     const unsigned nthreads = (unsigned)std::min<size_t>(
         {nelem_pool, pool_work >= 4096 ? (size_t)std::max(1u, std::thread::hardware_concurrency()) : 1, 16});
     auto build_range = [&](size_t e0, size_t e1) {
-        for (size_t e = e0; e < e1; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e);
+        for (size_t e = e0; e < e1; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e, basis);
     };
     if (nthreads <= 1) {
         build_range(0, nelem_pool);
@@ -969,7 +987,7 @@ int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     if (num_qubits < kChunkBits || num_qubits > 34) return fail(SC_E_ARG, "sweep plans need 12..34 qubits");
     if (low_qubits < 0 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8");
     if (threads < 1 || threads > 64) return fail(SC_E_ARG, "threads must be 1..64");
-    if (jit < 1 || jit > 3) return fail(SC_E_ARG, "jit must be 1, 2 or 3");
+    if (jit < 1 || jit > 4) return fail(SC_E_ARG, "jit must be 1, 2, 3 or 4");
     if (chunk_bits != kChunkBits && !(chunk_bits == kMaxChunkBits && jit != 3))
         return fail(SC_E_ARG, "chunk bits must be 12, or 13 for forward passes");
     SweepPlan plan;
@@ -986,12 +1004,19 @@ int sc_jit_compile_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
         std::vector<const double *> elems{mats};
         std::vector<PlanOp> pops;
         if ((rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops))) return rc;
+        std::vector<std::pair<int, int>> absorbed;
+        if (jit == 4) {  // the product-state plan from a basis state
+            absorb_leading_ops(num_qubits, pops, absorbed);
+            if (absorbed.empty() || absorbed.size() == pops.size())
+                return fail(SC_E_ARG, "no leading single-qubit ops to absorb");
+        }
         if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, kCompiledPassCost,
                               chunk_bits)))
             return rc;
+        if (jit == 4 && !plan.passes.empty()) finalize_product(num_qubits, plan, absorbed);
     }
     if (kernels) *kernels = plan.passes.size();
-    return jit_compile_host(plan, jit, threads, seconds, code_bytes, dump_dir);
+    return jit_compile_host(plan, jit == 4 ? 2 : jit, threads, seconds, code_bytes, dump_dir);
 }
 
 int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches, uint64_t *fused_blocks) {
@@ -1008,6 +1033,42 @@ int sc_last_plan_geometry(sc_ctx *ctx, int *chunk_bits, int *low_qubits, int *ca
     if (low_qubits) *low_qubits = ctx->last_low_qubits;
     if (candidates) *candidates = ctx->last_candidates;
     if (ns_per_amp) *ns_per_amp = ctx->last_ns_per_amp;
+    return SC_OK;
+}
+
+int sc_plan_product_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const double *mats,
+                         uint64_t n_mat_complex, uint64_t basis, int low_qubits, int chunk_bits, int *absorbed,
+                         double *state_out) {
+    if (n_ops > 0) SC_REQUIRE(ops, "ops");
+    if (n_mat_complex > 0) SC_REQUIRE(mats, "mats");
+    SC_REQUIRE(state_out, "state_out");
+    if (num_qubits < kChunkBits || num_qubits > 26) return fail(SC_E_ARG, "host product plans take 12..26 qubits");
+    if (num_qubits < 64 && (basis >> num_qubits) != 0) return fail(SC_E_ARG, "basis index out of range");
+    if (chunk_bits != 0 && chunk_bits != kChunkBits && chunk_bits != kMaxChunkBits)
+        return fail(SC_E_ARG, "chunk bits must be 12 or 13, or 0 (chosen per circuit)");
+    if (low_qubits < -1 || low_qubits > 8) return fail(SC_E_ARG, "low qubits must be 0..8, or -1 (chosen per circuit)");
+    std::vector<const double *> elems{mats};
+    std::vector<PlanOp> pops;
+    int rc = analyze_ops(num_qubits, ops, n_ops, elems, n_mat_complex, pops);
+    if (rc) return rc;
+    std::vector<std::pair<int, int>> abs;
+    absorb_leading_ops(num_qubits, pops, abs);
+    if (absorbed) *absorbed = (int)abs.size();
+    if (abs.empty() || abs.size() == pops.size())
+        return fail(SC_E_ARG, "no leading single-qubit ops to absorb, or nothing after them");
+    int cb = chunk_bits;
+    if (cb == kMaxChunkBits && num_qubits < kMaxChunkBits) cb = kChunkBits;
+    SweepPlan plan;
+    GeometryPick pick;
+    if ((rc = plan_sweeps_geometry(num_qubits, pops, low_qubits, cb, false, true, plan, kRegBits, kCompiledPassCost,
+                                   &pick)))
+        return rc;
+    if (plan.passes.empty()) return fail(SC_E_ARG, "the remaining ops need no pass");
+    finalize_product(num_qubits, plan, abs);
+    std::vector<double> pool(2 * std::max<uint64_t>(plan.pool_complex, 1));
+    build_pool(plan, mats, pool.data(), (long long)basis);
+    product_input_host(num_qubits, plan, pool.data(), state_out);
+    execute_plan_host(num_qubits, plan, pool.data(), state_out);
     return SC_OK;
 }
 

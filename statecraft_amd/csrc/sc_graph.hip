@@ -66,7 +66,8 @@ void build_pools(const sc_job &job, const GraphJob &gj, char *host) {
         {B, B * (gj.plan.pool.size() + 1) >= 4096 ? (uint64_t)std::max(1u, std::thread::hardware_concurrency()) : 1,
          16});
     auto range = [&](uint64_t e0, uint64_t e1) {
-        for (uint64_t e = e0; e < e1; e++) build_pool(gj.plan, job.mats + 2 * job.mat_stride * e, dst + 2 * pc * e);
+        for (uint64_t e = e0; e < e1; e++)
+            build_pool(gj.plan, job.mats + 2 * job.mat_stride * e, dst + 2 * pc * e, (long long)job.basis);
     };
     if (threads <= 1) {
         range(0, B);
@@ -214,7 +215,7 @@ int run_jobs(Context *ctx, const sc_job *jobs, int n_jobs, int *used_graph) {
     const int32_t geo[2] = {ctx->cu_count, ctx->sweep_blocks_per_cu};
     put(key, geo, sizeof geo);
     for (int j = 0; j < n_jobs; j++) {
-        int rc = sweep_plan_entry(ctx, jobs[j].state->n, pops[j], &ents[j]);
+        int rc = sweep_plan_entry(ctx, jobs[j].state->n, pops[j], &ents[j], true);
         if (rc) return rc;
         if (ents[j]->fns.empty()) return run_sequential(jobs, n_jobs);
         const State *st = jobs[j].state;

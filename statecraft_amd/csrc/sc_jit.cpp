@@ -181,6 +181,30 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
             s << "    " << dst << "[" << r << "] = (i0 + " << l_reg[r]
               << "ULL == (uint64_t)basis) ? make_v2d(1.0, 0.0) : make_v2d(0.0, 0.0);\n";
     };
+    // product plans: the first pass generates prod_q v_q[i_q] from the pool's tables (SweepPlan::product):
+    // the chunk's factor CL[low chunk bits] CH[high chunk bits], the thread's TL[low tid bits] TH[high tid
+    // bits], then one product per register with R[r]
+    const bool product = plan.product && &ps == &plan.passes[0];
+    const int ptl = plan.prod_tl, pth = TB - plan.prod_tl, ph = plan.prod_h;
+    const uint64_t oTL = kRegs, oTH = oTL + (1ULL << ptl), oCL = oTH + (1ULL << pth), oCH = oCL + (1ULL << ph);
+    // The thread factor only changes with the batch element: it is kept in registers (pel: its element),
+    // so that a chunk's generation waits on no vector load.
+    auto regs_product = [&](const char *dst, const char *cvar) {
+        s << "    {\n"
+             "    const uint64_t pe = (uint64_t)elem_begin + (" << cvar << " >> log_chunks);\n"
+             "    CV2 *pm = mats + pe * mat_stride + " << plan.prod_off << "ULL;\n"
+             "    if (pe != pel) {\n"
+             "      pel = pe;\n"
+             "      pt = cmul(pm[" << oTL << "u + (tid & " << ((1u << ptl) - 1u) << "u)], pm[" << oTH << "u + (tid >> "
+          << ptl << ")]);\n"
+             "    }\n"
+             "    const uint64_t pci = " << cvar << " & chunk_mask;\n"
+             "    const v2d pc = cmul(pm[" << oCL << "ULL + (pci & " << hex64((1ULL << ph) - 1ULL) << ")], pm[" << oCH
+          << "ULL + (pci >> " << ph << ")]);\n"
+             "    const v2d pct = cmul(pc, pt);\n";
+        for (int r = 0; r < kRegs; r++) s << "    " << dst << "[" << r << "] = cmul(pct, pm[" << r << "]);\n";
+        s << "    }\n";
+    };
     auto regs_store = [&](const char *dstp, const char *src) {
         for (int r = 0; r < kRegs; r++) s << "    " << dstp << "[" << s_reg[r] << "] = " << src << "[" << r << "];\n";
     };
@@ -200,9 +224,11 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     // the chunk's input: one register set; the prefetching variant loads the next chunk into a third one
     const char *in0 = prefetch ? "q" : "p";
     s << (prefetch ? "  Regs a, p, q;\n" : "  Regs a, p;\n");
+    if (product) s << "  uint64_t pel = ~0ULL;\n  v2d pt = make_v2d(0.0, 0.0);\n";
     s << "  uint64_t cb = cbase(c & chunk_mask);\n"
          "  if (basis >= 0) {\n";
-    regs_basis(in0, "cb");
+    if (product) regs_product(in0, "c");
+    else regs_basis(in0, "cb");
     s << "  } else {\n"
          "    const v2d *src = amp + (((uint64_t)elem_begin + (c >> log_chunks)) << n) + cb + opaque(gload);\n";
     regs_load(in0, "src");
@@ -214,7 +240,8 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "    const uint64_t ncb = next < c_end ? cbase(next & chunk_mask) : 0ULL;\n";
     auto next_loads = [&](const char *dst) {
         s << "    if (next < c_end && basis >= 0) {\n";
-        regs_basis(dst, "ncb");
+        if (product) regs_product(dst, "next");
+        else regs_basis(dst, "ncb");
         s << "    } else if (next < c_end) {\n"
              "    const v2d *src = amp + (((uint64_t)elem_begin + (next >> log_chunks)) << n) + ncb + opaque(gload);\n";
         regs_load(dst, "src");

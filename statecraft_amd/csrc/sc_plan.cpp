@@ -1225,6 +1225,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             std::vector<bool> sreal;  // op's matrix is real
             for (int oi : stages[s].ops) {
                 const PlanOp &op = ops[oi];
+                if (op.skip) continue;  // part of the product-state input
                 SweepOp so = new_op();
                 switch (op.kind) {
                 case kPlanGrad:
@@ -1713,7 +1714,7 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
         const int32_t f[4] = {op.kind, op.q0, op.q1, op.mat};
         const uint64_t m[3] = {op.controls, op.targets, op.diag};
         const uint8_t c = (uint8_t)((op.is_x ? 1 : 0) | (op.is_diag ? 2 : 0) | (op.is_real ? 4 : 0) | (op.gen << 3));
-        const uint8_t u = (uint8_t)((op.unit + 1) | (op.unit_form << 3));
+        const uint8_t u = (uint8_t)((op.unit + 1) | (op.unit_form << 3) | (op.skip ? 0x40 : 0));
         put(f, sizeof f);
         put(m, sizeof m);
         put(&c, 1);
@@ -1731,7 +1732,88 @@ void probe_factors(const SweepPlan &plan, const double *mats, std::vector<std::p
     }
 }
 
-void build_pool(const SweepPlan &plan, const double *mats, double *out) {
+namespace {
+
+// The first pass's input layout of a product plan: global positions of its register bits, thread-index bits
+// and chunk-index bits (the chunk index deposits into the outer positions in ascending order).
+struct ProductLayout {
+    int tb = 0, lc = 0, tl = 0, th = 0, h = 0, hh = 0;
+    int rpos[kRegBits] = {0};
+    int tpos[kMaxThreadBits] = {0};
+    std::vector<int> opos;
+};
+
+ProductLayout product_layout(int n, const SweepPlan &plan) {
+    ProductLayout L;
+    const SweepPass &ps = plan.passes[0];
+    const bool direct = ps.nstages > 0 && (ps.direct & 1) != 0;  // the kernel fills the first stage's layout
+    const uint64_t *lt = direct ? ps.ld_thread : ps.glob_thread, *lr = direct ? ps.ld_reg : ps.glob_reg;
+    L.tb = plan.chunk_bits - kRegBits;
+    for (int i = 0; i < L.tb; i++) L.tpos[i] = __builtin_ctzll(lt[i]);
+    for (int j = 0; j < kRegBits; j++) L.rpos[j] = __builtin_ctzll(lr[1 << j]);
+    for (int q = 0; q < n; q++)
+        if ((ps.outer_mask >> q) & 1ULL) L.opos.push_back(q);
+    L.lc = (int)L.opos.size();
+    L.tl = plan.prod_tl;
+    L.th = L.tb - L.tl;
+    L.h = plan.prod_h;
+    L.hh = L.lc - L.h;
+    return L;
+}
+
+inline void cmul_host(const double *m, const double *x, double *out) {
+    const double re = m[0] * x[0] - m[1] * x[1], im = m[0] * x[1] + m[1] * x[0];
+    out[0] = re;
+    out[1] = im;
+}
+
+}  // namespace
+
+void absorb_leading_ops(int n, std::vector<PlanOp> &ops, std::vector<std::pair<int, int>> &absorbed) {
+    std::vector<char> open((size_t)n, 1);
+    for (PlanOp &op : ops) {
+        if (op.kind == SC_OP_SINGLE && op.controls == 0 && op.q0 >= 0 && op.q0 < n && open[op.q0] && op.mat >= 0) {
+            op.skip = true;
+            absorbed.push_back({op.q0, op.mat});
+            continue;
+        }
+        uint64_t touched = op.targets | op.diag | op.controls;
+        if (op.q0 >= 0) touched |= 1ULL << op.q0;
+        if (op.q1 >= 0) touched |= 1ULL << op.q1;
+        for (int q = 0; q < n; q++)
+            if ((touched >> q) & 1ULL) open[q] = 0;
+    }
+}
+
+void finalize_product(int n, SweepPlan &plan, const std::vector<std::pair<int, int>> &absorbed) {
+    plan.product = true;
+    plan.product_ops = absorbed;
+    const int tb = plan.chunk_bits - kRegBits, lc = n - plan.chunk_bits;
+    plan.prod_tl = tb / 2;
+    plan.prod_h = lc / 2;
+    plan.prod_off = plan.pool_complex;
+    plan.pool_complex += (uint64_t)kRegs + (1ULL << plan.prod_tl) + (1ULL << (tb - plan.prod_tl)) +
+                         (1ULL << plan.prod_h) + (1ULL << (lc - plan.prod_h));
+}
+
+void product_input_host(int n, const SweepPlan &plan, const double *pool, double *state) {
+    const ProductLayout L = product_layout(n, plan);
+    const double *R = pool + 2 * plan.prod_off, *TL = R + 2 * kRegs, *TH = TL + 2 * (1 << L.tl);
+    const double *CL = TH + 2 * (1 << L.th), *CH = CL + 2 * (1ULL << L.h);
+    for (uint64_t i = 0; i < (1ULL << n); i++) {
+        uint64_t ci = 0, t = 0, r = 0;
+        for (int k = 0; k < L.lc; k++) ci |= ((i >> L.opos[k]) & 1ULL) << k;
+        for (int k = 0; k < L.tb; k++) t |= ((i >> L.tpos[k]) & 1ULL) << k;
+        for (int k = 0; k < kRegBits; k++) r |= ((i >> L.rpos[k]) & 1ULL) << k;
+        double pc[2], pt[2], pct[2];
+        cmul_host(CL + 2 * (ci & ((1ULL << L.h) - 1ULL)), CH + 2 * (ci >> L.h), pc);
+        cmul_host(TL + 2 * (t & ((1ULL << L.tl) - 1ULL)), TH + 2 * (t >> L.tl), pt);
+        cmul_host(pc, pt, pct);
+        cmul_host(pct, R + 2 * r, state + 2 * i);
+    }
+}
+
+void build_pool(const SweepPlan &plan, const double *mats, double *out, long long basis) {
     // first pass: plain recipes (kinds 0, 1) at their offsets
     std::vector<uint64_t> offs(plan.pool.size());
     uint64_t off = 0;
@@ -1820,6 +1902,37 @@ void build_pool(const SweepPlan &plan, const double *mats, double *out) {
         double *dst = out + 2 * offs[r];
         for (int k = 0; k < 2 * rc.count; k++) dst[k] *= f;
     }
+    if (!plan.product || plan.passes.empty()) return;
+    // product input: v_q = the absorbed ops on qubit q applied to the basis state's bit, then the tables
+    int n = plan.chunk_bits;
+    for (int q = 0; q < 64; q++)
+        if ((plan.passes[0].outer_mask >> q) & 1ULL) n++;
+    std::vector<double> v((size_t)4 * n, 0.0);
+    for (int q = 0; q < n; q++) v[4 * q + 2 * (int)(((uint64_t)basis >> q) & 1ULL)] = 1.0;
+    for (const auto &a : plan.product_ops) {
+        const double *g = mats + 2 * (int64_t)a.second;
+        double *x = &v[4 * a.first];
+        double y0[2], y1[2], t[2];
+        cmul_host(g, x, y0), cmul_host(g + 2, x + 2, t), y0[0] += t[0], y0[1] += t[1];
+        cmul_host(g + 4, x, y1), cmul_host(g + 6, x + 2, t), y1[0] += t[0], y1[1] += t[1];
+        x[0] = y0[0], x[1] = y0[1], x[2] = y1[0], x[3] = y1[1];
+    }
+    const ProductLayout L = product_layout(n, plan);
+    double *R = out + 2 * plan.prod_off, *TL = R + 2 * kRegs, *TH = TL + 2 * (1 << L.tl);
+    double *CL = TH + 2 * (1 << L.th), *CH = CL + 2 * (1ULL << L.h);
+    // entry e of a table over positions pos[0..k): prod_j v_{pos[j]}[bit j of e], multiplied in order of j
+    auto table = [&](const int *pos, int k, double *dst) {
+        for (uint64_t e = 0; e < (1ULL << k); e++) {
+            double acc[2] = {1.0, 0.0};
+            for (int j = 0; j < k; j++) cmul_host(acc, &v[4 * pos[j] + 2 * (int)((e >> j) & 1ULL)], acc);
+            dst[2 * e] = acc[0], dst[2 * e + 1] = acc[1];
+        }
+    };
+    table(L.rpos, kRegBits, R);
+    table(L.tpos, L.tl, TL);
+    table(L.tpos + L.tl, L.th, TH);
+    table(L.opos.data(), L.h, CL);
+    table(L.opos.data() + L.h, L.hh, CH);
 }
 
 }  // namespace sc

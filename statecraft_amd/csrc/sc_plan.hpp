@@ -29,6 +29,9 @@ struct PlanOp {
     // 2 purely imaginary off-pivot entries.
     int unit = -1;
     int unit_form = 0;
+    // Absorbed into a product-state input (absorb_leading_ops): the op keeps its place in the plan, so the
+    // passes and stages are those of the full circuit, but emits no register op.
+    bool skip = false;
 };
 
 // Adjoint passes: a probe that accumulates <lambda| G |psi> for the Pauli generator G on q0 (and q1 for ZZ).
@@ -97,6 +100,15 @@ struct SweepPlan {
     // ops its pass ran before it.  Both states then carry the factor 1 / prod(pivots) (the pass's scaling op
     // comes last), so the probe's sum is prod(pivots)^2 times too small.
     std::vector<std::pair<uint64_t, std::vector<std::pair<int, int>>>> probe_pivots;
+    // Product-state input (runtime-compiled passes from a basis state): the circuit's leading single-qubit ops
+    // were taken out of the plan (absorb_leading_ops), and the first pass generates amp(i) = prod_q v_q[i_q]
+    // with v_q those ops applied to the basis state's qubit q.  The pool then ends in the tables the first
+    // pass multiplies (finalize_product): R[16] over its register index, TL / TH over the low / high
+    // thread-index bits, CL / CH over the low / high chunk-index bits.
+    bool product = false;
+    std::vector<std::pair<int, int>> product_ops;  // absorbed (qubit, caller matrix offset), circuit order
+    uint64_t prod_off = 0;                          // pool offset of R
+    int prod_tl = 0, prod_h = 0;                    // thread-index bits in TL, chunk-index bits in CL
 };
 // prod(pivots)^2 of every probe of a plan for one matrix array: (slot, factor) pairs.
 void probe_factors(const SweepPlan &plan, const double *mats, std::vector<std::pair<uint64_t, double>> &out);
@@ -172,10 +184,24 @@ std::string plan_cache_key(int n, const std::vector<PlanOp> &ops, int low_qubits
 // The plan (and runtime-compiled pass kernels) of an analysed circuit on n qubits under the context's options,
 // from the context's plan cache (planned and compiled on a miss).  The entry stays valid until 16 other
 // structures have been planned after it.
-int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCache::Entry **out);
+// basis_input: the state is a pending basis state, so with runtime-compiled passes the leading single-qubit
+// ops become a product-state input generated by the first pass (SweepPlan::product).
+int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops, PlanCache::Entry **out,
+                     bool basis_input = false);
 
 // Build the matrix pool of a plan for one caller matrix array into out (pool_complex complex values).
-void build_pool(const SweepPlan &plan, const double *mats, double *out);
+// basis: the basis-state input of a product plan (ignored otherwise).
+void build_pool(const SweepPlan &plan, const double *mats, double *out, long long basis = -1);
+
+// Leading single-qubit ops of a circuit that starts from a basis state: every uncontrolled single-qubit op
+// before any other op touches its qubit.  They are marked `skip` and listed as (qubit, matrix offset).
+void absorb_leading_ops(int n, std::vector<PlanOp> &ops, std::vector<std::pair<int, int>> &absorbed);
+// Makes a plan of the remaining ops a product plan: records the absorbed ops and appends the table space to
+// its pool.  The plan must have a pass.
+void finalize_product(int n, SweepPlan &plan, const std::vector<std::pair<int, int>> &absorbed);
+// The first pass's generated input of a product plan on the host, with the kernel's table products
+// (pool built by build_pool with the basis): 2^n interleaved complex values.
+void product_input_host(int n, const SweepPlan &plan, const double *pool, double *state);
 // Execute a sweep plan on a host array with exactly the kernel's data movement (load phase, swizzled LDS
 // slots, per-thread register sets, stage ops, store phase).  Used to validate plans on the CPU.  With lam, an
 // adjoint plan runs every op on both states and adds each probe's <lambda| G |psi> to grads[2 * slot].

@@ -369,3 +369,56 @@ def test_autotuned_plan_matches_oracle(n):
     _, got = capi.plan_sweeps_host(n, ops, mats, low_qubits=lq, state=s0, remap=False, jit=True, chunk_bits=cb)
     want = oracle.run_ops(n, ops, mats, s0.copy())
     assert np.max(np.abs(got - want)) <= 1e-12
+
+
+@pytest.mark.parametrize("family,n,basis,geom", [
+    ("layered", 14, 0, (0, -1)), ("layered", 16, 0b1011000000101, (12, 4)), ("layered", 15, 5, (13, 3)),
+    ("grover", 14, 0, (0, -1)), ("qft", 13, 0, (0, -1)), ("qaoa", 17, 0, (13, 4)), ("hea", 14, 3, (12, 3)),
+])
+def test_product_state_input_matches_oracle(family, n, basis, geom):
+    """From a basis state the leading single-qubit ops become the first pass's product-state input."""
+    if family == "layered":
+        circ = workloads.random_layered(n, 4, n)
+    elif family == "grover":
+        circ = workloads.grover(n, 2, 7)[0]
+    elif family == "qft":
+        circ = workloads.ry_qft(n, 42)[0]
+    elif family == "qaoa":
+        n, _, ops, mats, _ = next(b for b in workloads.qaoa_batches(64, 4, 4) if b[0] == n)
+        circ = None
+    else:
+        circ = workloads.hea(n, 2, np.random.default_rng(n).random(4 * n) * 6.28)
+    if circ is not None:
+        ops, mats = lower(circ)
+    else:
+        mats = mats[0]
+    got, absorbed = capi.plan_product_host(n, ops, mats, basis, geom[1], geom[0])
+    assert absorbed >= 1
+    s0 = np.zeros(1 << n, complex)
+    s0[basis] = 1.0
+    want = oracle.run_ops(n, ops, mats, s0)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_product_state_absorbs_only_leading_single_qubit_ops():
+    c = QuantumCircuit(13).append(Hadamard(), 0).append(rotation_y(0.3), 0).append(PauliZ(), 5)
+    c = c.append_cnot(0, 1).append(Hadamard(), 0).append(Hadamard(), 2).append(rotation_z(0.7), 1)
+    ops, mats = lower(c)
+    got, absorbed = capi.plan_product_host(13, ops, mats, 0b10)
+    assert absorbed == 4  # H, Ry on 0, Z on 5, H on 2; not the H on 0 after the CNOT nor Rz on 1
+    s0 = np.zeros(1 << 13, complex)
+    s0[0b10] = 1.0
+    assert np.max(np.abs(got - oracle.run_ops(13, ops, mats, s0))) <= 1e-12
+    with pytest.raises(capi.IllegalArgumentException):  # nothing left after the absorbed ops
+        capi.plan_product_host(13, *lower(QuantumCircuit(13).append(Hadamard(), 3)))
+
+
+@pytest.mark.parametrize("n,chunk_bits", [(14, 12), (30, 12), (33, 13)])
+def test_product_state_first_pass_compiles(n, chunk_bits, tmp_path):
+    """The first pass of a product-state plan (tables in the pool, no input read) compiles for gfx950."""
+    ops, mats = lower(workloads.random_layered(n, 2, 3) if n < 33 else workloads.grover(n, 1, 33)[0])
+    r = capi.jit_compile_host(n, ops, mats, low_qubits=4 if chunk_bits == 12 else 3, remap=False, threads=4,
+                              dump_dir=str(tmp_path), variant=4, chunk_bits=chunk_bits)
+    assert r["kernels"] >= 1 and r["code_bytes"] > 0
+    src = (tmp_path / "sc_passes_0.hip").read_text()
+    assert "const v2d pct = cmul(pc, pt);" in src
