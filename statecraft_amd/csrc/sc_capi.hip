@@ -783,7 +783,7 @@ int sc_sample(sc_state *st, int elem, const int32_t *measured, int m, const doub
     SC_HIP_CHECK(hipMemcpyAsync(u_dev, uniforms, (size_t)shots * 8, hipMemcpyHostToDevice, ctx->stream));
     seq::SampleStats stats;
     if ((rc = seq::exact_sample(ctx, st->amp + ((uint64_t)elem << st->n), st->n, measured, m,
-                                identity ? nullptr : (const double *)p.p, u_dev, shots, o_dev,
+                                identity ? nullptr : (const double *)p.p, uniforms, u_dev, shots, o_dev,
                                 boundary_gap ? g_dev : nullptr, &stats)))
         return rc;
     ctx->last_sample[0] = stats.fast;

@@ -162,31 +162,26 @@ __global__ void __launch_bounds__(kBlock) k_seq_gather(Source s, const uint64_t 
     }
 }
 
-// One workgroup per shot (grid-stride): the first segment whose end reaches u, then the smallest index in it
-// whose exact running sum reaches u (the last index of the sequence counts as 1.0).  out[s] = -1 when no
-// index qualifies, which the exact plan rules out; the host reports it as an internal error.
+// One workgroup per segment that holds shots (grid-stride over them; the host found each shot's segment, the
+// first whose exact end reaches u, and grouped the shots by segment): the segment's exact running sums once,
+// then for each of its shots the smallest index whose running sum reaches u (the last index of the sequence
+// counts as 1.0).  out[s] = -1 when no index qualifies, which the exact plan rules out; the host reports it as
+// an internal error.
 __global__ void __launch_bounds__(kBlock) k_seq_search(Source s, uint64_t nseg, const double *__restrict__ start,
                                                         const double *__restrict__ ends,
                                                         const int32_t *__restrict__ binade,
                                                         const uint8_t *__restrict__ mode,
-                                                        const double *__restrict__ u, int64_t shots,
+                                                        const double *__restrict__ u,
+                                                        const uint64_t *__restrict__ segs,
+                                                        const uint64_t *__restrict__ seg_begin,
+                                                        const int64_t *__restrict__ order, uint64_t nsegs,
                                                         int64_t *__restrict__ out, double *__restrict__ gap) {
     __shared__ double vals[kSeg];
     __shared__ double tsum[kBlock];
     __shared__ uint64_t best[kBlock];
     const uint64_t last = s.len - 1;
-    for (int64_t shot = blockIdx.x; shot < shots; shot += gridDim.x) {
-        const double value = u[shot];
-        uint64_t g = nseg - 1;
-        if (value < 1.0) {
-            uint64_t lo = 0, hi = nseg - 1;  // smallest g with ends[g] >= value (ends[nseg - 1] = 1.0)
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                if (ends[mid] >= value) hi = mid;
-                else lo = mid + 1;
-            }
-            g = lo;
-        }
+    for (uint64_t b = blockIdx.x; b < nsegs; b += gridDim.x) {
+        const uint64_t g = segs[b];
         const uint64_t base = g * kSeg;
         const double S0 = start[g];
         const int md = mode[g];
@@ -237,36 +232,44 @@ __global__ void __launch_bounds__(kBlock) k_seq_search(Source s, uint64_t nseg, 
 #pragma unroll
             for (int k = 0; k < kPer; k++) cum[k] = S0;
         }
-        uint64_t cand = ~0ULL;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-            const uint64_t i = base + (uint64_t)(j0 + k);
-            if (i == last) cum[k] = 1.0;
-            if (i <= last && cand == ~0ULL && (value >= 1.0 ? i == last : cum[k] >= value)) cand = i;
-        }
+        for (int k = 0; k < kPer; k++)
+            if (base + (uint64_t)(j0 + k) == last) cum[k] = 1.0;
         __syncthreads();  // every thread has read vals
 #pragma unroll
         for (int k = 0; k < kPer; k++) vals[j0 + k] = cum[k];
-        best[threadIdx.x] = cand;
         __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w && best[threadIdx.x + w] < best[threadIdx.x]) best[threadIdx.x] = best[threadIdx.x + w];
+        for (uint64_t t = seg_begin[b]; t < seg_begin[b + 1]; t++) {
+            const int64_t shot = order[t];
+            const double value = u[shot];
+            uint64_t cand = ~0ULL;
+#pragma unroll
+            for (int k = 0; k < kPer; k++) {
+                const uint64_t i = base + (uint64_t)(j0 + k);
+                if (i <= last && cand == ~0ULL && (value >= 1.0 ? i == last : cum[k] >= value)) cand = i;
+            }
+            best[threadIdx.x] = cand;
+            __syncthreads();
+            for (int w = kBlock / 2; w > 0; w >>= 1) {
+                if ((int)threadIdx.x < w && best[threadIdx.x + w] < best[threadIdx.x])
+                    best[threadIdx.x] = best[threadIdx.x + w];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                const uint64_t r = best[0];
+                out[shot] = r == ~0ULL ? -1 : (int64_t)r;
+                if (gap) {
+                    double gv = 0.0;
+                    if (r != ~0ULL) {
+                        gv = fabs(value - vals[r - base]);
+                        const double prev = r > base ? vals[r - 1 - base] : (g > 0 ? ends[g - 1] : -1.0);
+                        if (prev >= 0.0) gv = fmin(gv, fabs(value - prev));
+                    }
+                    gap[shot] = gv;
+                }
+            }
             __syncthreads();
         }
-        if (threadIdx.x == 0) {
-            const uint64_t r = best[0];
-            out[shot] = r == ~0ULL ? -1 : (int64_t)r;
-            if (gap) {
-                double gv = 0.0;
-                if (r != ~0ULL) {
-                    gv = fabs(value - vals[r - base]);
-                    const double prev = r > base ? vals[r - 1 - base] : (g > 0 ? ends[g - 1] : -1.0);
-                    if (prev >= 0.0) gv = fmin(gv, fabs(value - prev));
-                }
-                gap[shot] = gv;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -464,9 +467,13 @@ struct DeviceProvider : Provider {
         SC_HIP_CHECK(hipMemcpyAsync(l, segs.data(), k * 8, hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(k_seq_gather, dim3((unsigned)k), dim3(kBlock), 0, ctx->stream, src, l, v);
         SC_HIP_CHECK(hipGetLastError());
-        vals.resize(k * kSeg);
-        SC_HIP_CHECK(hipMemcpyAsync(vals.data(), v, k * kSeg * 8, hipMemcpyDeviceToHost, ctx->stream));
+        // through the pinned staging, after the provider's own region (its contents were copied out already)
+        const size_t off = (size_t)nseg * kWorkBytesPerSeg;
+        if ((rc = ensure_host_scratch(ctx, off + k * kSeg * 8))) return rc;
+        h = (char *)ctx->scratch_host;
+        SC_HIP_CHECK(hipMemcpyAsync(h + off, v, k * kSeg * 8, hipMemcpyDeviceToHost, ctx->stream));
         SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        vals.assign((const double *)(h + off), (const double *)(h + off) + k * kSeg);
         return SC_OK;
     }
 };
@@ -643,7 +650,8 @@ int exact_marginals(Context *ctx, const double2 *amp, int n, const int32_t *meas
 }
 
 int exact_sample(Context *ctx, const double2 *amp, int n, const int32_t *measured, int m, const double *marginals_dev,
-                 const double *u_dev, int64_t shots, int64_t *out_dev, double *gap_dev, SampleStats *stats) {
+                 const double *u_host, const double *u_dev, int64_t shots, int64_t *out_dev, double *gap_dev,
+                 SampleStats *stats) {
     Source src = marginals_dev ? make_array_source(marginals_dev, 1ULL << m)
                                : make_state_source((const double *)amp, n, measured, m, false);
     Plan plan;
@@ -651,26 +659,62 @@ int exact_sample(Context *ctx, const double2 *amp, int n, const int32_t *measure
     if (rc) return rc;
     const uint64_t nseg = src.segs_per_seq;
     const std::vector<double> ends = plan_ends(plan, nseg);
+    // each shot's segment: the first whose exact end reaches u (binary search over the ends, the last one 1.0),
+    // and the shots grouped by segment so that each segment's running sums are formed once
+    std::vector<std::pair<uint64_t, int64_t>> by_seg((size_t)shots);
+    for (int64_t t = 0; t < shots; t++) {
+        const double value = u_host[t];
+        uint64_t g = nseg - 1;
+        if (value < 1.0) {
+            uint64_t lo = 0, hi = nseg - 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                if (ends[mid] >= value) hi = mid;
+                else lo = mid + 1;
+            }
+            g = lo;
+        }
+        by_seg[(size_t)t] = {g, t};
+    }
+    std::sort(by_seg.begin(), by_seg.end());
+    std::vector<uint64_t> segs, seg_begin;
+    for (size_t t = 0; t < by_seg.size(); t++)
+        if (t == 0 || by_seg[t].first != by_seg[t - 1].first) {
+            segs.push_back(by_seg[t].first);
+            seg_begin.push_back(t);
+        }
+    seg_begin.push_back(by_seg.size());
+    const uint64_t nsegs = segs.size();
     // the search tables through the context's pinned staging into its device scratch (the providers are done
-    // with both): start, ends, binade, mode
-    const size_t tbytes = (size_t)nseg * 21 + 64;
-    if ((rc = ensure_scratch(ctx, tbytes)) || (rc = ensure_host_scratch(ctx, tbytes))) return rc;
+    // with both): start, ends, binade, mode, then the shot order, the segments and their first shots
+    const size_t tbytes = ((size_t)nseg * 21 + 255) & ~(size_t)255;
+    const size_t obytes = (size_t)shots * 8, sbytes = nsegs * 8, bbytes = (nsegs + 1) * 8;
+    const size_t all = tbytes + obytes + sbytes + bbytes;
+    if ((rc = ensure_scratch(ctx, all)) || (rc = ensure_host_scratch(ctx, all))) return rc;
     char *h = (char *)ctx->scratch_host;
     std::memcpy(h, plan.start.data(), nseg * 8);
     std::memcpy(h + nseg * 8, ends.data(), nseg * 8);
     std::memcpy(h + nseg * 16, plan.binade.data(), nseg * 4);
     std::memcpy(h + nseg * 20, plan.mode.data(), nseg);
-    double *d_start = (double *)ctx->scratch;
+    int64_t *h_order = (int64_t *)(h + tbytes);
+    for (size_t t = 0; t < by_seg.size(); t++) h_order[t] = by_seg[t].second;
+    std::memcpy(h + tbytes + obytes, segs.data(), sbytes);
+    std::memcpy(h + tbytes + obytes + sbytes, seg_begin.data(), bbytes);
+    char *d = (char *)ctx->scratch;
+    double *d_start = (double *)d;
     double *d_ends = d_start + nseg;
     int32_t *d_bin = (int32_t *)(d_ends + nseg);
     uint8_t *d_mode = (uint8_t *)(d_bin + nseg);
-    SC_HIP_CHECK(hipMemcpyAsync(d_start, h, nseg * 21, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t *d_order = (const int64_t *)(d + tbytes);
+    const uint64_t *d_segs = (const uint64_t *)(d + tbytes + obytes);
+    const uint64_t *d_begin = (const uint64_t *)(d + tbytes + obytes + sbytes);
+    SC_HIP_CHECK(hipMemcpyAsync(d, h, all, hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_MEASURE, 0.0);
-        const unsigned grid = (unsigned)std::min<uint64_t>((uint64_t)shots, (uint64_t)ctx->cu_count * 8);
+        const unsigned grid = (unsigned)std::min<uint64_t>(nsegs, (uint64_t)ctx->cu_count * 8);
         hipLaunchKernelGGL(k_seq_search, dim3(grid), dim3(kBlock), 0, ctx->stream, src, nseg,
                            (const double *)d_start, (const double *)d_ends, (const int32_t *)d_bin,
-                           (const uint8_t *)d_mode, u_dev, shots, out_dev, gap_dev);
+                           (const uint8_t *)d_mode, u_dev, d_segs, d_begin, d_order, nsegs, out_dev, gap_dev);
         SC_HIP_CHECK(hipGetLastError());
     }
     SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the tables live in the context's scratch

@@ -63,8 +63,10 @@ int exact_marginals(Context *ctx, const double2 *amp, int n, const int32_t *meas
 
 // normalize + cumulative + sampleOutcome (:298-338) for every uniform, exactly: over the whole state in index
 // order when marginals_dev is null (every qubit measured in order), else over the 2^m marginals_dev values.
+// u_host and u_dev hold the same uniforms (the host groups the shots by segment).
 int exact_sample(Context *ctx, const double2 *amp, int n, const int32_t *measured, int m, const double *marginals_dev,
-                 const double *u_dev, int64_t shots, int64_t *out_dev, double *gap_dev, SampleStats *stats);
+                 const double *u_host, const double *u_dev, int64_t shots, int64_t *out_dev, double *gap_dev,
+                 SampleStats *stats);
 
 }  // namespace seq
 }  // namespace sc
