@@ -11,6 +11,7 @@
 #include "sc_plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -347,25 +348,6 @@ struct StageBuild {
     uint64_t R = 0;  // global qubits that are register bits (targets)
     std::vector<int> ops;
 };
-
-// Rank over GF(2) of the given 4-bit (or narrower) column vectors.
-int gf2_rank(std::vector<uint32_t> cols) {
-    int rank = 0;
-    for (int bit = 3; bit >= 0; bit--) {
-        int piv = -1;
-        for (size_t i = rank; i < cols.size(); i++)
-            if ((cols[i] >> bit) & 1u) {
-                piv = (int)i;
-                break;
-            }
-        if (piv < 0) continue;
-        std::swap(cols[rank], cols[piv]);
-        for (size_t i = 0; i < cols.size(); i++)
-            if ((int)i != rank && ((cols[i] >> bit) & 1u)) cols[i] ^= cols[rank];
-        rank++;
-    }
-    return rank;
-}
 
 // Mirror of the device run_op (sc_sweep.hip): applies one register op to the 16 complex values (32 doubles)
 // of thread `tid` in the chunk whose outer offset is cb.  m is the op's matrix / factor table.  SW_DIAGL may
@@ -1075,8 +1057,11 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             }
             pass.wave_local = pass_wave_local;
         }
-        // ---- swizzle: hv[p] for local bits p >= kSwzBits, chosen so every stage's lanes 0..3 (and 0..2)
-        // land on distinct low slot bits.
+        // ---- swizzle: slot(l) = l ^ (XOR of hv[p] over the set bits p of l), hv[p] below bit min(p, kSwzBits)
+        // (a unitriangular map: one slot per local index), chosen so every stage's lanes 0..3 land on distinct
+        // slots mod 16 and its lanes 0..2 on distinct slots mod 8: a 16-byte LDS access then serves 8 (or 16)
+        // lanes per cycle without bank conflicts.  Low positions take part too: a stage whose lane bits 0-2
+        // include position 3 (slot bit 3, 0 mod 8) needs hv[3] to keep its 8-lane groups apart.
         uint32_t hv[kMaxChunkBits] = {0};
         {
             uint64_t rng = 0x243F6A8885A308D3ULL;
@@ -1086,22 +1071,41 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 rng ^= rng << 17;
                 return rng;
             };
+            // lane bits 0..3 of every stage, then of the store phase
+            std::vector<std::array<int, 4>> lanes;
+            for (size_t s = 0; s <= stages.size(); s++) {
+                std::array<int, 4> a;
+                for (int i = 0; i < 4; i++) a[i] = s < stages.size() ? layouts[s].tpos[i] : pi[i];
+                lanes.push_back(a);
+            }
+            // columns independent over GF(2): no nonempty subset XORs to zero (reachable sums as a 16-bit set)
+            auto independent = [](const uint32_t *c, int k) {
+                uint32_t reach = 1u;  // the empty sum
+                for (int i = 0; i < k; i++) {
+                    uint32_t moved = 0;
+                    for (uint32_t v = 0; v < 16; v++)
+                        if ((reach >> v) & 1u) moved |= 1u << (v ^ c[i]);
+                    if (moved & reach) return false;
+                    reach |= moved;
+                }
+                return true;
+            };
             int best_bad = 1 << 30;
             uint32_t best[kMaxChunkBits] = {0};
-            for (int attempt = 0; attempt < 4000 && best_bad > 0; attempt++) {
+            for (int attempt = 0; attempt < 100000 && best_bad > 0; attempt++) {
                 uint32_t cand[kMaxChunkBits] = {0};
-                for (int p = kSwzBits; p < CB; p++) cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & 15u);
+                for (int p = 1; p < CB; p++)
+                    cand[p] = attempt == 0 ? 0u : (uint32_t)(next_rand() & ((1u << std::min(p, kSwzBits)) - 1u));
                 int bad = 0;
-                for (size_t s = 0; s <= stages.size(); s++) {  // every stage, then the store phase
-                    std::vector<uint32_t> c16, c8;
+                for (const auto &a : lanes) {
+                    uint32_t c16[4], c8[3];
                     for (int i = 0; i < 4; i++) {
-                        int p = s < stages.size() ? layouts[s].tpos[i] : pi[i];
-                        uint32_t col = p < kSwzBits ? (1u << p) : cand[p];
-                        c16.push_back(col);
-                        if (i < 3) c8.push_back(col & 7u);
+                        c16[i] = ((a[i] < kSwzBits ? (1u << a[i]) : 0u) ^ cand[a[i]]) & 15u;
+                        if (i < 3) c8[i] = c16[i] & 7u;
                     }
-                    if (gf2_rank(c16) < 4) bad += 2;
-                    if (gf2_rank(c8) < 3) bad += 1;
+                    if (!independent(c16, 4)) bad += 2;
+                    if (!independent(c8, 3)) bad += 1;
+                    if (bad >= best_bad) break;
                 }
                 if (bad < best_bad) {
                     best_bad = bad;
@@ -1112,7 +1116,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         }
         auto slot_of = [&](uint32_t l) -> uint32_t {
             uint32_t h = 0;
-            for (int p = kSwzBits; p < CB; p++)
+            for (int p = 1; p < CB; p++)
                 if ((l >> p) & 1u) h ^= hv[p];
             return l ^ h;
         };
