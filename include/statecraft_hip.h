@@ -117,6 +117,15 @@ typedef struct sc_op {
 #define SC_OPT_DIRECT_IO 11     /* 1: a runtime-compiled pass whose first (last) stage keeps the chunk's low positions
                                    on lane bits reads HBM straight into (writes it straight from) that stage's
                                    registers, skipping an LDS round trip; 0: always through LDS; default 1 */
+#define SC_OPT_MFMA 12          /* fused mode: dense blocks run as complex GEMMs on the FP64 matrix cores
+                                   (v_mfma_f64_16x16x4_f64) instead of FP64 VALU multiply-adds; 0: never; 1: 5-qubit
+                                   blocks (2.5x faster than the VALU kernel on MI355X); 2: 4- and 5-qubit blocks;
+                                   default 1 */
+#define SC_OPT_PASS_SCHED 13    /* wave priority in runtime-compiled sweep passes (s_setprio): 0 none; 1 / 2: the
+                                   second half / the odd workgroups of the grid raised; 3: raised during each
+                                   stage's arithmetic; 4: during its LDS reads and arithmetic; 5: reads, arithmetic
+                                   and LDS writes; 6: arithmetic and writes; default 4 (4-5 % faster passes than 0
+                                   on the 30-qubit layered circuit) */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */

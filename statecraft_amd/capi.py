@@ -25,6 +25,8 @@ GEN_NONE, GEN_X, GEN_Y, GEN_Z, GEN_ZZ = range(5)
 (OPT_EXEC_MODE, OPT_MAX_FUSED, OPT_PROFILE, OPT_LOW_QUBITS, OPT_TEST_DRIFT, OPT_REMAP,
  OPT_SWEEP_BLOCKS_PER_CU, OPT_JIT, OPT_SWEEP_PASS_CAP, OPT_CHUNK_BITS) = range(1, 11)
 OPT_DIRECT_IO = 11
+OPT_MFMA = 12
+OPT_PASS_SCHED = 13
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
 KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 

@@ -677,7 +677,8 @@ int fused_backward(Context *ctx, State *st, double2 *lam, const sc_op *ops, uint
         // plan and kernels of this segment's structure, cached like forward plans
         const int max_fuse = std::min(ctx->max_fused, 2);  // both states' registers stay within the budget
         const int low = ctx->low_qubits < 0 ? kDefaultLowQubits : ctx->low_qubits;
-        const std::string key = plan_cache_key(n, seq, low, false, max_fuse, ctx->jit, 0.0) + "adjoint";
+        const std::string key = plan_cache_key(n, seq, low, false, max_fuse, ctx->jit, 0.0) + "adjoint" +
+                                (ctx->pass_sched >= 3 ? "+prio" : "");
         PlanCache::Entry *ent = ctx->plan_cache->find(key);
         if (!ent) {
             std::unique_ptr<PlanCache::Entry> e(new PlanCache::Entry());

@@ -213,7 +213,7 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops_in, Pla
     int cb = ctx->chunk_bits;
     if (!ctx->jit || (cb == kMaxChunkBits && n < kMaxChunkBits)) cb = kChunkBits;
     std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
-                                     ctx->jit | (cm.direct_io ? 8 : 0), cm.pass_cap, cb);
+                                     ctx->jit | (cm.direct_io ? 8 : 0) | (ctx->pass_sched << 4), cm.pass_cap, cb);
     if (!absorbed.empty()) {
         key += "|product";
         for (const auto &a : absorbed) key += " " + std::to_string(a.first) + ":" + std::to_string(a.second);
@@ -546,6 +546,14 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
     case SC_OPT_DIRECT_IO:
         ctx->direct_io = value != 0;
         return SC_OK;
+    case SC_OPT_MFMA:
+        if (value < 0 || value > 2) return fail(SC_E_ARG, "MFMA option must be 0, 1 or 2");
+        ctx->mfma = (int)value;
+        return SC_OK;
+    case SC_OPT_PASS_SCHED:
+        if (value < 0 || value > 6) return fail(SC_E_ARG, "pass scheduling variant must be 0..6");
+        ctx->pass_sched = (int)value;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -565,6 +573,8 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_SWEEP_PASS_CAP: *value = ctx->pass_cap; return SC_OK;
     case SC_OPT_CHUNK_BITS: *value = ctx->chunk_bits; return SC_OK;
     case SC_OPT_DIRECT_IO: *value = ctx->direct_io; return SC_OK;
+    case SC_OPT_MFMA: *value = ctx->mfma; return SC_OK;
+    case SC_OPT_PASS_SCHED: *value = ctx->pass_sched; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }

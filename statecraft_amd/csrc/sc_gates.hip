@@ -174,6 +174,84 @@ __global__ void __launch_bounds__(kBlock) k_dense(double2 *__restrict__ amp, int
     }
 }
 
+// The same dense 2^K x 2^K block (K = 4, 5) as a complex GEMM on the FP64 matrix cores: Y = U X, where column
+// j of X is the 2^K amplitudes of index group j.  A wavefront takes 16 groups at a time and forms their outputs
+// with v_mfma_f64_16x16x4_f64 (D[16x16] += A[16x4] B[4x16]) as four real products per complex product:
+// Yr = Ur Xr + Ui (-Xi), Yi = Ur Xi + Ui Xr.  Operand layouts of that instruction on gfx950 (measured with a
+// one-wavefront probe): lane L supplies A[L % 16][L / 16] and B[L / 16][L % 16] (one double each) and holds
+// D[4 v + L / 16][L % 16] in accumulator element v.  So lane L reads block rows k = 4 s + L / 16 of group L % 16
+// straight from HBM (16 lanes = 16 consecutive groups, 256-byte runs when no block qubit is below position 4)
+// and writes output rows 4 v + L / 16 (+16 per row tile): no
+// LDS and no cross-lane exchange.  U's operands stay in registers for the kernel's life (2^K / 4 k-steps x
+// 2^(K-4) row tiles x 2 doubles per lane).  Every group is read and written by the four lanes of one wavefront
+// that own it, after all of the wavefront's loads, so the block is applied in place.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_dense_mfma(double2 *__restrict__ amp, int n, uint64_t fixed,
+                                                       uint64_t qoff0, uint64_t qoff1, uint64_t qoff2, uint64_t qoff3,
+                                                       uint64_t qoff4, int log_groups, uint64_t total,
+                                                       const double2 *__restrict__ mat) {
+    static_assert(K == 4 || K == 5, "MFMA dense blocks span 4 or 5 qubits");
+    constexpr int D = 1 << K;
+    constexpr int KS = D / 4;   // k-steps of 4
+    constexpr int MT = D / 16;  // row tiles of 16
+    const uint64_t qo[5] = {qoff0, qoff1, qoff2, qoff3, qoff4};
+    const uint32_t lane = threadIdx.x & 63u, col = lane & 15u, kq = lane >> 4;
+    auto block_off = [&](uint32_t c) -> uint64_t {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if ((c >> j) & 1u) o |= qo[j];
+        return o;
+    };
+    double ur[MT][KS], ui[MT][KS];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const double2 u = mat[(16 * mt + col) * D + 4 * s + kq];
+            ur[mt][s] = u.x;
+            ui[mt][s] = u.y;
+        }
+    uint64_t koff[KS], moff[MT][4];
+#pragma unroll
+    for (int s = 0; s < KS; s++) koff[s] = block_off(4u * s + kq);
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) moff[mt][v] = block_off(16u * mt + 4u * v + kq);
+    const uint64_t gmask = (1ULL << log_groups) - 1ULL;
+    const uint64_t tiles = total >> 4;
+    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t t = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < tiles; t += waves) {
+        const uint64_t g = (t << 4) | col;
+        const uint64_t b = expand_mask(g & gmask, fixed);
+        double2 *base = amp + ((g >> log_groups) << n) + b;
+        double xr[KS], xi[KS], nxi[KS];
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const double2 x = base[koff[s]];
+            xr[s] = x.x;
+            xi[s] = x.y;
+            nxi[s] = -x.y;
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+            v4d yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xr[s], yr, 0, 0, 0);
+                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], nxi[s], yr, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xi[s], yi, 0, 0, 0);
+                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], xr[s], yi, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) base[moff[mt][v]] = make_double2(yr[v], yi[v]);
+        }
+    }
+}
+
 // Plain device copy, 16 B per lane per iteration: the bandwidth reference for the sweep kernel.
 // Each block moves 4 x 256 consecutive values (4 independent 16 B loads in flight per lane).
 __global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
@@ -273,6 +351,23 @@ int launch_dense(Context *ctx, State *st, int k, const int *qubits, const double
     unsigned grid = grid_for(ctx, total);
     const double2 *m = reinterpret_cast<const double2 *>(mat_dev);
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_FUSED, 32.0 * (double)((uint64_t)st->batch << st->n));
+    // matrix-core variant (blocks with whole 16-group tiles, n - k >= 4): 5-qubit blocks by default, where the
+    // VALU kernel is arithmetic-bound (17.6 ms vs 7.0 ms per 30-qubit sweep on MI355X); 4-qubit blocks with
+    // SC_OPT_MFMA 2 (the VALU kernel is at HBM speed there: 6.8 ms vs 7.2 ms)
+    if (((ctx->mfma >= 1 && k == 5) || (ctx->mfma >= 2 && k == 4)) && log_groups >= 4) {
+        const uint64_t tiles = total >> 4;
+        uint64_t blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
+        const uint64_t cap = (uint64_t)ctx->cu_count * 16;
+        if (blocks > cap) blocks = cap;
+        if (k == 4)
+            hipLaunchKernelGGL(k_dense_mfma<4>, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
+                               fixed, qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
+        else
+            hipLaunchKernelGGL(k_dense_mfma<5>, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
+                               fixed, qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
+        SC_HIP_CHECK(hipGetLastError());
+        return SC_OK;
+    }
     switch (k) {
     case 1:
         hipLaunchKernelGGL(k_dense<1>, dim3(grid), dim3(kBlock), 0, ctx->stream, st->amp, st->n, fixed, qo[0], qo[1],

@@ -68,6 +68,8 @@ struct Context {
     int pass_cap = -1;
     int chunk_bits = 0;   // SC_OPT_CHUNK_BITS; 0: chosen per circuit structure
     int direct_io = 1;    // SC_OPT_DIRECT_IO
+    int mfma = 1;         // SC_OPT_MFMA
+    int pass_sched = 4;   // SC_OPT_PASS_SCHED
     int cu_count = 256;
     Profiler prof;
     // scratch (grown on demand, freed with the context)

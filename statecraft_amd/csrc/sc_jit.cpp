@@ -132,7 +132,7 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
 // A pass with direct loads (SweepPass::direct bit 0) reads HBM straight into its first stage's registers,
 // one with direct stores (bit 1) writes its last stage's registers straight to HBM; the LDS round trips of
 // the load and store phases are then skipped.
-std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch) {
+std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch, int sched = 0) {
     std::ostringstream s;
     const int TB = plan.chunk_bits - kRegBits;  // thread-index bits: 8 (256 threads) or 9 (512)
     const int S = ps.nstages;
@@ -155,8 +155,13 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "  const uint64_t span = blocked ? (total_chunks + gridDim.x - 1) / gridDim.x : 0;\n"
          "  uint64_t c = blocked ? blockIdx.x * span : blockIdx.x;\n"
          "  const uint64_t c_end = blocked ? (c + span < total_chunks ? c + span : total_chunks) : total_chunks;\n"
-         "  if (c >= c_end) return;\n"
-         "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
+         "  if (c >= c_end) return;\n";
+    // wave priority: one of the two workgroups a CU holds issues first when both have VALU work, so the two
+    // drift apart and one's LDS exchanges overlap the other's arithmetic (1: the grid's second half, 2: odd
+    // workgroups)
+    if (sched == 1) s << "  if (blockIdx.x >= gridDim.x / 2u) __builtin_amdgcn_s_setprio(1);\n";
+    if (sched == 2) s << "  if (blockIdx.x & 1u) __builtin_amdgcn_s_setprio(1);\n";
+    s << "  const uint64_t chunk_mask = (1ULL << log_chunks) - 1ULL;\n"
          "  auto cbase = [](uint64_t x) -> uint64_t { return "
       << deposit_expr("x", ps.outer_mask) << "; };\n";
     // global offsets of the load side (first stage's layout when direct, else the load phase's) and of the
@@ -280,22 +285,30 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         const bool to_lds = !(st == S - 1 && DS);
         if (from_lds || to_lds)
             s << "    const uint32_t ts = " << lane_expr("t", stg.thread_slot, TB, "^", false, 16) << ";\n";
+        // wave priority raised over part of the stage (3: arithmetic; 4: LDS reads and arithmetic; 5: reads,
+        // arithmetic and writes; 6: arithmetic and writes)
+        const bool hi_read = sched == 4 || sched == 5, hi_write = sched == 5 || sched == 6;
+        const bool hi_arith = sched >= 3;
         bool in_a;
+        if (hi_read) s << "    __builtin_amdgcn_s_setprio(1);\n";
         if (from_lds) {
             lds_read("a", "ts", stg.lds_off);
             in_a = true;
         } else {
             in_a = std::string(cur) == "a";
         }
+        if (hi_arith && !hi_read) s << "    __builtin_amdgcn_s_setprio(1);\n";
         for (int o = stg.op_begin; o < stg.op_end; o++) {
             s << "    " << op_call(plan.ops[o], in_a ? "a" : "p", in_a ? "p" : "a") << "\n";
             in_a = !in_a;
         }
+        if (hi_arith && !(hi_write && to_lds)) s << "    __builtin_amdgcn_s_setprio(0);\n";
         cur = in_a ? "a" : "p";
         if (to_lds) {
             if (st == 0 && DL) before_first_write();
             s << "    const uint32_t tw = opaque(ts);\n";
             lds_write("tw", stg.lds_off, cur);
+            if (hi_write) s << "    __builtin_amdgcn_s_setprio(0);\n";
             sync(st + 1 < S ? ps.stages[st + 1].wave_local != 0 : (ps.wave_local & 1) != 0);
         }
         s << "    }\n";
@@ -328,7 +341,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
 // workgroup in LDS.  Chunks move through one 64 KiB LDS buffer, psi then lambda.  Workgroup w serves batch
 // element e0 + w % nb and that element's chunks sub, sub + gsub, ... (sub = w / nb); at the end it writes
 // its sum for probe k to part[(slot_k * B + e) * gsub + sub].
-std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps) {
+std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps, bool prio = false) {
     std::ostringstream s;
     std::vector<uint64_t> probes;
     for (int st = 0; st < ps.nstages; st++)
@@ -404,6 +417,8 @@ std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps) {
                 s << "    __syncthreads();\n    }\n";
             }
         }
+        // wave priority raised over the stage's arithmetic, as in the forward passes
+        if (prio) s << "    __builtin_amdgcn_s_setprio(1);\n";
         for (int o = stg.op_begin; o < stg.op_end; o++) {
             const SweepOp &op = plan.ops[o];
             const char *P0 = in_a ? "a" : "p", *P1 = in_a ? "p" : "a";
@@ -421,6 +436,7 @@ std::string adjoint_pass_source(const SweepPlan &plan, const SweepPass &ps) {
             s << "    " << op_call(op, L0, L1) << "\n";
             in_a = !in_a;
         }
+        if (prio) s << "    __builtin_amdgcn_s_setprio(0);\n";
     }
     // store phase: each state through LDS into the store layout, then to HBM
     const uint32_t *last_off = ps.nstages > 0 ? ps.stages[ps.nstages - 1].lds_off : ps.io_reg_slot;
@@ -593,7 +609,8 @@ int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vecto
 
 int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
     std::vector<std::string> bodies;
-    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(pass_source(plan, plan.passes[p], ctx->jit == 2));
+    for (size_t p = 0; p < plan.passes.size(); p++)
+        bodies.push_back(pass_source(plan, plan.passes[p], ctx->jit == 2, ctx->pass_sched));
     return jit_kernels(ctx, bodies, out);
 }
 
@@ -608,7 +625,8 @@ std::vector<uint64_t> jit_pass_probes(const SweepPlan &plan, size_t pass) {
 
 int jit_adjoint_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
     std::vector<std::string> bodies;
-    for (size_t p = 0; p < plan.passes.size(); p++) bodies.push_back(adjoint_pass_source(plan, plan.passes[p]));
+    for (size_t p = 0; p < plan.passes.size(); p++)
+        bodies.push_back(adjoint_pass_source(plan, plan.passes[p], ctx->pass_sched >= 3));
     return jit_kernels(ctx, bodies, out);
 }
 
