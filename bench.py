@@ -6,11 +6,14 @@ depth 20 (per layer Ry Rz on every qubit, then CNOT bricks; 1490 ops), the proba
 circuit with the amplitudes it must change (1q / diagonal: 2^n, CNOT: 2^(n-1)).
 
 Also reported: the sweep kernel's achieved HBM bandwidth (HIP events on the engine's stream, bytes =
-32 B x 2^n per sweep) against the MI355X peak, and the CPU oracle (plain C port of the reference
+32 B x 2^n per sweep) against the MI355X peak; the adjoint-gradient half of the metric (24-qubit HEA, 384
+parameters, batch 64, against parameter shift); short runs of the other configurations (20-qubit Ry+QFT,
+33-qubit Grover, batched QAOA + GHZ-20 trajectories); and the CPU oracle (plain C port of the reference
 StatevectorOps split over the host's threads like StatevectorParallelExecutor) timed on a bounded sample of
 the same circuit family at 29 qubits, the reference's dense limit.
 
-  python bench.py [--steps K] [--warmup W] [--workload c2|c1|c3|c5] [--no-cpu-baseline]
+  python bench.py [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5] [--no-cpu-baseline] [--no-adjoint]
+                  [--no-other-configs]
 """
 from __future__ import annotations
 
@@ -56,13 +59,20 @@ def make_context(args) -> capi.Context:
     ctx.set_option(capi.OPT_DIRECT_IO, 0 if args.no_direct_io else 1)
     if args.chunk_bits is not None:
         ctx.set_option(capi.OPT_CHUNK_BITS, args.chunk_bits)
+    if args.mfma is not None:
+        ctx.set_option(capi.OPT_MFMA, args.mfma)
+    if args.max_fused is not None:
+        ctx.set_option(capi.OPT_MAX_FUSED, args.max_fused)
+    if args.pass_sched is not None:
+        ctx.set_option(capi.OPT_PASS_SCHED, args.pass_sched)
     return ctx
 
 
 def context_options(ctx) -> dict:
     return {"jit": ctx.get_option(capi.OPT_JIT), "remap": bool(ctx.get_option(capi.OPT_REMAP)),
             "low_qubits_option": ctx.get_option(capi.OPT_LOW_QUBITS),
-            "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS)}
+            "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS),
+            "pass_sched": ctx.get_option(capi.OPT_PASS_SCHED), "mfma": ctx.get_option(capi.OPT_MFMA)}
 
 
 def host_threads() -> int:
@@ -219,8 +229,15 @@ def bench_c2(args):
         out, gap = st.sample(measured, u)
         return cs, out, gap
 
+    # time to first result: the first step of a fresh context plans the circuit under each candidate geometry
+    # and compiles the chosen plan's pass kernels (hipRTC) before running it
+    first_step_ms = None
     for i in range(args.warmup):
+        tw = time.perf_counter()
         step()
+        ctx.synchronize()
+        if i == 0:
+            first_step_ms = 1e3 * (time.perf_counter() - tw)
         log(f"warmup step {i + 1}/{args.warmup} done")
     ctx.synchronize()
     ctx.profile_reset()
@@ -302,7 +319,7 @@ def bench_c2(args):
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_source,
-            "kernel": {"sweep": "sc_pass_*" if opts["jit"] else "k_sweep", "fused": "k_dense",
+            "kernel": {"sweep": "sc_pass_*" if opts["jit"] else "k_sweep", "fused": "k_dense, k_dense_mfma",
                        "direct": "k_single"}[args.mode],
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_ms,
@@ -316,6 +333,7 @@ def bench_c2(args):
             "fp64_ops_per_amp_per_step": fp64_per_step,
         },
         "amp_updates_per_step": credit,
+        "first_step_ms": first_step_ms,
         "gate_ms_per_step": circuit_ms,
         "measure_ms_per_step": m_ms / args.steps,
         "checksum": list(cs),
@@ -537,6 +555,25 @@ def bench_c5(args):
                  adjoint_stats=ctx.adjoint_stats(), energy_0=float(energies[0]))
 
 
+def other_configs(args):
+    """Short runs of the other benchmark configurations at their full sizes, reported beside the headline:
+    20-qubit Ry+QFT (3 steps), 33-qubit Grover (1 step), batched QAOA + GHZ-20 trajectories (2 steps); one
+    warm-up step each (planning and kernel compilation)."""
+    out = {}
+    for name, fn, steps, keys in (
+            ("c1_ry_qft_20q", bench_c1, 3, ()),
+            ("c3_grover_33q", bench_c3, 1, ("marked_probability", "rel_error", "roofline")),
+            ("c4_qaoa_batched_and_ghz20", bench_c4, 2, ("trajectories_per_s", "qaoa_seconds",
+                                                        "qaoa_one_graph_launch", "mean_cut_expectation"))):
+        log(f"{name} ...")
+        a = argparse.Namespace(**vars(args))
+        a.steps, a.warmup, a.qubits = steps, 1, 30
+        r = fn(a)
+        out[name] = {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "steps": steps,
+                     "ms_per_step": r["ms_per_step"], **{k: r[k] for k in keys}}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -553,8 +590,17 @@ def main(argv=None) -> int:
     ap.add_argument("--shots", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", default="sweep", choices=list(MODES))
+    ap.add_argument("--max-fused", type=int, default=None, choices=[0, 1, 2, 3, 4, 5],
+                    help="most qubits of a dense fused block (SC_OPT_MAX_FUSED; fused mode: 1..5)")
+    ap.add_argument("--pass-sched", type=int, default=None, choices=[0, 1, 2, 3, 4, 5, 6],
+                    help="SC_OPT_PASS_SCHED: wave-priority variant of the runtime-compiled passes (library default 4)")
+    ap.add_argument("--mfma", type=int, default=None, choices=[0, 1, 2],
+                    help="SC_OPT_MFMA, fused mode: 0 VALU blocks only, 1 5-qubit blocks on the FP64 matrix cores "
+                         "(library default), 2 also 4-qubit blocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c2: skip the adjoint-gradient block of the metric")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="c2: skip the short runs of the 20-qubit QFT, 33-qubit Grover and batched configurations")
     ap.add_argument("--no-direct-io", action="store_true",
                     help="SC_OPT_DIRECT_IO 0: every pass goes through LDS at both ends")
     ap.add_argument("--no-copy-reference", action="store_true")
@@ -589,6 +635,8 @@ def main(argv=None) -> int:
                                              "speedup_vs_param_shift", "max_abs_diff_vs_param_shift",
                                              "param_shift_sample")}
         res["adjoint"]["config"] = c5["config"]
+    if args.workload == "c2" and not args.no_other_configs:
+        res["other_configs"] = other_configs(args)
     if args.workload == "c2" and not args.no_cpu_baseline:
         log("cpu baseline (reference path restated in C, all host threads, 29 qubits) ...")
         res["cpu_baseline"] = cpu_baseline(args.depth, args.seed, args.cpu_seconds)
