@@ -65,6 +65,8 @@ def make_context(args) -> capi.Context:
         ctx.set_option(capi.OPT_MAX_FUSED, args.max_fused)
     if args.pass_sched is not None:
         ctx.set_option(capi.OPT_PASS_SCHED, args.pass_sched)
+    if args.permlane is not None:
+        ctx.set_option(capi.OPT_PERMLANE, args.permlane)
     return ctx
 
 
@@ -72,7 +74,8 @@ def context_options(ctx) -> dict:
     return {"jit": ctx.get_option(capi.OPT_JIT), "remap": bool(ctx.get_option(capi.OPT_REMAP)),
             "low_qubits_option": ctx.get_option(capi.OPT_LOW_QUBITS),
             "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS),
-            "pass_sched": ctx.get_option(capi.OPT_PASS_SCHED), "mfma": ctx.get_option(capi.OPT_MFMA)}
+            "pass_sched": ctx.get_option(capi.OPT_PASS_SCHED), "mfma": ctx.get_option(capi.OPT_MFMA),
+            "permlane": ctx.get_option(capi.OPT_PERMLANE)}
 
 
 def host_threads() -> int:
@@ -594,6 +597,9 @@ def main(argv=None) -> int:
                     help="most qubits of a dense fused block (SC_OPT_MAX_FUSED; fused mode: 1..5)")
     ap.add_argument("--pass-sched", type=int, default=None, choices=[0, 1, 2, 3, 4, 5, 6],
                     help="SC_OPT_PASS_SCHED: wave-priority variant of the runtime-compiled passes (library default 4)")
+    ap.add_argument("--permlane", type=int, default=None, choices=[0, 1],
+                    help="SC_OPT_PERMLANE: stage transitions inside the registers where the plan allows (library "
+                         "default 0)")
     ap.add_argument("--mfma", type=int, default=None, choices=[0, 1, 2],
                     help="SC_OPT_MFMA, fused mode: 0 VALU blocks only, 1 5-qubit blocks on the FP64 matrix cores "
                          "(library default), 2 also 4-qubit blocks")

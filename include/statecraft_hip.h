@@ -126,6 +126,11 @@ typedef struct sc_op {
                                    stage's arithmetic; 4: during its LDS reads and arithmetic; 5: reads, arithmetic
                                    and LDS writes; 6: arithmetic and writes; default 4 (4-5 % faster passes than 0
                                    on the 30-qubit layered circuit) */
+#define SC_OPT_PERMLANE 14      /* 1: runtime-compiled passes enter a stage whose register qubits differ from the
+                                   previous stage's in at most two (found on lane bits 4 / 5) by exchanging them
+                                   inside each wavefront (v_permlane32_swap / v_permlane16_swap) instead of through
+                                   LDS; default 0 (measured neutral: the transitions it can take are the wave-local
+                                   ones, whose LDS traffic already overlaps the other workgroup's arithmetic) */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */

@@ -209,11 +209,12 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops_in, Pla
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
     cm.direct_io = cm.direct_io && ctx->direct_io;
+    cm.permlane = cm.permlane && ctx->permlane;
     // a fixed 13-bit chunk applies from 13 qubits with runtime-compiled passes; 0 leaves the choice to the planner
     int cb = ctx->chunk_bits;
     if (!ctx->jit || (cb == kMaxChunkBits && n < kMaxChunkBits)) cb = kChunkBits;
     std::string key = plan_cache_key(n, ops, ctx->low_qubits, ctx->remap != 0, max_fuse,
-                                     ctx->jit | (cm.direct_io ? 8 : 0) | (ctx->pass_sched << 4), cm.pass_cap, cb);
+                                     ctx->jit | (cm.direct_io ? 8 : 0) | (ctx->pass_sched << 4) | (cm.permlane ? 256 : 0), cm.pass_cap, cb);
     if (!absorbed.empty()) {
         key += "|product";
         for (const auto &a : absorbed) key += " " + std::to_string(a.first) + ":" + std::to_string(a.second);
@@ -554,6 +555,9 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
         if (value < 0 || value > 6) return fail(SC_E_ARG, "pass scheduling variant must be 0..6");
         ctx->pass_sched = (int)value;
         return SC_OK;
+    case SC_OPT_PERMLANE:
+        ctx->permlane = value != 0;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -575,6 +579,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_DIRECT_IO: *value = ctx->direct_io; return SC_OK;
     case SC_OPT_MFMA: *value = ctx->mfma; return SC_OK;
     case SC_OPT_PASS_SCHED: *value = ctx->pass_sched; return SC_OK;
+    case SC_OPT_PERMLANE: *value = ctx->permlane; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }

@@ -69,6 +69,7 @@ struct Context {
     int chunk_bits = 0;   // SC_OPT_CHUNK_BITS; 0: chosen per circuit structure
     int direct_io = 1;    // SC_OPT_DIRECT_IO
     int mfma = 1;         // SC_OPT_MFMA
+    int permlane = 0;     // SC_OPT_PERMLANE
     int pass_sched = 4;   // SC_OPT_PASS_SCHED
     int cu_count = 256;
     Profiler prof;
@@ -152,7 +153,11 @@ struct SweepStage {
     // 1: the exchange into this stage stays inside each wavefront (the wave-index bits, thread bits 6 and 7,
     // sit on the same local positions before and after), so it needs no workgroup barrier
     int32_t wave_local;
-    int32_t pad;
+    // Runtime-compiled passes only: nonzero when this stage is entered from the previous one inside the
+    // registers (bit 6), its layout being the previous layout with register bit j5 exchanged with lane bit 5
+    // (bits 0-2: j5 + 1, 0 for none) and register bit j4 with lane bit 4 (bits 3-5: j4 + 1); the LDS tables
+    // still describe the layout, so other executors may take the exchange through LDS
+    int32_t perm;
     uint32_t lds_off[kRegs];            // swizzled LDS slot offset per register index
     uint32_t thread_slot[kMaxThreadBits];  // swizzled slot contribution of each thread-index bit
 };

@@ -144,7 +144,10 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
          "@K@(sc::v2d *__restrict__ amp, sc::v2d *__restrict__ out, int n, int log_chunks, uint64_t total_chunks,\n"
          "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin, long long basis) {\n"
          "  using namespace sc;\n";
-    const bool uses_lds = !(DL && DS && S == 1);
+    // stage st > 0 with SweepStage::perm is entered inside the registers, with no LDS round trip
+    auto in_reg = [&](int st) { return st > 0 && st < S && ps.stages[st].perm != 0; };
+    bool uses_lds = !DL || !DS;
+    for (int st = 1; st < S; st++) uses_lds = uses_lds || !in_reg(st);
     if (uses_lds) s << "  __shared__ v2d lds[" << (1 << plan.chunk_bits) << "];\n";
     s << "  CV2 *mats = (CV2 *)mats_arg;\n"
          "  const uint32_t tid = threadIdx.x;\n"
@@ -281,8 +284,8 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         const SweepStage &stg = ps.stages[st];
         s << "    {\n"
              "    const uint32_t t = opaque(tid);\n";
-        const bool from_lds = !(st == 0 && DL);
-        const bool to_lds = !(st == S - 1 && DS);
+        const bool from_lds = !(st == 0 && DL) && !in_reg(st);
+        const bool to_lds = !(st == S - 1 && DS) && !in_reg(st + 1);
         if (from_lds || to_lds)
             s << "    const uint32_t ts = " << lane_expr("t", stg.thread_slot, TB, "^", false, 16) << ";\n";
         // wave priority raised over part of the stage (3: arithmetic; 4: LDS reads and arithmetic; 5: reads,
@@ -298,6 +301,11 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
             in_a = std::string(cur) == "a";
         }
         if (hi_arith && !hi_read) s << "    __builtin_amdgcn_s_setprio(1);\n";
+        if (in_reg(st)) {
+            const int j5 = (stg.perm & 7) - 1, j4 = ((stg.perm >> 3) & 7) - 1;
+            if (j5 >= 0) s << "    lane_exchange<" << j5 << ", false>(" << cur << ");\n";
+            if (j4 >= 0) s << "    lane_exchange<" << j4 << ", true>(" << cur << ");\n";
+        }
         for (int o = stg.op_begin; o < stg.op_end; o++) {
             s << "    " << op_call(plan.ops[o], in_a ? "a" : "p", in_a ? "p" : "a") << "\n";
             in_a = !in_a;
@@ -305,7 +313,7 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         if (hi_arith && !(hi_write && to_lds)) s << "    __builtin_amdgcn_s_setprio(0);\n";
         cur = in_a ? "a" : "p";
         if (to_lds) {
-            if (st == 0 && DL) before_first_write();
+            before_first_write();
             s << "    const uint32_t tw = opaque(ts);\n";
             lds_write("tw", stg.lds_off, cur);
             if (hi_write) s << "    __builtin_amdgcn_s_setprio(0);\n";

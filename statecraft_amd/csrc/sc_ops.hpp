@@ -111,6 +111,43 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// In-register transition of a stage: register bit J exchanges its qubit with lane bit 4 (ROW16) or 5.  For every
+// register pair (r, r | 2^J) the lanes of r whose lane bit is set trade places with the lanes of r | 2^J whose
+// lane bit is clear; v_permlane16_swap / v_permlane32_swap (gfx950) swap exactly those halves of two VGPRs
+// (odd 16-lane rows of the first with even rows of the second / upper 32 lanes of the first with lower 32 of
+// the second), one instruction per dword pair.
+__device__ __forceinline__ void lane_swap_u32(uint32_t &lo, uint32_t &hi, bool row16) {
+    if (row16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+        lo = r[0];
+        hi = r[1];
+    }
+}
+__device__ __forceinline__ void lane_swap_f64(double &lo, double &hi, bool row16) {
+    const uint64_t a = __builtin_bit_cast(uint64_t, lo), b = __builtin_bit_cast(uint64_t, hi);
+    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    lane_swap_u32(a0, b0, row16);
+    lane_swap_u32(a1, b1, row16);
+    lo = __builtin_bit_cast(double, ((uint64_t)a1 << 32) | a0);
+    hi = __builtin_bit_cast(double, ((uint64_t)b1 << 32) | b0);
+}
+template <int J, bool ROW16>
+__device__ __forceinline__ void lane_exchange(Regs &x) {
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) {
+        if (r & (1 << J)) continue;
+        double lx = x[r].x, ly = x[r].y, hx = x[r | (1 << J)].x, hy = x[r | (1 << J)].y;
+        lane_swap_f64(lx, hx, ROW16);
+        lane_swap_f64(ly, hy, ROW16);
+        x[r] = make_v2d(lx, ly);
+        x[r | (1 << J)] = make_v2d(hx, hy);
+    }
+}
+
 __device__ __forceinline__ void copy_all(const Regs &s, Regs &d) {
 #pragma unroll
     for (int r = 0; r < kRegs; r++) d[r] = s[r];

@@ -599,10 +599,16 @@ double predicted_pass_ns_per_amp(const SweepPlan &plan, const SweepPass &pass) {
     constexpr double kFloor12 = 6.014e-3, kFloor13 = 6.601e-3;  // ns per amplitude
     constexpr double kBase = 2.550e-3, kPerFp64 = 3.560e-5, kPerRoundTrip = 4.597e-4;
     double f = 0.0;
-    for (int s = 0; s < pass.nstages; s++)
+    int in_reg = 0;
+    for (int s = 0; s < pass.nstages; s++) {
         for (int o = pass.stages[s].op_begin; o < pass.stages[s].op_end; o++) f += op_fp64(plan.ops[o]);
-    // LDS round trips: one before each stage and the store phase's, less those the direct HBM paths skip
-    const int rt = pass.nstages + 1 - (pass.direct & 1) - ((pass.direct >> 1) & 1);
+        // an in-register transition: 32 lane exchanges per register bit and lane (2 per amplitude)
+        const int pm = pass.stages[s].perm;
+        if (pm) in_reg++, f += 2.0 * ((pm & 7) != 0) + 2.0 * (((pm >> 3) & 7) != 0);
+    }
+    // LDS round trips: one before each stage and the store phase's, less those the direct HBM paths and the
+    // in-register transitions skip
+    const int rt = pass.nstages + 1 - (pass.direct & 1) - ((pass.direct >> 1) & 1) - in_reg;
     const double floor = plan.chunk_bits == kMaxChunkBits ? kFloor13 : kFloor12;
     return std::max(floor, kBase + kPerFp64 * f + kPerRoundTrip * rt);
 }
@@ -945,6 +951,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
         };
         std::vector<StageLayout> layouts(stages.size());
         std::vector<uint32_t> stage_rp(stages.size(), 0u);
+        std::vector<int> stage_perm(stages.size(), 0);  // SweepStage::perm
         int pass_wave_local = 0;
         for (size_t s = 0; s < stages.size(); s++) {
             uint32_t rp = 0;
@@ -1045,6 +1052,71 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
                 // with 512 threads the exchange stays in the wavefront only if bit 8 keeps its position too:
                 // such chunks always synchronise the workgroup
                 layouts[s].wave_local = TB == 8 && wsel[s] == (s == 0 ? w_load : wsel[s - 1]);
+            }
+            // ---- in-register transitions (runtime-compiled passes, 256 threads).  A stage whose register
+            // qubits differ from the previous stage's in at most two, the incoming ones on the previous
+            // layout's lane bits 4 / 5 and every other lane and wave bit keeping its qubit, is entered by
+            // exchanging register bits with those lane bits inside each wavefront (v_permlane32_swap /
+            // v_permlane16_swap, one VALU instruction per dword pair): no LDS round trip and no barrier.  A
+            // layout not fixed by such a transition moves the next stage's incoming qubits onto its lane bits
+            // 4 / 5 when that makes the next transition one.
+            if (cm.permlane && TB == 8 && S > 1) {
+                std::vector<char> locked(S, 0);
+                for (int s = 0; s + 1 < S; s++) {
+                    if (wsel[s] != wsel[s + 1]) continue;
+                    StageLayout &A = layouts[s];
+                    uint32_t rpA = 0;
+                    for (int j = 0; j < kRegBits; j++) rpA |= 1u << A.rpos[j];
+                    const uint32_t in = stage_rp[s + 1] & ~rpA;
+                    if (popc(in) > 2) continue;
+                    // a stage that touches HBM directly keeps positions 0-3 on lane bits 0-3, and a transition
+                    // keeps the lane bits 0-3 it finds
+                    const bool needs_low = (s == 0 && direct_load) || (s + 1 == S - 1 && direct_store);
+                    auto low_ok = [&](const StageLayout &L) {
+                        for (int i = 0; i < 4; i++)
+                            if (L.tpos[i] != i) return false;
+                        return true;
+                    };
+                    if ((in & ~((1u << A.tpos[4]) | (1u << A.tpos[5]))) != 0) {
+                        if (locked[s]) continue;
+                        // re-order A's lane bits 0-5: the incoming qubits on 5 (and 4), the rest ascending
+                        StageLayout C = A;
+                        std::vector<int> rest, inc;
+                        for (int i = 0; i < 6; i++) ((in >> A.tpos[i]) & 1u ? inc : rest).push_back(A.tpos[i]);
+                        std::sort(rest.begin(), rest.end());
+                        std::sort(inc.begin(), inc.end());
+                        for (int i = 0; i < 4; i++) C.tpos[i] = rest[i];
+                        if (inc.size() == 2) {
+                            C.tpos[4] = inc[0];
+                            C.tpos[5] = inc[1];
+                        } else {
+                            C.tpos[4] = rest[4];
+                            C.tpos[5] = inc[0];
+                        }
+                        if (needs_low && !low_ok(C)) continue;
+                        A = C;
+                    } else if (needs_low && !low_ok(A)) {
+                        continue;
+                    }
+                    // B: A with each outgoing register qubit exchanged with an incoming lane-bit qubit
+                    StageLayout B = A;
+                    int perm = 64;
+                    std::vector<int> outj;
+                    for (int j = 0; j < kRegBits; j++)
+                        if (!((stage_rp[s + 1] >> A.rpos[j]) & 1u)) outj.push_back(j);
+                    size_t k = 0;
+                    for (int l : {5, 4}) {
+                        if (!((in >> A.tpos[l]) & 1u)) continue;
+                        const int j = outj[k++];
+                        B.rpos[j] = A.tpos[l];
+                        B.tpos[l] = A.rpos[j];
+                        perm |= (j + 1) << (l == 5 ? 0 : 3);
+                    }
+                    B.wave_local = true;
+                    layouts[s + 1] = B;
+                    stage_perm[s + 1] = perm;
+                    locked[s + 1] = 1;
+                }
             }
             pass_wave_local = 0;
             if (TB == 8) {
@@ -1163,6 +1235,7 @@ int plan_sweeps(int n, const std::vector<PlanOp> &ops, int low_qubits, bool rema
             SweepStage &stg = pass.stages[s];
             const StageLayout &lay = layouts[s];
             stg.wave_local = lay.wave_local ? 1 : 0;
+            stg.perm = stage_perm[s];
             for (int i = 0; i < TB; i++) stg.thread_slot[i] = slot_of(1u << lay.tpos[i]);
             for (int r = 0; r < kRegs; r++) {
                 uint32_t l = 0;

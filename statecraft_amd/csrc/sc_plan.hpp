@@ -123,14 +123,15 @@ struct SweepCostModel {
     double pass_cap;     // most estimated FP64 operations per amplitude a pass takes on (0: no limit)
     bool unit_1q;        // single-qubit ops with a unit-pivot form use it (SW_U1U) and fold their pivots
     bool direct_io;      // passes may load into their first stage's and store from their last stage's registers
+    bool permlane;       // stages may be entered by in-register lane exchanges (SweepStage::perm)
 };
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
-constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false, false};
-constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true, true};
+constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false, false, false};
+constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true, true, true};
 // Adjoint passes: as the runtime-compiled passes; a pass with gradient probes applies its unit-pivot factor
 // by a scaling op at its end, and each probe's sum is corrected on the host (SweepPlan::probe_pivots).  Their
 // kernels always go through LDS at both ends.
-constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true, false};
+constexpr SweepCostModel kAdjointPassCost{0.5, true, 0.0, true, false, false};
 
 // max_fuse_bits: register bits a fused dense block may span (0: no fusion beyond what a diagonal on two
 // register bits needs).

@@ -123,6 +123,37 @@ def test_jit_random_layered_20q(jit_ctx):
     assert np.max(np.abs(got - oracle.run_ops(n, ops, mats))) <= EPS
 
 
+@pytest.mark.parametrize("case", ["layered20", "random15", "batched13"])
+def test_jit_in_register_transitions(jit_ctx, case):
+    """SC_OPT_PERMLANE 1: stages entered through v_permlane32_swap / v_permlane16_swap exchanges (the 20-qubit
+    layered circuit's plan has such transitions in most passes)."""
+    jit_ctx.set_option(capi.OPT_LOW_QUBITS, 4)
+    jit_ctx.set_option(capi.OPT_REMAP, 0)
+    jit_ctx.set_option(capi.OPT_PERMLANE, 1)
+    try:
+        if case == "layered20":
+            n = 20
+            ops, mats = lower(workloads.random_layered(n, 20, 11))
+            got = run(jit_ctx, n, ops, mats)
+            want = oracle.run_ops(n, ops, mats)
+        elif case == "random15":
+            n = 15
+            rng = np.random.default_rng(515)
+            ops, mats = random_circuit(n, 400, rng)
+            init = random_state(n, rng)
+            got = run(jit_ctx, n, ops, mats, init)
+            want = oracle.run_ops(n, ops, mats, init)
+        else:
+            n, batch = 13, 3
+            ops, _ = lower(workloads.random_layered(n, 8, 3))
+            per = [lower(workloads.random_layered(n, 8, 3 + 10 * b))[1] for b in range(batch)]
+            got = run(jit_ctx, n, ops, np.stack(per), batch=batch, batched=True)
+            want = np.stack([oracle.run_ops(n, ops, per[b]) for b in range(batch)])
+    finally:
+        jit_ctx.set_option(capi.OPT_PERMLANE, 0)
+    assert np.max(np.abs(got - want)) <= EPS
+
+
 @pytest.mark.parametrize("n,general,diagonals", [(13, False, False), (15, True, True)])
 def test_jit_unit_pivot_forms(jit_ctx, n, general, diagonals):
     """Ry / Rx / H as unit-pivot ops with the pass's common factor in a table, a dense op or a scaling op."""
