@@ -300,7 +300,7 @@ int sc_last_plan_stats(sc_ctx *ctx, uint64_t *sweeps, uint64_t *kernel_launches,
 int sc_last_plan_geometry(sc_ctx *ctx, int *chunk_bits, int *low_qubits, int *candidates, double *ns_per_amp);
 /* Host-only plan inspection (no device needed): builds the SC_EXEC_SWEEP plan for an n-qubit circuit
  * (n >= 12) as a context with the given low-qubit, remap, jit, pass-cap and chunk-bit (12, or 13 with jit)
- * options would, and reports its passes,
+ * options would (jit 2: also with SC_OPT_PERMLANE 1), and reports its passes,
  * stages, fused 16x16 blocks and FP64 operations per amplitude.  When state_inout is non-NULL (2^n
  * interleaved complex values) the plan is also executed on the host with the sweep kernel's exact data
  * movement, so plans can be validated without a GPU. */

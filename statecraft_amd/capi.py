@@ -196,8 +196,9 @@ def complex_to_f64(a) -> np.ndarray:
 
 
 def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: bool = True, jit: bool = False,
-                     pass_cap: int = -1, chunk_bits: int = 12):
-    """Host-only sweep plan statistics; with `state` (complex 2^n) also executes the plan on the host."""
+                     pass_cap: int = -1, chunk_bits: int = 12, permlane: bool = False):
+    """Host-only sweep plan statistics; with `state` (complex 2^n) also executes the plan on the host.  jit: the
+    runtime-compiled passes' plan; permlane (with jit): with in-register stage transitions."""
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
     a, b, c, f = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
@@ -207,7 +208,8 @@ def plan_sweeps_host(n: int, ops, mats, low_qubits: int = 4, state=None, remap: 
         out = np.array(state, dtype=np.complex128, copy=True)
         ptr = dptr(out.view(np.float64))
     check(lib().sc_plan_sweeps_host(int(n), ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2, int(low_qubits),
-                                    1 if remap else 0, 1 if jit else 0, int(pass_cap), int(chunk_bits),
+                                    1 if remap else 0, (2 if permlane else 1) if jit else 0, int(pass_cap),
+                                    int(chunk_bits),
                                     ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(f), ptr))
     return {"passes": a.value, "stages": b.value, "fused_blocks": c.value, "fp64_ops_per_amp": f.value}, out
 

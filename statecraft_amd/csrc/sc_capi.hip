@@ -209,7 +209,7 @@ int sweep_plan_entry(Context *ctx, int n, const std::vector<PlanOp> &ops_in, Pla
     SweepCostModel cm = ctx->jit ? kCompiledPassCost : kInterpretedCost;
     if (ctx->pass_cap >= 0) cm.pass_cap = (double)ctx->pass_cap;
     cm.direct_io = cm.direct_io && ctx->direct_io;
-    cm.permlane = cm.permlane && ctx->permlane;
+    cm.permlane = ctx->jit && ctx->permlane;
     // a fixed 13-bit chunk applies from 13 qubits with runtime-compiled passes; 0 leaves the choice to the planner
     int cb = ctx->chunk_bits;
     if (!ctx->jit || (cb == kMaxChunkBits && n < kMaxChunkBits)) cb = kChunkBits;
@@ -942,6 +942,7 @@ int sc_plan_sweeps_host(int num_qubits, const sc_op *ops, uint64_t n_ops, const 
     if (rc) return rc;
     SweepCostModel cm = jit ? kCompiledPassCost : kInterpretedCost;
     if (pass_cap >= 0) cm.pass_cap = (double)pass_cap;
+    cm.permlane = jit == 2;  // in-register stage transitions (SC_OPT_PERMLANE)
     SweepPlan plan;
     if ((rc = plan_sweeps(num_qubits, pops, low_qubits, remap != 0, plan, kRegBits, cm, chunk_bits))) return rc;
     uint64_t ns = 0;

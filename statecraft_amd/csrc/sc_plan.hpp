@@ -127,7 +127,7 @@ struct SweepCostModel {
 };
 // The compiled sweep kernel: on the 30-qubit layered circuit an overhead of 1 beat 4 and 8 (measured).
 constexpr SweepCostModel kInterpretedCost{1.0, false, 0.0, false, false, false};
-constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true, true, true};
+constexpr SweepCostModel kCompiledPassCost{0.5, true, 0.0, true, true, false};
 // Adjoint passes: as the runtime-compiled passes; a pass with gradient probes applies its unit-pivot factor
 // by a scaling op at its end, and each probe's sum is corrected on the host (SweepPlan::probe_pivots).  Their
 // kernels always go through LDS at both ends.

@@ -320,6 +320,27 @@ def test_direct_hbm_passes_match_oracle(n, chunk_bits, family):
         assert any(d & 1 for *_, d in per_pass) and any(d & 2 for *_, d in per_pass), per_pass
 
 
+@pytest.mark.parametrize("n", [14, 17, 20])
+@pytest.mark.parametrize("family", ["random", "layered", "qaoa"])
+def test_in_register_transitions_match_oracle(n, family):
+    # SC_OPT_PERMLANE plans: lane bits re-ordered so that some stages are entered by exchanging register bits
+    # with lane bits 4 / 5 inside each wavefront; the layouts stay described by the LDS tables, so the host
+    # executor replays the same data movement
+    rng = np.random.default_rng(31 * n)
+    if family == "random":
+        ops, mats = random_circuit(n, 300, rng)
+    elif family == "layered":
+        ops, mats = lower(workloads.random_layered(n, 10, n))
+    else:
+        ops, mats = lower(workloads.qaoa_ring(n, 3, [0.3, 0.6, 0.9], [0.2, 0.5, 0.8], np.linspace(0.6, 1.4, n)))
+    init = random_state(n, rng)
+    st_on, out = capi.plan_sweeps_host(n, ops, mats, 4, init, False, True, -1, 12, permlane=True)
+    st_off, _ = capi.plan_sweeps_host(n, ops, mats, 4, None, False, True, -1, 12)
+    assert np.max(np.abs(out - oracle.run_ops(n, ops, mats, init))) <= 1e-12
+    # the transitions change only layouts: the same passes, stages and arithmetic
+    assert st_on == st_off
+
+
 def test_geometry_autotuner_picks_the_fastest_predicted_plan():
     """chunk_bits 0 / low_qubits -1: every measured geometry is planned and the smallest predicted time is kept."""
     n = 33
