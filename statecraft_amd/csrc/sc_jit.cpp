@@ -12,8 +12,12 @@
 // as text (kOpsHeaderText) and handed to hipRTC as a header.
 #include <hip/hiprtc.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -488,8 +492,98 @@ std::string replace_name(const std::string &body, const std::string &name) {
 const char *const kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
                                    "-Xclang", "-packed-fp32-ops"};
 
-// Compiles one module; code receives the code object.
+// ---- on-disk cache of compiled modules.  A module's code object is kept under the cache directory, named by a
+// 64-bit FNV-1a hash of everything that determines it (the hipRTC version, the options, the op library text and
+// the module source); the file starts with a magic word, a second hash of the same material and the code size,
+// so a truncated, foreign or stale file is recompiled instead of loaded.  Directory: $SC_KERNEL_CACHE_DIR (empty:
+// no cache), else $XDG_CACHE_HOME/statecraft_amd, else $HOME/.cache/statecraft_amd.  Files are written under a
+// temporary name and renamed, so concurrent processes never see a partial file.
+uint64_t fnv1a(const char *p, size_t n, uint64_t h) {
+    for (size_t i = 0; i < n; i++) {
+        h ^= (uint8_t)p[i];
+        h *= 0x100000001b3ULL;
+    }
+    return h;
+}
+
+std::string kernel_cache_dir() {
+    if (const char *e = std::getenv("SC_KERNEL_CACHE_DIR")) return e;
+    if (const char *x = std::getenv("XDG_CACHE_HOME"); x && *x) return std::string(x) + "/statecraft_amd";
+    if (const char *h = std::getenv("HOME"); h && *h) return std::string(h) + "/.cache/statecraft_amd";
+    return "";
+}
+
+constexpr char kCacheMagic[8] = {'S', 'C', 'K', 'C', 'O', 'D', 'E', '1'};
+
+struct CacheKey {
+    uint64_t name = 0, check = 0;
+};
+
+CacheKey cache_key(const std::string &src) {
+    int major = 0, minor = 0;
+    (void)hiprtcVersion(&major, &minor);
+    std::string material = std::to_string(major) + "." + std::to_string(minor) + "\n";
+    for (const char *o : kRtcOptions) material += std::string(o) + "\n";
+    CacheKey k;
+    for (uint64_t seed : {0xcbf29ce484222325ULL, 0x84222325cbf29ce4ULL}) {
+        uint64_t h = fnv1a(material.data(), material.size(), seed);
+        h = fnv1a(kOpsHeaderText, std::strlen(kOpsHeaderText), h);
+        h = fnv1a(src.data(), src.size(), h);
+        (k.name ? k.check : k.name) = h;
+    }
+    return k;
+}
+
+std::string cache_path(const std::string &dir, const CacheKey &k) {
+    char b[40];
+    std::snprintf(b, sizeof b, "/sc_%016llx.co", (unsigned long long)k.name);
+    return dir + b;
+}
+
+bool cache_load(const std::string &dir, const CacheKey &k, std::vector<char> &code) {
+    FILE *f = std::fopen(cache_path(dir, k).c_str(), "rb");
+    if (!f) return false;
+    char magic[8];
+    uint64_t check = 0, size = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCacheMagic, 8) == 0 &&
+              std::fread(&check, 8, 1, f) == 1 && check == k.check && std::fread(&size, 8, 1, f) == 1 &&
+              size > 0 && size < (1ULL << 30);
+    if (ok) {
+        code.resize(size);
+        ok = std::fread(code.data(), 1, size, f) == size && std::fgetc(f) == EOF;
+    }
+    std::fclose(f);
+    if (!ok) code.clear();
+    return ok;
+}
+
+void cache_store(const std::string &dir, const CacheKey &k, const std::vector<char> &code) {
+    // create the directory and its parent (0700); failures only mean no cache
+    const size_t slash = dir.find_last_of('/');
+    if (slash != std::string::npos && slash > 0) (void)mkdir(dir.substr(0, slash).c_str(), 0700);
+    (void)mkdir(dir.c_str(), 0700);
+    const std::string path = cache_path(dir, k);
+    char suffix[64];
+    std::snprintf(suffix, sizeof suffix, ".tmp.%ld.%zx", (long)getpid(),
+                  std::hash<std::thread::id>()(std::this_thread::get_id()));
+    const std::string tmp = path + suffix;
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const uint64_t size = code.size();
+    bool ok = std::fwrite(kCacheMagic, 1, 8, f) == 8 && std::fwrite(&k.check, 8, 1, f) == 1 &&
+              std::fwrite(&size, 8, 1, f) == 1 && std::fwrite(code.data(), 1, size, f) == size;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) (void)std::remove(tmp.c_str());
+}
+
+// Compiles one module (or takes it from the on-disk cache); code receives the code object.
 int compile_module(const std::string &src, std::vector<char> &code) {
+    const std::string dir = kernel_cache_dir();
+    CacheKey key;
+    if (!dir.empty()) {
+        key = cache_key(src);
+        if (cache_load(dir, key, code)) return SC_OK;
+    }
     hiprtcProgram prog = nullptr;
     const char *headers[] = {kOpsHeaderText};
     const char *names[] = {"sc_ops.hpp"};
@@ -515,6 +609,7 @@ int compile_module(const std::string &src, std::vector<char> &code) {
     }
     (void)hiprtcDestroyProgram(&prog);
     if (r != HIPRTC_SUCCESS) return fail(SC_E_HIP, std::string("hiprtcGetCode: ") + hiprtcGetErrorString(r));
+    if (!dir.empty()) cache_store(dir, key, code);
     return SC_OK;
 }
 

@@ -11,6 +11,14 @@ if str(ROOT) not in sys.path:
 TESTS = Path(__file__).resolve().parent
 if str(TESTS) not in sys.path:
     sys.path.insert(0, str(TESTS))
+# runtime-compiled kernels are cached on disk; the test session keeps its own cache instead of the user's
+if "SC_KERNEL_CACHE_DIR" not in os.environ:
+    import atexit
+    import shutil
+    import tempfile
+
+    os.environ["SC_KERNEL_CACHE_DIR"] = tempfile.mkdtemp(prefix="sc_kernel_cache_")
+    atexit.register(shutil.rmtree, os.environ["SC_KERNEL_CACHE_DIR"], True)
 
 
 def pytest_configure(config):

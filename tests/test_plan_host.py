@@ -443,3 +443,28 @@ def test_product_state_first_pass_compiles(n, chunk_bits, tmp_path):
     assert r["kernels"] >= 1 and r["code_bytes"] > 0
     src = (tmp_path / "sc_passes_0.hip").read_text()
     assert "const v2d pct = cmul(pc, pt);" in src
+
+
+def test_compiled_modules_are_cached_on_disk(tmp_path, monkeypatch):
+    """The second compilation of the same passes takes the code objects from the on-disk cache: same bytes,
+    much less time; a damaged cache file is recompiled and replaced."""
+    cache = tmp_path / "cache"
+    monkeypatch.setenv("SC_KERNEL_CACHE_DIR", str(cache))
+    n = 14
+    ops, mats = lower(workloads.random_layered(n, 6, 3))
+    d1, d2, d3 = tmp_path / "d1", tmp_path / "d2", tmp_path / "d3"
+    for d in (d1, d2, d3):
+        d.mkdir()
+    cold = capi.jit_compile_host(n, ops, mats, 4, False, 2, str(d1), variant=2)
+    files = sorted(cache.glob("sc_*.co"))
+    assert len(files) == 2 and not list(cache.glob("*.tmp.*"))
+    warm = capi.jit_compile_host(n, ops, mats, 4, False, 2, str(d2), variant=2)
+    assert warm["code_bytes"] == cold["code_bytes"] and warm["seconds"] < 0.5 * cold["seconds"]
+    for a, b in zip(sorted(d1.glob("*.co")), sorted(d2.glob("*.co"))):
+        assert a.read_bytes() == b.read_bytes()
+    files[0].write_bytes(files[0].read_bytes()[:100])  # truncated
+    capi.jit_compile_host(n, ops, mats, 4, False, 2, str(d3), variant=2)
+    for a, b in zip(sorted(d1.glob("*.co")), sorted(d3.glob("*.co"))):
+        assert a.read_bytes() == b.read_bytes()
+    assert files[0].stat().st_size > 1000
+
