@@ -67,6 +67,8 @@ def make_context(args) -> capi.Context:
         ctx.set_option(capi.OPT_PASS_SCHED, args.pass_sched)
     if args.permlane is not None:
         ctx.set_option(capi.OPT_PERMLANE, args.permlane)
+    if args.fused_terms is not None:
+        ctx.set_option(capi.OPT_FUSED_TERMS, args.fused_terms)
     return ctx
 
 
@@ -75,7 +77,7 @@ def context_options(ctx) -> dict:
             "low_qubits_option": ctx.get_option(capi.OPT_LOW_QUBITS),
             "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS),
             "pass_sched": ctx.get_option(capi.OPT_PASS_SCHED), "mfma": ctx.get_option(capi.OPT_MFMA),
-            "permlane": ctx.get_option(capi.OPT_PERMLANE)}
+            "permlane": ctx.get_option(capi.OPT_PERMLANE), "fused_terms": ctx.get_option(capi.OPT_FUSED_TERMS)}
 
 
 def host_threads() -> int:
@@ -600,6 +602,8 @@ def main(argv=None) -> int:
     ap.add_argument("--permlane", type=int, default=None, choices=[0, 1],
                     help="SC_OPT_PERMLANE: stage transitions inside the registers where the plan allows (library "
                          "default 0)")
+    ap.add_argument("--fused-terms", type=int, default=None, choices=[0, 1],
+                    help="SC_OPT_FUSED_TERMS: c4's Z-string terms summed in each job's last pass (library default 1)")
     ap.add_argument("--mfma", type=int, default=None, choices=[0, 1, 2],
                     help="SC_OPT_MFMA, fused mode: 0 VALU blocks only, 1 5-qubit blocks on the FP64 matrix cores "
                          "(library default), 2 also 4-qubit blocks")

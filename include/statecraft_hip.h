@@ -131,6 +131,9 @@ typedef struct sc_op {
                                    inside each wavefront (v_permlane32_swap / v_permlane16_swap) instead of through
                                    LDS; default 0 (measured neutral: the transitions it can take are the wave-local
                                    ones, whose LDS traffic already overlaps the other workgroup's arithmetic) */
+#define SC_OPT_FUSED_TERMS 15   /* 1: sc_run_jobs sums a job's Z-string terms in its last sweep pass as the states are
+                                   stored (one read of the batch fewer); results agree with the separate calls to
+                                   rounding instead of bit for bit; 0: a term kernel after the circuit; default 1 */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */
@@ -267,7 +270,8 @@ typedef struct sc_job {
     int32_t reserved;
     double *out;
 } sc_job;
-/* Runs the jobs in order with results identical to the separate calls.  With runtime-compiled sweep passes
+/* Runs the jobs in order with results identical to the separate calls (with SC_OPT_FUSED_TERMS, a job whose
+ * terms are all Z strings sums them in its last pass: equal to rounding).  With runtime-compiled sweep passes
  * (SC_OPT_JIT > 0, every state of at least 12 qubits) the first call of a job structure (shapes, circuit
  * structure and matrix classes, bases, terms) records pool uploads, every pass launch, the term kernels and
  * the result copies as one HIP graph; every later call rebuilds only the matrix pools and launches that graph

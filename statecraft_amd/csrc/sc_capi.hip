@@ -558,6 +558,9 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
     case SC_OPT_PERMLANE:
         ctx->permlane = value != 0;
         return SC_OK;
+    case SC_OPT_FUSED_TERMS:
+        ctx->fused_terms = value != 0;
+        return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -580,6 +583,7 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_MFMA: *value = ctx->mfma; return SC_OK;
     case SC_OPT_PASS_SCHED: *value = ctx->pass_sched; return SC_OK;
     case SC_OPT_PERMLANE: *value = ctx->permlane; return SC_OK;
+    case SC_OPT_FUSED_TERMS: *value = ctx->fused_terms; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }

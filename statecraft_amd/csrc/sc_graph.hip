@@ -2,7 +2,9 @@
 //
 // sc_run_jobs takes a list of jobs (a batch of states, a circuit with one matrix array per element, Pauli
 // terms) and does for each what sc_state_set_basis + sc_apply_ops_batched + sc_expectation_pauli_terms do,
-// with the same plans, kernels and partial sums, so the results are the same bit for bit.  The first call of
+// with the same plans and kernels.  When every term of a job is a Z string (SC_OPT_FUSED_TERMS), its last pass
+// also sums the terms while it stores the states, instead of a term kernel reading the batch again; the
+// results then agree with the separate calls to rounding (other summation order), otherwise bit for bit.  The first call of
 // a job structure plans every circuit (runtime-compiled passes), allocates the matrix pools, the partial sums
 // and pinned staging once, and records the whole sequence -- pool uploads, every pass launch of every job,
 // the term kernels and the copies of the row sums back -- as one hipGraph on the context's stream.  Later
@@ -28,6 +30,11 @@ namespace {
 struct GraphJob {
     SweepPlan plan;                  // the job's plan (kept: the context's plan cache may evict its entry)
     std::vector<hipFunction_t> fns;  // its pass kernels (cached for the life of the process)
+    // Z-string terms only (every x mask 0, at most 32 terms, a batch in one launch): the last pass sums the
+    // terms as it stores the states (terms_fn, tparts partial slots per row) instead of a term kernel reading
+    // the batch again
+    hipFunction_t terms_fn = nullptr;
+    int tparts = 0;
     size_t pool_off = 0, pool_bytes = 0;      // device and host offsets of the per-element pools
     size_t scratch_off = 0, scratch_bytes = 0;  // device: term partial sums and row sums
     size_t rows_off = 0, rows_bytes = 0;      // device row sums / host staging of them
@@ -91,7 +98,8 @@ int record_job(Context *ctx, const sc_job &job, GraphJob &gj, char *dev, char *h
     const size_t stride = gj.batch > 1 ? gj.plan.pool_complex : 0;
     const int log_chunks = st->n - gj.plan.chunk_bits;
     const int per_launch = std::max(1, (int)std::min<uint64_t>(gj.batch, (1ULL << 20) >> log_chunks));
-    for (size_t p = 0; p < gj.plan.passes.size(); p++)
+    const size_t fused_last = gj.terms_fn ? gj.plan.passes.size() - 1 : gj.plan.passes.size();
+    for (size_t p = 0; p < fused_last; p++)
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             const int cnt = std::min(per_launch, st->batch - e0);
             int rc = launch_sweep_jit(ctx, st, gj.fns[p], dpool, e0, cnt, stride, gj.plan.chunk_bits, st->amp, st->amp,
@@ -99,8 +107,10 @@ int record_job(Context *ctx, const sc_job &job, GraphJob &gj, char *dev, char *h
             if (rc) return rc;
         }
     double *rows = (double *)(dev + gj.rows_off);
-    int rc = pauli_terms_enqueue(ctx, st, st->amp, job.xmask, job.zmask, job.n_terms, dev + gj.scratch_off,
-                                 gj.scratch_bytes, rows);
+    int rc = gj.terms_fn ? launch_sweep_jit_terms(ctx, st, gj.terms_fn, dpool, stride, gj.plan.chunk_bits, job.n_terms,
+                                                  (double *)(dev + gj.scratch_off), gj.tparts, rows)
+                         : pauli_terms_enqueue(ctx, st, st->amp, job.xmask, job.zmask, job.n_terms, dev + gj.scratch_off,
+                                               gj.scratch_bytes, rows);
     if (rc) return rc;
     SC_HIP_CHECK(hipMemcpyAsync(host + gj.host_rows_off, rows, gj.rows_bytes, hipMemcpyDeviceToHost, ctx->stream));
     return SC_OK;
@@ -122,8 +132,22 @@ int build_graph(Context *ctx, const sc_job *jobs, int n_jobs, std::vector<PlanCa
     }
     for (int j = 0; j < n_jobs; j++) {
         GraphJob &gj = g.jobs[j];
-        const uint64_t rows = (uint64_t)jobs[j].n_terms;
-        gj.scratch_bytes = term_scratch_bytes(gj.batch, rows, term_parts(ctx, gj.batch, jobs[j].state->n, rows));
+        const sc_job &job = jobs[j];
+        const uint64_t rows = (uint64_t)job.n_terms;
+        const int n = job.state->n, log_chunks = n - gj.plan.chunk_bits;
+        // (the first pass generates the basis input, so the term pass must be a later one)
+        bool fuse = ctx->fused_terms && gj.batch > 1 && job.n_terms <= 32 && gj.plan.passes.size() >= 2 &&
+                    (gj.batch << log_chunks) <= (1ULL << 20);
+        for (int t = 0; t < job.n_terms && fuse; t++) fuse = job.xmask[t] == 0;
+        if (fuse) {
+            const std::vector<uint64_t> z(job.zmask, job.zmask + job.n_terms);
+            int rc = jit_last_pass_terms_kernel(ctx, gj.plan, z, gj.terms_fn);
+            if (rc) return rc;
+            gj.tparts = sweep_terms_parts(ctx, n, gj.plan.chunk_bits, (int)gj.batch);
+            gj.scratch_bytes = (size_t)(gj.batch * rows * (uint64_t)gj.tparts) * sizeof(double);
+        } else {
+            gj.scratch_bytes = term_scratch_bytes(gj.batch, rows, term_parts(ctx, gj.batch, n, rows));
+        }
         gj.scratch_off = dev_bytes;
         dev_bytes = align256(dev_bytes + gj.scratch_bytes);
         gj.rows_bytes = (size_t)(gj.batch * rows) * sizeof(double);
@@ -212,7 +236,7 @@ int run_jobs(Context *ctx, const sc_job *jobs, int n_jobs, int *used_graph) {
     // plans, and the graph key: every job's shape, plan, basis and terms, and the launch geometry
     std::vector<PlanCache::Entry *> ents(n_jobs);
     std::string key;
-    const int32_t geo[2] = {ctx->cu_count, ctx->sweep_blocks_per_cu};
+    const int32_t geo[3] = {ctx->cu_count, ctx->sweep_blocks_per_cu, ctx->fused_terms};
     put(key, geo, sizeof geo);
     for (int j = 0; j < n_jobs; j++) {
         int rc = sweep_plan_entry(ctx, jobs[j].state->n, pops[j], &ents[j], true);

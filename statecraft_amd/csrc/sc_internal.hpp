@@ -70,6 +70,7 @@ struct Context {
     int direct_io = 1;    // SC_OPT_DIRECT_IO
     int mfma = 1;         // SC_OPT_MFMA
     int permlane = 0;     // SC_OPT_PERMLANE
+    int fused_terms = 1;  // SC_OPT_FUSED_TERMS
     int pass_sched = 4;   // SC_OPT_PASS_SCHED
     int cu_count = 256;
     Profiler prof;

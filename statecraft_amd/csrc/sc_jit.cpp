@@ -136,18 +136,24 @@ std::string op_call(const SweepOp &o, const char *S, const char *D) {
 // A pass with direct loads (SweepPass::direct bit 0) reads HBM straight into its first stage's registers,
 // one with direct stores (bit 1) writes its last stage's registers straight to HBM; the LDS round trips of
 // the load and store phases are then skipped.
-std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch, int sched = 0) {
+std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetch, int sched = 0,
+                        const std::vector<uint64_t> *zterms = nullptr) {
     std::ostringstream s;
     const int TB = plan.chunk_bits - kRegBits;  // thread-index bits: 8 (256 threads) or 9 (512)
     const int S = ps.nstages;
     const bool DL = S > 0 && (ps.direct & 1) != 0, DS = S > 0 && (ps.direct & 2) != 0;
+    const int NT = zterms ? (int)zterms->size() : 0;
     s << "extern \"C\" __global__ void __launch_bounds__(" << (1 << TB)
       << ") __attribute__((amdgpu_waves_per_eu(2, 2)))\n"
          // amp and out may be the same buffer: every chunk is read once and written once, by one workgroup,
          // so no access of one pointer can observe an access of the other
          "@K@(sc::v2d *__restrict__ amp, sc::v2d *__restrict__ out, int n, int log_chunks, uint64_t total_chunks,\n"
-         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin, long long basis) {\n"
+         "    const sc::v2d *__restrict__ mats_arg, uint64_t mat_stride, int elem_begin, long long basis"
+      << (NT ? ",\n    double *__restrict__ tpart, int tparts" : "") << ") {\n"
          "  using namespace sc;\n";
+    // Z-string terms (zterms, one mask each): the pass also sums sign_t(i) |a_i|^2 over every amplitude it stores,
+    // per batch element, into tpart (see below)
+    if (NT) s << "  __shared__ double tsum[" << (1 << TB) / 64 << "][" << NT << "];\n";
     // stage st > 0 with SweepStage::perm is entered inside the registers, with no LDS round trip
     auto in_reg = [&](int st) { return st > 0 && st < S && ps.stages[st].perm != 0; };
     bool uses_lds = !DL || !DS;
@@ -177,6 +183,15 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     const uint64_t *s_thread = DS ? ps.sd_thread : ps.glob_thread, *s_reg = DS ? ps.sd_reg : ps.glob_reg;
     s << "  const uint64_t gload = " << lane_expr("tid", l_thread, TB, "+", true) << ";\n";
     s << "  const uint64_t gstore = " << lane_expr("tid", s_thread, TB, "+", true) << ";\n";
+    // term t's sign at index cb + gstore + s_reg[r] is (-1)^(parity(z_t & gstore) + parity(z_t & cb) +
+    // parity(z_t & s_reg[r])): the lane's part once here, the chunk's part per chunk (uniform), the register
+    // part through a Walsh-Hadamard transform of the 16 probabilities over the register bits
+    if (NT) {
+        s << "  uint32_t tlane = 0u;\n  double tacc[" << NT << "];\n";
+        for (int t = 0; t < NT; t++)
+            s << "  tlane |= (uint32_t)(__builtin_popcountll(" << hex64((*zterms)[t]) << " & gstore) & 1) << " << t
+              << ";\n  tacc[" << t << "] = 0.0;\n";
+    }
     // LDS addresses are byte offsets (slot * 16), so that each access costs one XOR
     if (uses_lds) {
         s << "  char *const ldsb = (char *)lds;\n";
@@ -333,6 +348,27 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
         cur = "a";
     }
     const std::string fin = cur ? cur : "a";
+    if (NT) {
+        s << "    {\n    double w[16];\n";
+        for (int r = 0; r < kRegs; r++)
+            s << "    w[" << r << "] = fma(" << fin << "[" << r << "].x, " << fin << "[" << r << "].x, " << fin << "[" << r
+              << "].y * " << fin << "[" << r << "].y);\n";
+        for (int b = 0; b < kRegBits; b++)
+            for (int r = 0; r < kRegs; r++)
+                if (!((r >> b) & 1))
+                    s << "    { const double u = w[" << r << "], v = w[" << (r | (1 << b)) << "]; w[" << r
+                      << "] = u + v; w[" << (r | (1 << b)) << "] = u - v; }\n";
+        s << "    uint32_t tsig = tlane;\n";
+        for (int t = 0; t < NT; t++) {
+            const uint64_t z = (*zterms)[t];
+            int m = 0;
+            for (int j = 0; j < kRegBits; j++)
+                if (s_reg[1 << j] & z) m |= 1 << j;
+            s << "    tsig ^= (uint32_t)(__builtin_popcountll(" << hex64(z) << " & cb) & 1) << " << t << ";\n"
+              << "    tacc[" << t << "] += ((tsig >> " << t << ") & 1u) ? -w[" << m << "] : w[" << m << "];\n";
+        }
+        s << "    }\n";
+    }
     // without prefetch the next chunk's loads go into p: after the store when p holds the results
     if (!prefetch && fin != "p") next_loads("p");
     s << "    {\n"
@@ -340,6 +376,25 @@ std::string pass_source(const SweepPlan &plan, const SweepPass &ps, bool prefetc
     regs_store("dst", fin.c_str());
     s << "    }\n";
     if (!prefetch && fin == "p") next_loads("p");
+    // Term sums leave a workgroup when its contiguous chunk range moves to another element (or ends): reduced
+    // over each wavefront, then over the wavefronts in a fixed order, into tpart[(element * NT + t) * tparts +
+    // slot], slot = this workgroup's place among those whose ranges cover the element (blocked assignment only).
+    if (NT) {
+        s << "    {\n    const uint64_t ce = c >> log_chunks;\n"
+             "    if (next >= c_end || (next >> log_chunks) != ce) {\n";
+        for (int t = 0; t < NT; t++)
+            s << "      { double v = tacc[" << t << "]; for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);\n"
+                 "        if ((tid & 63u) == 0u) tsum[tid >> 6][" << t << "] = v; tacc[" << t << "] = 0.0; }\n";
+        s << "      __syncthreads();\n"
+             "      if (tid < " << NT << "u) {\n"
+             "        double v = tsum[0][tid];\n";
+        for (int wv = 1; wv < (1 << TB) / 64; wv++) s << "        v += tsum[" << wv << "][tid];\n";
+        s << "        const uint64_t wf = (ce << log_chunks) / span;\n"
+             "        tpart[(ce * " << NT << "ULL + tid) * (uint64_t)tparts + (blockIdx.x - wf)] = v;\n"
+             "      }\n"
+             "      __syncthreads();\n"
+             "    }\n    }\n";
+    }
     s << "    if (next >= c_end) break;\n"
          "    c = next;\n"
          "    cb = ncb;\n";
@@ -710,6 +765,16 @@ int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vecto
 
 }  // namespace
 
+int jit_last_pass_terms_kernel(Context *ctx, const SweepPlan &plan, const std::vector<uint64_t> &zterms,
+                               hipFunction_t &out) {
+    std::vector<std::string> bodies{pass_source(plan, plan.passes.back(), ctx->jit == 2, ctx->pass_sched, &zterms)};
+    std::vector<hipFunction_t> fns;
+    int rc = jit_kernels(ctx, bodies, fns);
+    if (rc) return rc;
+    out = fns[0];
+    return SC_OK;
+}
+
 int jit_pass_kernels(Context *ctx, const SweepPlan &plan, std::vector<hipFunction_t> &out) {
     std::vector<std::string> bodies;
     for (size_t p = 0; p < plan.passes.size(); p++)
@@ -767,16 +832,72 @@ int jit_compile_host(const SweepPlan &plan, int variant, int threads, double *se
     return SC_OK;
 }
 
+uint64_t sweep_jit_grid(Context *ctx, int chunk_bits, uint64_t total_chunks) {
+    // 12-qubit chunks: as many 256-thread workgroups per CU as k_sweep's occupancy allows (two 64 KiB LDS
+    // chunks); 13-qubit chunks: one 512-thread workgroup per CU (128 KiB of LDS)
+    const uint64_t resident = chunk_bits == kChunkBits ? (uint64_t)sweep_grid_size(ctx) : (uint64_t)ctx->cu_count;
+    return std::min<uint64_t>(total_chunks, resident);
+}
+
+int sweep_terms_parts(Context *ctx, int n, int chunk_bits, int elem_count) {
+    const uint64_t cpe = 1ULL << (n - chunk_bits), total = (uint64_t)elem_count * cpe;
+    const uint64_t grid = sweep_jit_grid(ctx, chunk_bits, total);
+    const uint64_t span = (total + grid - 1) / grid;
+    // an element's chunks [e cpe, (e+1) cpe) meet at most ceil(cpe / span) + 1 workgroup ranges
+    return (int)((cpe + span - 1) / span + 1);
+}
+
+namespace {
+
+// rows[e * nt + t] = the slots of (e, t) in workgroup order (only the slots the covering workgroups wrote)
+__global__ void k_terms_rows(const double *__restrict__ tpart, uint64_t elems, int nt, int tparts, uint64_t span,
+                             int log_chunks, double *__restrict__ rows) {
+    const uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (r >= elems * (uint64_t)nt) return;
+    const uint64_t e = r / (uint64_t)nt;
+    const uint64_t w0 = (e << log_chunks) / span, w1 = (((e + 1) << log_chunks) - 1) / span;
+    double v = 0.0;
+    for (uint64_t k = 0; k <= w1 - w0; k++) v += tpart[r * (uint64_t)tparts + k];
+    rows[r] = v;
+}
+
+}  // namespace
+
+int launch_sweep_jit_terms(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, size_t mat_stride,
+                           int chunk_bits, int n_terms, double *tpart, int tparts, double *rows) {
+    if (mat_stride == 0) return fail(SC_E_STATE, "internal error: term sums need one matrix pool per element");
+    int log_chunks = st->n - chunk_bits;
+    uint64_t total = (uint64_t)st->batch << log_chunks;
+    const int threads = 1 << (chunk_bits - kRegBits);
+    const uint64_t grid = sweep_jit_grid(ctx, chunk_bits, total);
+    const uint64_t span = (total + grid - 1) / grid;
+    if ((uint64_t)tparts < ((1ULL << log_chunks) + span - 1) / span + 1)
+        return fail(SC_E_STATE, "internal error: too few term partial slots");
+    v2d *amp = reinterpret_cast<v2d *>(st->amp);
+    v2d *out = amp;
+    const v2d *mats = reinterpret_cast<const v2d *>(mats_dev);
+    int n = st->n, elem_begin = 0;
+    uint64_t stride = (uint64_t)mat_stride;
+    long long basis = -1;
+    void *args[] = {&amp, &out, &n, &log_chunks, &total, &mats, &stride, &elem_begin, &basis, &tpart, &tparts};
+    {
+        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 32.0 * (double)((uint64_t)st->batch << st->n));
+        SC_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, threads, 1, 1, 0, ctx->stream, args, nullptr));
+    }
+    const uint64_t nrows = (uint64_t)st->batch * (uint64_t)n_terms;
+    hipLaunchKernelGGL(k_terms_rows, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, ctx->stream, tpart,
+                       (uint64_t)st->batch, n_terms, tparts, span, log_chunks, rows);
+    SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
+}
+
 int launch_sweep_jit(Context *ctx, State *st, hipFunction_t fn, const double *mats_dev, int elem_begin,
                      int elem_count, size_t mat_stride, int chunk_bits, const double2 *src, double2 *dst,
                      long long basis) {
     int log_chunks = st->n - chunk_bits;
     uint64_t total = (uint64_t)elem_count << log_chunks;
-    // 12-qubit chunks: as many 256-thread workgroups per CU as k_sweep's occupancy allows (two 64 KiB LDS
-    // chunks); 13-qubit chunks: one 512-thread workgroup per CU (128 KiB of LDS)
     const int threads = 1 << (chunk_bits - kRegBits);
-    const uint64_t resident = chunk_bits == kChunkBits ? (uint64_t)sweep_grid_size(ctx) : (uint64_t)ctx->cu_count;
-    const uint64_t grid = std::min<uint64_t>(total, resident);
+    const uint64_t grid = sweep_jit_grid(ctx, chunk_bits, total);
     // a pass reads every chunk from src and writes it to dst (the same addresses; in place when equal)
     v2d *amp = reinterpret_cast<v2d *>(const_cast<double2 *>(src ? src : st->amp));
     v2d *out = reinterpret_cast<v2d *>(dst ? dst : st->amp);
