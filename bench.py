@@ -468,9 +468,10 @@ def bench_c4(args):
     tstate = capi.State(ctx, nq, nt)
 
     def trajectories():
+        # the noisy circuit and every trajectory's parity <Z^20> (summed by the circuit's last pass)
         tstate.set_basis(0)
-        tstate.run_depolarizing_trajectories(gops, gmats, noise_after, 0.01, u)
-        parity = tstate.expectation([0], [(1 << nq) - 1], [1.0])
+        parity = tstate.run_depolarizing_trajectories_z(gops, gmats, noise_after, 0.01, u,
+                                                        np.array([(1 << nq) - 1], np.uint64))[:, 0]
         ctx.synchronize()
         return parity
 

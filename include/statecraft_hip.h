@@ -221,6 +221,14 @@ int sc_apply_kraus(sc_state *st, int elem, int target, const double *kraus_op, d
 int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
                                      uint64_t n_mat_complex, const uint8_t *noise_after, double p,
                                      const double *uniforms, uint64_t n_draws);
+/* sc_run_depolarizing_trajectories, then out[b * n_terms + t] = <Z^zmask[t]>_b of every trajectory (no
+ * reference counterpart: the trajectory-averaged observables of a noisy batch).  With runtime-compiled passes
+ * and SC_OPT_FUSED_TERMS the circuit's last pass sums the terms as it stores the states; otherwise the term
+ * kernel reads the batch afterwards. */
+int sc_run_depolarizing_trajectories_z(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
+                                       uint64_t n_mat_complex, const uint8_t *noise_after, double p,
+                                       const double *uniforms, uint64_t n_draws, const uint64_t *zmask, int n_terms,
+                                       double *out);
 
 /* ---- observables and gradients (no reference counterpart) ----------------- */
 /* Pauli-sum observable H = sum_t coeff[t] * P_t with P_t given by an x-mask and z-mask per term (Y = X and Z

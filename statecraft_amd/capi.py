@@ -82,7 +82,8 @@ EXPORTED = [
     "sc_ctx_synchronize", "sc_ctx_device_info", "sc_state_create", "sc_state_destroy", "sc_state_qubits",
     "sc_state_set_basis", "sc_state_upload", "sc_state_download", "sc_state_copy", "sc_apply_ops",
     "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
-    "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_expectation_pauli",
+    "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_run_depolarizing_trajectories_z",
+    "sc_expectation_pauli",
     "sc_expectation_pauli_terms",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
     "sc_plan_sweeps_host", "sc_state_read_range", "sc_jit_compile_host", "sc_adjoint_plan_host",
@@ -131,6 +132,8 @@ def _sig(lib):
         "sc_kraus_weights": (i32, [P, i32, i32, DP, i32, DP]),
         "sc_apply_kraus": (i32, [P, i32, i32, DP, DP]),
         "sc_run_depolarizing_trajectories": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64]),
+        "sc_run_depolarizing_trajectories_z": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64, U64P, i32,
+                                                     DP]),
         "sc_expectation_pauli": (i32, [P, U64P, U64P, DP, i32, DP]),
         "sc_expectation_pauli_terms": (i32, [P, U64P, U64P, i32, DP]),
         "sc_adjoint_gradient": (i32, [P, P, u64, DP, u64, i32, U64P, U64P, DP, i32, i32, i32, DP, DP]),
@@ -568,6 +571,20 @@ class State:
         assert u.ndim == 2 and u.shape[0] == self.batch
         check(lib().sc_run_depolarizing_trajectories(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
                                                      na.ctypes.data_as(U8P), float(p), dptr(u), u.shape[1]))
+
+    def run_depolarizing_trajectories_z(self, ops, mats, noise_after, p: float, uniforms, zmask) -> np.ndarray:
+        """run_depolarizing_trajectories, then <Z^zmask[t]> of every trajectory: array [batch, n_terms]."""
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+        na = np.ascontiguousarray(noise_after, np.uint8)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        zm = np.ascontiguousarray(zmask, np.uint64)
+        assert u.ndim == 2 and u.shape[0] == self.batch
+        out = np.empty((self.batch, len(zm)))
+        check(lib().sc_run_depolarizing_trajectories_z(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
+                                                       na.ctypes.data_as(U8P), float(p), dptr(u), u.shape[1],
+                                                       zm.ctypes.data_as(U64P), len(zm), dptr(out)))
+        return out
 
     def close(self) -> None:
         if getattr(self, "h", None):

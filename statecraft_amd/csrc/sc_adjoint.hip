@@ -1092,9 +1092,11 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     return SC_OK;
 }
 
-int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
-                                     uint64_t n_mat_complex, const uint8_t *noise_after, double p,
-                                     const double *uniforms, uint64_t n_draws) {
+namespace {
+
+int run_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                     const uint8_t *noise_after, double p, const double *uniforms, uint64_t n_draws,
+                     ZTermsRequest *terms) {
     if (!st) return fail(SC_E_NULL, "state is null");
     if (n_ops > 0 && (!ops || !noise_after)) return fail(SC_E_NULL, "ops is null");
     if (!uniforms) return fail(SC_E_NULL, "uniforms is null");
@@ -1209,7 +1211,37 @@ int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_
         op.targets = 1ULL << op.q0;
         op.diag = 0;
     }
-    return apply_planned(st, pops, elems, true);
+    return apply_planned(st, pops, elems, true, terms);
+}
+
+}  // namespace
+
+int sc_run_depolarizing_trajectories(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
+                                     uint64_t n_mat_complex, const uint8_t *noise_after, double p,
+                                     const double *uniforms, uint64_t n_draws) {
+    return run_trajectories(st, ops, n_ops, mats, n_mat_complex, noise_after, p, uniforms, n_draws, nullptr);
+}
+
+int sc_run_depolarizing_trajectories_z(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats,
+                                       uint64_t n_mat_complex, const uint8_t *noise_after, double p,
+                                       const double *uniforms, uint64_t n_draws, const uint64_t *zmask, int n_terms,
+                                       double *out) {
+    if (!st) return fail(SC_E_NULL, "state is null");
+    if (!zmask || !out) return fail(SC_E_NULL, "zmask and out are required");
+    if (n_terms <= 0) return fail(SC_E_ARG, "n_terms must be positive");
+    const uint64_t dim = 1ULL << st->n;
+    for (int t = 0; t < n_terms; t++)
+        if (zmask[t] >= dim) return fail(SC_E_ARG, "Pauli mask beyond the state's qubits");
+    ZTermsRequest req;
+    req.z.assign(zmask, zmask + n_terms);
+    req.vals = out;
+    int rc = run_trajectories(st, ops, n_ops, mats, n_mat_complex, noise_after, p, uniforms, n_draws, &req);
+    if (rc || req.done) return rc;
+    // the circuit ran without the term sums (conditions of the fused pass not met): evaluate them now
+    const std::vector<uint64_t> x((size_t)n_terms, 0ULL);
+    if ((rc = set_device(st->ctx))) return rc;
+    if ((rc = materialize_basis(st))) return rc;
+    return pauli_terms(st->ctx, st, st->amp, x.data(), zmask, n_terms, out);
 }
 
 }  // extern "C"

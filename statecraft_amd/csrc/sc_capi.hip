@@ -248,7 +248,7 @@ namespace {
 // src / dst (runtime-compiled passes only): the first pass reads the states from src and the last one
 // writes them to dst; passes in between work in st->amp.  nullptr means st->amp.
 int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const std::vector<const double *> &mats_elems,
-               const double2 *src = nullptr, double2 *dst = nullptr) {
+               const double2 *src = nullptr, double2 *dst = nullptr, ZTermsRequest *terms = nullptr) {
     PlanCache::Entry *ent = nullptr;
     {
         int rc = sweep_plan_entry(ctx, st->n, ops, &ent, st->pending_basis >= 0);
@@ -325,7 +325,29 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     // keep each launch's grid under ~2^20 blocks
     const int log_chunks = st->n - plan.chunk_bits;
     int per_launch = std::max(1, (int)std::min<uint64_t>((uint64_t)st->batch, (1ULL << 20) >> log_chunks));
+    // Z-string terms summed by the last pass: per-element pools (contiguous chunk ranges per workgroup), every
+    // element in one launch, in place, and a last pass that is not the first (which generates a basis input)
+    hipFunction_t terms_fn = nullptr;
+    int tparts = 0;
+    double *tpart = nullptr, *trows = nullptr;
+    if (terms && ctx->fused_terms && !records && stride != 0 && plan.passes.size() >= 2 && per_launch >= st->batch &&
+        src == st->amp && dst == st->amp && !terms->z.empty() && terms->z.size() <= 32) {
+        if ((rc = jit_last_pass_terms_kernel(ctx, plan, terms->z, terms_fn))) return rc;
+        tparts = sweep_terms_parts(ctx, st->n, plan.chunk_bits, st->batch);
+        const size_t rows = (size_t)st->batch * terms->z.size();
+        if ((rc = ensure_scratch(ctx, (rows * (size_t)tparts + rows) * sizeof(double)))) return rc;
+        tpart = (double *)ctx->scratch;
+        trows = tpart + rows * (size_t)tparts;
+    }
     for (size_t p = 0; p < plan.passes.size(); p++) {
+        if (terms_fn && p + 1 == plan.passes.size()) {
+            rc = launch_sweep_jit_terms(ctx, st, terms_fn, dpool, stride, plan.chunk_bits, (int)terms->z.size(), tpart,
+                                        tparts, trows);
+            if (rc) return rc;
+            ctx->last_launches++;
+            ctx->last_sweeps++;
+            continue;
+        }
         for (int e0 = 0; e0 < st->batch; e0 += per_launch) {
             int cnt = std::min(per_launch, st->batch - e0);
             const double2 *in = p == 0 ? src : st->amp;
@@ -338,6 +360,11 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
         }
         ctx->last_sweeps++;
         st->pending_basis = -1;
+    }
+    if (terms_fn) {
+        SC_HIP_CHECK(hipMemcpyAsync(terms->vals, trows, (size_t)st->batch * terms->z.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        terms->done = true;
     }
     ctx->last_blocks += plan.fused_blocks;
     // the pinned staging buffer and plan_dev are reused by the next call
@@ -362,7 +389,7 @@ int materialize_basis(State *st) {
 }
 
 int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
-                  bool per_elem) {
+                  bool per_elem, ZTermsRequest *terms) {
     Context *ctx = st->ctx;
     ctx->last_sweeps = ctx->last_launches = ctx->last_blocks = 0;
     if (plan_ops.empty()) return SC_OK;
@@ -386,7 +413,7 @@ int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vec
     } else if (mode == SC_EXEC_FUSED) {
         rc = exec_fused(ctx, st, plan_ops, elems[0]);
     } else {
-        rc = exec_sweep(ctx, st, plan_ops, elems);
+        rc = exec_sweep(ctx, st, plan_ops, elems, nullptr, nullptr, terms);
     }
     if (rc) return rc;
     SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));

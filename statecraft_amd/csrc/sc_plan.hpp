@@ -44,10 +44,17 @@ constexpr int kPlanGrad = 100;
 // with SC_E_UNSUPPORTED (StatevectorEngine.java:92-95).
 int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const double *> &mats_all,
                 uint64_t n_mat_complex, std::vector<PlanOp> &out);
+// Z-string terms summed by a circuit's last runtime-compiled pass (SC_OPT_FUSED_TERMS): vals[b * terms + t] =
+// <Z_t>_b of the output states.  `done` reports whether the pass did it (else the caller evaluates the terms).
+struct ZTermsRequest {
+    std::vector<uint64_t> z;
+    double *vals = nullptr;
+    bool done = false;
+};
 // Apply analysed ops to a state (one matrix array per batch element with per_elem, else elems[0] for all),
 // in the context's execution mode (sc_capi.hip).
 int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
-                  bool per_elem);
+                  bool per_elem, ZTermsRequest *terms = nullptr);
 
 bool ops_conflict(const PlanOp &a, const PlanOp &b);
 

@@ -105,6 +105,34 @@ def test_batched_trajectories(hip_ctx, n, B, p):
     st.close()
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("n,B", [(14, 48), (20, 24)])
+def test_trajectories_with_z_terms(n, B, fused):
+    """sc_run_depolarizing_trajectories_z: the GHZ parity and other Z strings of every trajectory, summed by the
+    noisy circuit's last pass (SC_OPT_FUSED_TERMS 1) or by the term kernel afterwards (0), against the plain
+    call followed by the term kernel (1e-14) and the oracle (1e-12)."""
+    ctx = capi.Context(0)  # library defaults: runtime-compiled passes
+    ctx.set_option(capi.OPT_FUSED_TERMS, fused)
+    try:
+        ops, mats, noise_after, u = workloads.ghz_trajectory_uniforms(B, n, 77)
+        z = np.array([(1 << n) - 1, 1, (1 << (n - 1)) | 1, 0b1011 << 3, 0], np.uint64)
+        st = capi.State(ctx, n, B)
+        st.set_basis(0)
+        got = st.run_depolarizing_trajectories_z(ops, mats, noise_after, 0.05, u, z)
+        states = st.download()
+        st.set_basis(0)
+        st.run_depolarizing_trajectories(ops, mats, noise_after, 0.05, u)
+        want = st.expectation_terms(np.zeros(len(z), np.uint64), z)
+        assert np.max(np.abs(got - want)) <= 1e-14
+        assert np.array_equal(states, st.download())
+        for b in (0, B - 1):
+            ref = [oracle.expect_pauli(states[b], [0], [int(zz)], [1.0]) for zz in z]
+            assert np.max(np.abs(got[b] - ref)) <= EPS
+        st.close()
+    finally:
+        ctx.close()
+
+
 def test_pauli_expectations(hip_ctx):
     rng = np.random.default_rng(17)
     n, B = 13, 3
