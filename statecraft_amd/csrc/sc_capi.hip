@@ -288,22 +288,10 @@ int exec_sweep(Context *ctx, State *st, const std::vector<PlanOp> &ops, const st
     double *hpool = (double *)(h + pass_bytes + ops_bytes);
     // per-element pools are independent: large batches build them on several host threads
     const size_t pool_work = nelem_pool * (plan.pool.size() + 1);
-    const unsigned nthreads = (unsigned)std::min<size_t>(
-        {nelem_pool, pool_work >= 4096 ? (size_t)std::max(1u, std::thread::hardware_concurrency()) : 1, 16});
-    auto build_range = [&](size_t e0, size_t e1) {
-        for (size_t e = e0; e < e1; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e, basis);
-    };
-    if (nthreads <= 1) {
-        build_range(0, nelem_pool);
-    } else {
-        std::vector<std::thread> pool_threads;
-        const size_t per = (nelem_pool + nthreads - 1) / nthreads;
-        for (unsigned t = 0; t < nthreads; t++) {
-            const size_t e0 = t * per, e1 = std::min(nelem_pool, e0 + per);
-            if (e0 < e1) pool_threads.emplace_back(build_range, e0, e1);
-        }
-        for (auto &th : pool_threads) th.join();
-    }
+    const uint64_t grain = pool_work < 4096 ? nelem_pool : std::max<uint64_t>(1, nelem_pool / (4 * kHostThreads));
+    parallel_ranges(nelem_pool, grain, [&](unsigned, uint64_t e0, uint64_t e1) {
+        for (uint64_t e = e0; e < e1; e++) build_pool(plan, mats_elems[e], hpool + 2 * plan.pool_complex * e, basis);
+    });
     if (total > ctx->plan_dev_bytes) {
         if (ctx->plan_dev) SC_HIP_CHECK(hipFree(ctx->plan_dev));
         ctx->plan_dev = nullptr;

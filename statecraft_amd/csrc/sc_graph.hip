@@ -64,29 +64,27 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 void put(std::string &k, const void *p, size_t bytes) { k.append((const char *)p, bytes); }
 
-// Builds every element's matrix pool of job j into the pinned staging.
-void build_pools(const sc_job &job, const GraphJob &gj, char *host) {
-    double *dst = (double *)(host + gj.pool_off);
-    const uint64_t pc = gj.plan.pool_complex;
-    const uint64_t B = gj.batch;
-    const unsigned threads = (unsigned)std::min<uint64_t>(
-        {B, B * (gj.plan.pool.size() + 1) >= 4096 ? (uint64_t)std::max(1u, std::thread::hardware_concurrency()) : 1,
-         16});
-    auto range = [&](uint64_t e0, uint64_t e1) {
-        for (uint64_t e = e0; e < e1; e++)
-            build_pool(gj.plan, job.mats + 2 * job.mat_stride * e, dst + 2 * pc * e, (long long)job.basis);
-    };
-    if (threads <= 1) {
-        range(0, B);
-        return;
+// Builds every element's matrix pool of every job into the pinned staging, on host threads shared by all jobs.
+void build_pools(const sc_job *jobs, const std::vector<GraphJob> &gjs, char *host) {
+    std::vector<uint64_t> first(gjs.size() + 1, 0);  // global element index of each job's element 0
+    uint64_t work = 0;
+    for (size_t j = 0; j < gjs.size(); j++) {
+        first[j + 1] = first[j] + gjs[j].batch;
+        work += gjs[j].batch * (gjs[j].plan.pool.size() + 1);
     }
-    std::vector<std::thread> pool;
-    const uint64_t per = (B + threads - 1) / threads;
-    for (unsigned t = 0; t < threads; t++) {
-        const uint64_t e0 = t * per, e1 = std::min<uint64_t>(B, e0 + per);
-        if (e0 < e1) pool.emplace_back(range, e0, e1);
-    }
-    for (auto &th : pool) th.join();
+    const uint64_t total = first.back();
+    const uint64_t grain = work < 4096 ? total : std::max<uint64_t>(1, total / (4 * kHostThreads));
+    parallel_ranges(total, grain, [&](unsigned, uint64_t e0, uint64_t e1) {
+        size_t j = std::upper_bound(first.begin(), first.end(), e0) - first.begin() - 1;
+        for (uint64_t g = e0; g < e1; g++) {
+            while (g >= first[j + 1]) j++;
+            const sc_job &job = jobs[j];
+            const GraphJob &gj = gjs[j];
+            const uint64_t e = g - first[j];
+            double *dst = (double *)(host + gj.pool_off) + 2 * gj.plan.pool_complex * e;
+            build_pool(gj.plan, job.mats + 2 * job.mat_stride * e, dst, (long long)job.basis);
+        }
+    });
 }
 
 // The enqueue half of one job (recorded into the graph): pool upload, passes, term kernels, row copy.
@@ -273,7 +271,7 @@ int run_jobs(Context *ctx, const sc_job *jobs, int n_jobs, int *used_graph) {
     }
     g->used = ++cache.tick;
     // replay: this call's matrices into the pools, one launch, one synchronisation
-    for (int j = 0; j < n_jobs; j++) build_pools(jobs[j], g->jobs[j], (char *)g->host);
+    build_pools(jobs, g->jobs, (char *)g->host);
     {
         ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_SWEEP, 0.0);
         SC_HIP_CHECK(hipGraphLaunch(g->exec, ctx->stream));

@@ -669,9 +669,9 @@ int compile_module(const std::string &src, std::vector<char> &code) {
 }
 
 // Compiles the given kernel bodies, split over up to `threads` modules compiled in parallel.  Kernel i of
-// the result is named sc_pass_<i>; modules[j] holds kernels [first[j], first[j+1]).
+// the result is named <prefix><i>; modules[j] holds kernels [first[j], first[j+1]).
 int compile_bodies(const std::vector<std::string> &bodies, int threads, std::vector<std::vector<char>> &codes,
-                   std::vector<size_t> &first) {
+                   std::vector<size_t> &first, const std::string &prefix = "sc_pass_") {
     const size_t nb = bodies.size();
     const size_t groups = std::max<size_t>(1, std::min<size_t>((size_t)std::max(threads, 1), nb));
     first.assign(groups + 1, 0);
@@ -679,7 +679,7 @@ int compile_bodies(const std::vector<std::string> &bodies, int threads, std::vec
     std::vector<std::string> srcs(groups);
     for (size_t g = 0; g < groups; g++) {
         std::string m = "#include \"sc_ops.hpp\"\n";
-        for (size_t i = first[g]; i < first[g + 1]; i++) m += replace_name(bodies[i], "sc_pass_" + std::to_string(i));
+        for (size_t i = first[g]; i < first[g + 1]; i++) m += replace_name(bodies[i], prefix + std::to_string(i));
         srcs[g] = std::move(m);
     }
     codes.assign(groups, {});
@@ -724,8 +724,10 @@ std::string jit_pass_source(const SweepPlan &plan, size_t pass, int variant) {
 
 namespace {
 
-// Kernels for the given bodies ("@K@" names the kernel), compiled on first use and cached per device.
-int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vector<hipFunction_t> &out) {
+// Kernels for the given bodies ("@K@" names the kernel), compiled on first use and cached per device.  The
+// kernels compiled by one call are named <prefix><i> (i: position among the bodies this call compiles).
+int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vector<hipFunction_t> &out,
+                const std::string &prefix = "sc_pass_") {
     JitCache &jc = cache();
     std::lock_guard<std::mutex> lock(jc.mu);
     const std::string dev = std::to_string(ctx->device) + "\n";
@@ -745,7 +747,7 @@ int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vecto
                                               std::to_string(kJitMaxKernels) + " kernels)");
         std::vector<std::vector<char>> codes;
         std::vector<size_t> first;
-        int rc = compile_bodies(missing, compile_threads(), codes, first);
+        int rc = compile_bodies(missing, compile_threads(), codes, first, prefix);
         if (rc) return rc;
         for (size_t g = 0; g + 1 < first.size(); g++) {
             hipModule_t mod = nullptr;
@@ -753,7 +755,7 @@ int jit_kernels(Context *ctx, const std::vector<std::string> &bodies, std::vecto
             jc.modules.push_back(mod);
             for (size_t i = first[g]; i < first[g + 1]; i++) {
                 hipFunction_t f = nullptr;
-                SC_HIP_CHECK(hipModuleGetFunction(&f, mod, ("sc_pass_" + std::to_string(i)).c_str()));
+                SC_HIP_CHECK(hipModuleGetFunction(&f, mod, (prefix + std::to_string(i)).c_str()));
                 jc.fns[dev + missing[i]] = f;
             }
         }
@@ -769,7 +771,7 @@ int jit_last_pass_terms_kernel(Context *ctx, const SweepPlan &plan, const std::v
                                hipFunction_t &out) {
     std::vector<std::string> bodies{pass_source(plan, plan.passes.back(), ctx->jit == 2, ctx->pass_sched, &zterms)};
     std::vector<hipFunction_t> fns;
-    int rc = jit_kernels(ctx, bodies, fns);
+    int rc = jit_kernels(ctx, bodies, fns, "sc_terms_pass_");
     if (rc) return rc;
     out = fns[0];
     return SC_OK;

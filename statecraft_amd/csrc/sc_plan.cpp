@@ -12,10 +12,16 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <tuple>
+
+#include <unistd.h>
 
 namespace sc {
 
@@ -61,6 +67,136 @@ bool ops_conflict(const PlanOp &a, const PlanOp &b) {
     return (a.targets & (b.targets | b.diag)) || (b.targets & a.diag);
 }
 
+namespace {
+
+// Matrix classes of one op's matrices in one batch element, as bits (AND over the elements gives the classes
+// that hold for all of them).
+constexpr uint32_t kClsReal = 1u;       // every entry has a zero imaginary part
+constexpr uint32_t kClsDiag = 2u;       // 2x2 diagonal
+constexpr uint32_t kClsX = 4u;          // 2x2 exactly Pauli X
+constexpr int kUnitOrder[4] = {2, 0, 3, 1};  // unit-pivot column pairs in order of preference
+uint32_t cls_unit_ok(int i) { return 8u << (3 * i); }     // pivot pair kUnitOrder[i] qualifies
+uint32_t cls_unit_real(int i) { return 16u << (3 * i); }  // ... with real off-pivot entries
+uint32_t cls_unit_imag(int i) { return 32u << (3 * i); }  // ... with purely imaginary off-pivot entries
+
+uint32_t classify_matrix(int kind, const double *g, int cnt) {
+    uint32_t c = 0;
+    bool real = true;
+    for (int e = 0; e < cnt; e++) real = real && g[2 * e + 1] == 0.0;
+    if (real) c |= kClsReal;
+    if (kind == SC_OP_SINGLE || kind == SC_OP_MCU) {
+        if (mat_is_diag(g)) c |= kClsDiag;
+        if (mat_is_x(g)) c |= kClsX;
+    }
+    if (kind == SC_OP_SINGLE) {
+        auto bounded = [](const double *ov, const double *pv) {
+            const double a = std::hypot(ov[0], ov[1]);
+            return std::isfinite(a) && a <= std::ldexp(std::fabs(pv[0]), 40);
+        };
+        for (int i = 0; i < 4; i++) {
+            const int cc = kUnitOrder[i], c0 = cc & 1, c1 = (cc >> 1) & 1;
+            const double *p0 = g + 2 * c0, *o0 = g + 2 * (1 - c0);
+            const double *p1 = g + 4 + 2 * c1, *o1 = g + 4 + 2 * (1 - c1);
+            if (!(p0[1] == 0.0 && p1[1] == 0.0 && p0[0] == p1[0] && p0[0] != 0.0 && std::isfinite(p0[0]) &&
+                  bounded(o0, p0) && bounded(o1, p1)))
+                continue;
+            c |= cls_unit_ok(i);
+            if (o0[1] == 0.0 && o1[1] == 0.0) c |= cls_unit_real(i);
+            if (o0[0] == 0.0 && o1[0] == 0.0) c |= cls_unit_imag(i);
+        }
+    }
+    return c;
+}
+
+}  // namespace
+
+namespace {
+
+// Worker threads kept for the life of the process (starting threads for every parallel region costs about as
+// much as the regions' work).  One region runs at a time; a caller that finds the pool busy runs its region
+// on its own thread.  A forked child starts a pool of its own.
+class HostPool {
+  public:
+    static HostPool *get() {
+        static std::mutex m;
+        static HostPool *pool = nullptr;
+        std::lock_guard<std::mutex> lock(m);
+        if (!pool || pool->pid_ != getpid()) pool = new HostPool();  // never destroyed: workers may still wait
+        return pool;
+    }
+    unsigned size() const { return (unsigned)workers_ + 1; }
+    // Runs task(slot) on the calling thread (slot 0) and on `extra` workers (slots 1..extra); false when another
+    // region holds the pool.
+    bool run(unsigned extra, const std::function<void(unsigned)> &task) {
+        std::unique_lock<std::mutex> region(region_mu_, std::try_to_lock);
+        if (!region.owns_lock()) return false;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            task_ = &task;
+            want_ = extra;
+            active_ = extra;
+            gen_++;
+        }
+        cv_.notify_all();
+        task(0);
+        std::unique_lock<std::mutex> lock(mu_);
+        done_.wait(lock, [&] { return active_ == 0; });
+        task_ = nullptr;
+        return true;
+    }
+
+  private:
+    HostPool() : pid_(getpid()) {
+        const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+        workers_ = (unsigned)std::min<uint64_t>(hc, kHostThreads) - 1;
+        for (unsigned i = 1; i <= workers_; i++) std::thread([this, i] { loop(i); }).detach();
+    }
+    void loop(unsigned slot) {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(unsigned)> *task;
+            {
+                std::unique_lock<std::mutex> lock(mu_);
+                cv_.wait(lock, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (slot > want_) continue;
+                task = task_;
+            }
+            (*task)(slot);
+            std::lock_guard<std::mutex> lock(mu_);
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    pid_t pid_;
+    unsigned workers_ = 0;
+    std::mutex region_mu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(unsigned)> *task_ = nullptr;
+    uint64_t gen_ = 0;
+    unsigned want_ = 0, active_ = 0;
+};
+
+}  // namespace
+
+void parallel_ranges(uint64_t n, uint64_t grain, const std::function<void(unsigned, uint64_t, uint64_t)> &f) {
+    grain = std::max<uint64_t>(grain, 1);
+    if (n <= grain) {
+        if (n) f(0, 0, n);
+        return;
+    }
+    HostPool *pool = HostPool::get();
+    const unsigned threads = (unsigned)std::min<uint64_t>((n + grain - 1) / grain, pool->size());
+    std::atomic<uint64_t> next{0};
+    const std::function<void(unsigned)> work = [&](unsigned t) {
+        for (;;) {
+            const uint64_t b = next.fetch_add(grain);
+            if (b >= n) return;
+            f(t, b, std::min(n, b + grain));
+        }
+    };
+    if (threads <= 1 || !pool->run(threads - 1, work)) work(0);
+}
+
 int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const double *> &mats_all,
                 uint64_t n_mat_complex, std::vector<PlanOp> &out) {
     out.clear();
@@ -70,6 +206,7 @@ int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const
         if (q >= n) return fail(SC_E_ARG, std::string(what) + " out of range: " + std::to_string(q));
         return SC_OK;
     };
+    // validation and qubit roles
     for (uint64_t i = 0; i < n_ops; i++) {
         const sc_op &o = ops[i];
         PlanOp p;
@@ -123,61 +260,10 @@ int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const
             p.controls = o.controls;
             break;
         }
-        int cnt = mat_count(o.kind);
-        p.is_real = true;
-        if (cnt > 0) {
-            if (o.mat < 0 || (uint64_t)o.mat + (uint64_t)cnt > n_mat_complex)
-                return fail(SC_E_ARG, "matrix offset out of range for op " + std::to_string(i));
-            for (const double *m : mats_all)
-                for (int e = 0; e < cnt; e++) p.is_real = p.is_real && m[2 * ((int64_t)o.mat + e) + 1] == 0.0;
-        }
-        // unit-pivot form of an uncontrolled 2x2 (pivot column pairs in order of preference)
-        if (o.kind == SC_OP_SINGLE && cnt > 0) {
-            for (int cc : {2, 0, 3, 1}) {
-                const int c0 = cc & 1, c1 = (cc >> 1) & 1;
-                bool ok = true, real = true, imag = true;
-                for (const double *m : mats_all) {
-                    const double *g = m + 2 * (int64_t)o.mat;
-                    const double *p0 = g + 2 * c0, *o0 = g + 2 * (1 - c0);
-                    const double *p1 = g + 4 + 2 * c1, *o1 = g + 4 + 2 * (1 - c1);
-                    auto bounded = [](const double *ov, const double *pv) {
-                        const double a = std::hypot(ov[0], ov[1]);
-                        return std::isfinite(a) && a <= std::ldexp(std::fabs(pv[0]), 40);
-                    };
-                    ok = p0[1] == 0.0 && p1[1] == 0.0 && p0[0] == p1[0] && p0[0] != 0.0 && std::isfinite(p0[0]) &&
-                         bounded(o0, p0) && bounded(o1, p1);
-                    if (!ok) break;
-                    real = real && o0[1] == 0.0 && o1[1] == 0.0;
-                    imag = imag && o0[0] == 0.0 && o1[0] == 0.0;
-                }
-                if (ok) {
-                    p.unit = cc;
-                    p.unit_form = real ? 1 : imag ? 2 : 0;
-                    break;
-                }
-            }
-        }
-        // roles
+        const int cnt = mat_count(o.kind);
+        if (cnt > 0 && (o.mat < 0 || (uint64_t)o.mat + (uint64_t)cnt > n_mat_complex))
+            return fail(SC_E_ARG, "matrix offset out of range for op " + std::to_string(i));
         switch (o.kind) {
-        case SC_OP_SINGLE:
-        case SC_OP_MCU: {
-            bool diag = true, x = true;
-            for (const double *m : mats_all) {
-                const double *g = m + 2 * (int64_t)o.mat;
-                diag = diag && mat_is_diag(g);
-                x = x && mat_is_x(g);
-            }
-            p.is_diag = diag;
-            p.is_x = x;
-            uint64_t tq = 1ULL << o.q0;
-            if (diag) {
-                p.diag = tq | p.controls;
-            } else {
-                p.targets = tq;
-                p.diag = p.controls;
-            }
-            break;
-        }
         case SC_OP_CNOT:
             p.is_x = true;
             p.targets = 1ULL << o.q1;
@@ -190,8 +276,50 @@ int analyze_ops(int n, const sc_op *ops, uint64_t n_ops, const std::vector<const
             p.is_diag = true;
             p.diag = (1ULL << o.q0) | (1ULL << o.q1);
             break;
+        default: break;  // single and multi-controlled 2x2: roles follow the matrix classes below
         }
         out.push_back(p);
+    }
+    // matrix classes that hold for every batch element; large batches classify on several host threads
+    const size_t nops = out.size(), nel = mats_all.size();
+    std::vector<uint32_t> cls(nops, ~0u);
+    std::vector<std::vector<uint32_t>> part;
+    const uint64_t grain = std::max<uint64_t>(1, 2048 / std::max<size_t>(nops, 1));
+    part.assign(std::min<uint64_t>(kHostThreads, (nel + grain - 1) / grain), std::vector<uint32_t>(nops, ~0u));
+    parallel_ranges(nel, grain, [&](unsigned t, uint64_t e0, uint64_t e1) {
+        std::vector<uint32_t> &c = part[t];
+        for (uint64_t e = e0; e < e1; e++)
+            for (size_t k = 0; k < nops; k++) {
+                const int cnt = mat_count(out[k].kind);
+                if (cnt > 0) c[k] &= classify_matrix(out[k].kind, mats_all[e] + 2 * (int64_t)out[k].mat, cnt);
+            }
+    });
+    for (const auto &c : part)
+        for (size_t k = 0; k < nops; k++) cls[k] &= c[k];
+    for (size_t k = 0; k < nops; k++) {
+        PlanOp &p = out[k];
+        const int cnt = mat_count(p.kind);
+        p.is_real = cnt == 0 || (cls[k] & kClsReal);
+        if (p.kind == SC_OP_SINGLE) {
+            for (int i = 0; i < 4; i++)
+                if (cls[k] & cls_unit_ok(i)) {
+                    // unit-pivot form of an uncontrolled 2x2
+                    p.unit = kUnitOrder[i];
+                    p.unit_form = (cls[k] & cls_unit_real(i)) ? 1 : (cls[k] & cls_unit_imag(i)) ? 2 : 0;
+                    break;
+                }
+        }
+        if (p.kind == SC_OP_SINGLE || p.kind == SC_OP_MCU) {
+            p.is_diag = (cls[k] & kClsDiag) != 0;
+            p.is_x = (cls[k] & kClsX) != 0;
+            const uint64_t tq = 1ULL << p.q0;
+            if (p.is_diag) {
+                p.diag = tq | p.controls;
+            } else {
+                p.targets = tq;
+                p.diag = p.controls;
+            }
+        }
     }
     return SC_OK;
 }

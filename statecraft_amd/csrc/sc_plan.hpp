@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -37,6 +38,13 @@ struct PlanOp {
 // Adjoint passes: a probe that accumulates <lambda| G |psi> for the Pauli generator G on q0 (and q1 for ZZ).
 // It leaves both states unchanged; its qubits must be register bits of its stage.
 constexpr int kPlanGrad = 100;
+
+// Host threads a planning step may use (pool builds, matrix classification of large batches).
+constexpr uint64_t kHostThreads = 16;
+// Calls f(thread, begin, end) over [0, n) in ranges of `grain` handed out from a shared counter, on up to
+// kHostThreads threads (the calling thread included; one thread when n <= grain).  `thread` < kHostThreads
+// names the calling thread's slot, for per-thread partial results.
+void parallel_ranges(uint64_t n, uint64_t grain, const std::function<void(unsigned, uint64_t, uint64_t)> &f);
 
 // Validate ops against n qubits (QuantumCircuit.Operation.validateTargets, QuantumCircuit.java:362-574) and
 // classify them.  `mats_all` lists every batch element's matrix array (one entry when shared) so diagonal /
