@@ -213,6 +213,39 @@ int sc_kraus_weights(sc_state *st, int elem, int target, const double *kraus, in
 /* applySingleQubitKraus + renormalizeState :426-439, :590-600: apply K (4 complex) on target, then rescale by
  * 1/sqrt(normSq) when |normSq-1| > 1e-12.  norm_sq_out (may be NULL) receives normSq. */
 int sc_apply_kraus(sc_state *st, int elem, int target, const double *kraus_op, double *norm_sq_out);
+/* One channel application of a noisy run: after op `after_op`, a channel of `n_kraus` single-qubit Kraus
+ * operators (4 complex each, row-major, from complex offset `kraus` of the kraus array) acts on `target`. */
+typedef struct sc_noise_event {
+    uint64_t after_op;
+    int32_t target;
+    int32_t n_kraus; /* 1..4 */
+    uint64_t kraus;
+} sc_noise_event;
+/* The first failing channel of a run (event -1: none).  kind 1: weight `branch` is negative or not finite;
+ * 2: the weights' total is <= 1e-15 (MIN_KRAUS_WEIGHT) or not finite; 3: the chosen branch's 1/sqrt(weight)
+ * overflows (the state would collapse). */
+typedef struct sc_noise_failure {
+    int64_t event;
+    int32_t elem;
+    int32_t kind;
+    int32_t branch;
+    int32_t pad;
+    double weights[4];
+    double norm_sq;
+} sc_noise_failure;
+/* A noisy circuit in one call: the ops in order, and after op events[e].after_op (events in circuit order) the
+ * channel of event e with the reference's unravelling (applyNoiseAfterOperation / applyErrorChannel
+ * StatevectorEngine.java:371-407, sampleKrausOperator :463-491, computeBranchWeight :511-530,
+ * sampleWeightedIndex :579-588, renormalizeState :590-600), the branch chosen on the device from the live
+ * weights: r = uniforms[b * n_events + e] * total (SplittableRandom.nextDouble(total) from the nextDouble()
+ * value the caller drew, one per event in circuit order), the chosen K_j applied divided by sqrt(w_j) (the
+ * norm the reference recomputes) when |w_j - 1| > 1e-12.  Every batch element runs its own branches.  Replaces
+ * the host loop of sc_kraus_weights / sc_apply_kraus calls per channel: nothing returns to the host until the
+ * circuit ends.  A failing channel (the reference's IllegalStateException) gives SC_E_STATE and fills
+ * `failure` (may be NULL); the state is then unspecified. */
+int sc_run_noisy(sc_state *st, const sc_op *ops, uint64_t n_ops, const double *mats, uint64_t n_mat_complex,
+                 const sc_noise_event *events, uint64_t n_events, const double *kraus, uint64_t n_kraus_complex,
+                 const double *uniforms, sc_noise_failure *failure);
 /* Batched trajectories: state `st` holds `batch` copies of one circuit's state; after every op i with
  * noise_after[i] != 0, a depolarising channel of probability p acts on each of the op's qubits
  * (q0, then q1) with the reference's unravelling (sampleKrausOperator :463-491 + sampleWeightedIndex

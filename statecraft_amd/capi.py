@@ -82,7 +82,8 @@ EXPORTED = [
     "sc_ctx_synchronize", "sc_ctx_device_info", "sc_state_create", "sc_state_destroy", "sc_state_qubits",
     "sc_state_set_basis", "sc_state_upload", "sc_state_download", "sc_state_copy", "sc_apply_ops",
     "sc_apply_ops_batched", "sc_probabilities", "sc_sample", "sc_norm_sq", "sc_probability_checksum",
-    "sc_kraus_weights", "sc_apply_kraus", "sc_run_depolarizing_trajectories", "sc_run_depolarizing_trajectories_z",
+    "sc_kraus_weights", "sc_apply_kraus", "sc_run_noisy", "sc_run_depolarizing_trajectories",
+    "sc_run_depolarizing_trajectories_z",
     "sc_expectation_pauli",
     "sc_expectation_pauli_terms",
     "sc_adjoint_gradient", "sc_adjoint_stats", "sc_profile_reset", "sc_profile_read", "sc_last_plan_stats",
@@ -134,6 +135,7 @@ def _sig(lib):
         "sc_run_depolarizing_trajectories": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64]),
         "sc_run_depolarizing_trajectories_z": (i32, [P, P, u64, DP, u64, U8P, ctypes.c_double, DP, u64, U64P, i32,
                                                      DP]),
+        "sc_run_noisy": (i32, [P, P, u64, DP, u64, P, u64, DP, u64, DP, P]),
         "sc_expectation_pauli": (i32, [P, U64P, U64P, DP, i32, DP]),
         "sc_expectation_pauli_terms": (i32, [P, U64P, U64P, i32, DP]),
         "sc_adjoint_gradient": (i32, [P, P, u64, DP, u64, i32, U64P, U64P, DP, i32, i32, i32, DP, DP]),
@@ -283,6 +285,33 @@ def jit_compile_host(n: int, ops, mats, low_qubits: int = 4, remap: bool = True,
                                     ctypes.byref(sec),
                                     ctypes.byref(size), dump_dir.encode() if dump_dir else None))
     return {"kernels": k.value, "seconds": sec.value, "code_bytes": size.value}
+
+
+class NoiseEvent(ctypes.Structure):
+    """sc_noise_event: after op `after_op`, `n_kraus` single-qubit Kraus operators (from complex offset `kraus`)
+    act on `target`."""
+    _fields_ = [("after_op", ctypes.c_uint64), ("target", ctypes.c_int32), ("n_kraus", ctypes.c_int32),
+                ("kraus", ctypes.c_uint64)]
+
+
+class NoiseFailure(ctypes.Structure):
+    """sc_noise_failure: the first failing channel of sc_run_noisy (event -1: none)."""
+    _fields_ = [("event", ctypes.c_int64), ("elem", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("branch", ctypes.c_int32), ("pad", ctypes.c_int32), ("weights", ctypes.c_double * 4),
+                ("norm_sq", ctypes.c_double)]
+
+
+class NoisyRunFailed(IllegalStateException):
+    """sc_run_noisy's SC_E_STATE, with the failing channel's record."""
+
+    def __init__(self, msg: str, failure: NoiseFailure):
+        super().__init__(msg)
+        self.event = int(failure.event)
+        self.elem = int(failure.elem)
+        self.kind = int(failure.kind)
+        self.branch = int(failure.branch)
+        self.weights = [float(x) for x in failure.weights]
+        self.norm_sq = float(failure.norm_sq)
 
 
 class _Job(ctypes.Structure):
@@ -562,6 +591,26 @@ class State:
                                         1 if per_element else 0, xm.ctypes.data_as(U64P), zm.ctypes.data_as(U64P),
                                         dptr(c), len(c), n_params, checkpoint_every, dptr(grads), dptr(energies)))
         return grads.reshape(self.batch, n_params), energies
+
+    def run_noisy(self, ops, mats, events, kraus, uniforms) -> None:
+        """sc_run_noisy: events are (after_op, target, first Kraus operator index, operator count) with the
+        operators [count][2][2] complex stacked in `kraus`; uniforms [batch, n_events] of nextDouble() values.
+        A failing channel raises NoisyRunFailed."""
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        m = complex_to_f64(mats if len(mats) else np.zeros(1, np.complex128))
+        k = complex_to_f64(np.concatenate([np.asarray(op, np.complex128).reshape(4) for op in kraus])
+                           if len(kraus) else np.zeros(4, np.complex128))
+        ev = (NoiseEvent * max(1, len(events)))()
+        for i, (after, target, first, count) in enumerate(events):
+            ev[i] = NoiseEvent(int(after), int(target), int(count), 4 * int(first))
+        u = np.ascontiguousarray(uniforms, np.float64).reshape(self.batch, len(events))
+        fail = NoiseFailure()
+        rc = lib().sc_run_noisy(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
+                                ctypes.cast(ev, P), len(events), dptr(k), k.size // 2,
+                                dptr(u) if u.size else dptr(np.zeros(1)), ctypes.byref(fail))
+        if rc == SC_E_STATE and fail.event >= 0:
+            raise NoisyRunFailed(lib().sc_last_error().decode("utf-8", "replace"), fail)
+        check(rc)
 
     def run_depolarizing_trajectories(self, ops, mats, noise_after, p: float, uniforms) -> None:
         ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)

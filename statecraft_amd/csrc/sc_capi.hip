@@ -120,6 +120,8 @@ int set_device(Context *ctx) {
     return SC_OK;
 }
 
+}  // namespace
+
 // ---- execution of an analysed op list ----------------------------------------
 int exec_direct(Context *ctx, State *st, const std::vector<PlanOp> &ops, const double *mats) {
     for (const PlanOp &op : ops) {
@@ -153,6 +155,8 @@ int exec_direct(Context *ctx, State *st, const std::vector<PlanOp> &ops, const d
     }
     return SC_OK;
 }
+
+namespace {
 
 int exec_fused(Context *ctx, State *st, const std::vector<PlanOp> &ops, const double *mats) {
     std::vector<FusedBlock> blocks;

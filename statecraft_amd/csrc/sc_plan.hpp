@@ -63,6 +63,9 @@ struct ZTermsRequest {
 // in the context's execution mode (sc_capi.hip).
 int apply_planned(State *st, const std::vector<PlanOp> &plan_ops, const std::vector<const double *> &elems,
                   bool per_elem, ZTermsRequest *terms = nullptr);
+// One kernel per op (SC_EXEC_DIRECT), matrices passed by value: enqueued on the context's stream, no
+// synchronisation.
+int exec_direct(Context *ctx, State *st, const std::vector<PlanOp> &ops, const double *mats);
 
 bool ops_conflict(const PlanOp &a, const PlanOp &b);
 

@@ -243,35 +243,54 @@ class HipStatevectorEngine:
 
     # StatevectorEngine.java:79-151 (noisy branch) and :371-407
     def _run_noisy(self, st, circuit, ops, mats, noise: NoiseModel, seed):
+        """The noisy circuit in one sc_run_noisy call.  Every scheduled channel and target becomes an event; its
+        uniform (the nextDouble() behind rng.nextDouble(total), one per event in circuit order) is drawn here and
+        the branch is chosen on the device from the live weights.  A validation error ends the event list where
+        the reference would raise it; the events before it still run, so a failing channel among them is
+        reported first, as in the reference's order."""
         rng = SplittableRandom(seed if seed is not None else int.from_bytes(os.urandom(8), "little"))
         n = circuit.qubit_count
+        events, kraus, where = [], [], []  # where[e]: (channel, target) for messages
         seen_measure = False
-        pending = 0  # ops [pending, i) are applied together, before the next op that schedules a channel
+        pending_error, end = None, len(ops)
         for i, op in enumerate(circuit.operations):
             if op.kind == capi.OP_MEASURE:
-                if i > pending:
-                    st.apply(ops[pending:i], mats)
-                pending = i + 1
                 seen_measure = True
                 continue
             if seen_measure:
-                raise UnsupportedOperationException(
+                pending_error = UnsupportedOperationException(
                     "Unitary operations cannot follow measurement operations in the circuit")
-            scheduled = noise.scheduled_channels_after(op)
-            if not scheduled:
-                continue
-            st.apply(ops[pending:i + 1], mats)
-            pending = i + 1
-            for sch in scheduled:
-                self._apply_channel(st, sch.channel, sch.target_qubits, rng, n)
-        if len(ops) > pending:
-            st.apply(ops[pending:], mats)
+                end = i
+                break
+            try:
+                for sch in noise.scheduled_channels_after(op):
+                    self._collect_channel(sch.channel, sch.target_qubits, n, i, events, kraus, where)
+            except (IllegalArgumentException, IllegalStateException) as e:
+                pending_error, end = e, i + 1
+                break
+        u = rng.next_doubles(len(events)) if events else np.zeros(0)
+        try:
+            st.run_noisy(ops[:end], mats, events, kraus, u)
+        except capi.NoisyRunFailed as f:
+            channel, t = where[f.event]
+            w = f.weights[:len(channel.kraus)]
+            if f.kind == 1:
+                raise IllegalStateException(f"invalid Kraus branch weight for channel {channel} on targets [{t}]: "
+                                            f"weights={w}, branch={w[f.branch]}") from None
+            if f.kind == 2:
+                raise IllegalStateException(f"invalid Kraus branch weights for channel {channel} on targets [{t}]: "
+                                            f"total={sum(w)}, weights={w}") from None
+            raise IllegalStateException(
+                f"state collapsed to invalid norm after Kraus application: normSq={f.norm_sq}") from None
+        if pending_error is not None:
+            raise pending_error
 
-    def _apply_channel(self, st, channel, mapped, rng, n):
+    def _collect_channel(self, channel, mapped, n, after, events, kraus, where):
+        # applyErrorChannel (StatevectorEngine.java:382-407) and resolveTargets (:409-424), composite recursion
         if isinstance(channel, CompositeChannel):
             comp_map = mapped if mapped is not None and len(mapped) == 1 else None
             for c in channel.channels:
-                self._apply_channel(st, c, comp_map, rng, n)
+                self._collect_channel(c, comp_map, n, after, events, kraus, where)
             return
         targets = tuple(mapped) if mapped is not None else channel.affected_qubits()
         if len(targets) == 0:
@@ -283,26 +302,11 @@ class HipStatevectorEngine:
             if t in seen:
                 raise IllegalArgumentException(f"noise channel contains duplicate qubit: {t}")
             seen.add(t)
+        first = len(kraus)
+        kraus.extend(np.asarray(k, np.complex128).reshape(2, 2) for k in channel.kraus)
         for t in targets:  # every built-in channel decomposes to single-qubit operators
-            w = st.kraus_weights(t, channel.kraus)
-            total = 0.0
-            for x in w:
-                if not np.isfinite(x) or x < 0.0:
-                    raise IllegalStateException(f"invalid Kraus branch weight for channel {channel} on targets "
-                                                f"[{t}]: weights={list(w)}, branch={x}")
-                total += float(x)
-            if not np.isfinite(total) or total <= MIN_KRAUS_WEIGHT:
-                raise IllegalStateException(f"invalid Kraus branch weights for channel {channel} on targets [{t}]: "
-                                            f"total={total}, weights={list(w)}")
-            r = rng.next_double_bounded(total)
-            idx = len(w) - 1
-            cum = 0.0
-            for j, x in enumerate(w):  # sampleWeightedIndex :579-588
-                cum += float(x)
-                if r < cum:
-                    idx = j
-                    break
-            st.apply_kraus(t, channel.kraus[idx])
+            events.append((after, t, first, len(channel.kraus)))
+            where.append((channel, t))
 
     # StatevectorEngine.java:229-272
     def _measure(self, st, n, instr: MeasurementInstruction, measure_ops):

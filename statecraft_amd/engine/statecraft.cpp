@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <random>
 #include <set>
 
@@ -535,75 +536,73 @@ SimulationResult HipStatevectorEngine::simulate(const SimulationRequest &req) {
     if (mats.empty()) mats.assign(2, 0.0);
     const auto &noise = req.noiseModel();
     if (noise && noise->hasNoise()) {
+        // Every scheduled channel and target becomes an sc_run_noisy event whose nextDouble() is drawn here in
+        // circuit order; the branch is chosen on the device from the live weights.  A validation error ends
+        // the event list where the reference raises it, after the events before it have run.
         SplittableRandom rng(req.noiseSeed() ? *req.noiseSeed() : (int64_t)random_seed());
-        bool measured = false;
-        size_t pending = 0;  // ops [pending, i) are applied together, before the next op that schedules a channel
-        auto flush = [&](size_t end) {
-            if (end > pending) check_status(sc_apply_ops(st.s, &ops[pending], end - pending, mats.data(), mats.size() / 2));
-            pending = end;
+        std::vector<sc_noise_event> events;
+        std::vector<double> kraus;
+        std::exception_ptr pending;
+        size_t end = ops.size();
+        struct Collect {
+            std::vector<sc_noise_event> &events;
+            std::vector<double> &kraus;
+            int n;
+            uint64_t after;
+            // applyErrorChannel (StatevectorEngine.java:382-407), composite recursion included
+            void operator()(const ErrorChannel &ch, const std::vector<int> *mapped) {
+                if (auto comps = ch.components()) {
+                    const std::vector<int> *m = (mapped && mapped->size() == 1) ? mapped : nullptr;
+                    for (auto &part : *comps) (*this)(*part, m);
+                    return;
+                }
+                std::vector<int> targets = mapped ? *mapped : ch.affectedQubits();
+                if (targets.empty()) iae("noise channel must target at least one qubit");
+                std::set<int> seen;
+                for (int t : targets) {
+                    if (t < 0 || t >= n) iae("noise channel qubit out of range: " + std::to_string(t));
+                    if (!seen.insert(t).second) iae("noise channel contains duplicate qubit: " + std::to_string(t));
+                }
+                const uint64_t first = kraus.size() / 2;
+                for (auto &op : ch.kraus())
+                    for (auto &v : op) kraus.push_back(v.real()), kraus.push_back(v.imag());
+                for (int t : targets) events.push_back({after, t, (int32_t)ch.kraus().size(), first});
+            }
         };
-        for (size_t i = 0; i < ops.size(); i++) {
+        bool measured = false;
+        for (size_t i = 0; i < ops.size() && !pending; i++) {
             if (ops[i].kind == SC_OP_MEASURE) {
-                flush(i);
-                pending = i + 1;
                 measured = true;
                 continue;
             }
-            if (measured)
-                throw UnsupportedOperationException("Unitary operations cannot follow measurement operations in the circuit");
-            const auto scheduled = noise->scheduledChannelsAfter(c.operations()[i]);
-            if (scheduled.empty()) continue;
-            flush(i + 1);
-            for (auto &sch : scheduled) {
-                // applyErrorChannel (StatevectorEngine.java:382-407), composite recursion included
-                struct Apply {
-                    sc_state *s;
-                    SplittableRandom &rng;
-                    int n;
-                    void operator()(const ErrorChannel &ch, const std::vector<int> *mapped) {
-                        if (auto comps = ch.components()) {
-                            const std::vector<int> *m = (mapped && mapped->size() == 1) ? mapped : nullptr;
-                            for (auto &part : *comps) (*this)(*part, m);
-                            return;
-                        }
-                        std::vector<int> targets = mapped ? *mapped : ch.affectedQubits();
-                        if (targets.empty()) iae("noise channel must target at least one qubit");
-                        std::set<int> seen;
-                        for (int t : targets) {
-                            if (t < 0 || t >= n) iae("noise channel qubit out of range: " + std::to_string(t));
-                            if (!seen.insert(t).second) iae("noise channel contains duplicate qubit: " + std::to_string(t));
-                        }
-                        const auto &k = ch.kraus();
-                        std::vector<double> kr;
-                        for (auto &op : k)
-                            for (auto &v : op) kr.push_back(v.real()), kr.push_back(v.imag());
-                        for (int t : targets) {
-                            std::vector<double> w(k.size());
-                            check_status(sc_kraus_weights(s, 0, t, kr.data(), (int)k.size(), w.data()));
-                            double total = 0.0;
-                            for (double x : w) {
-                                if (!std::isfinite(x) || x < 0.0) throw IllegalStateException("invalid Kraus branch weight");
-                                total += x;
-                            }
-                            if (!std::isfinite(total) || total <= 1e-15)
-                                throw IllegalStateException("invalid Kraus branch weights");
-                            double r = rng.nextDouble(total), cum = 0.0;
-                            size_t idx = w.size() - 1;
-                            for (size_t j = 0; j < w.size(); j++) {
-                                cum += w[j];
-                                if (r < cum) {
-                                    idx = j;
-                                    break;
-                                }
-                            }
-                            check_status(sc_apply_kraus(s, 0, t, kr.data() + 8 * idx, nullptr));
-                        }
-                    }
-                } apply{st.s, rng, n};
-                apply(*sch.channel, &sch.targetQubits);
+            if (measured) {
+                pending = std::make_exception_ptr(
+                    UnsupportedOperationException("Unitary operations cannot follow measurement operations in the circuit"));
+                end = i;
+                break;
+            }
+            try {
+                Collect collect{events, kraus, n, (uint64_t)i};
+                for (auto &sch : noise->scheduledChannelsAfter(c.operations()[i])) collect(*sch.channel, &sch.targetQubits);
+            } catch (const IllegalArgumentException &) {
+                pending = std::current_exception();
+                end = i + 1;
             }
         }
-        flush(ops.size());
+        std::vector<double> u(events.size());
+        for (auto &x : u) x = rng.nextDouble();
+        if (kraus.empty()) kraus.assign(8, 0.0);
+        if (u.empty()) u.assign(1, 0.0);
+        sc_noise_failure failure;
+        const int rc = sc_run_noisy(st.s, ops.data(), end, mats.data(), mats.size() / 2, events.data(), events.size(),
+                                    kraus.data(), kraus.size() / 2, u.data(), &failure);
+        if (rc == SC_E_STATE && failure.event >= 0) {
+            if (failure.kind == 1) throw IllegalStateException("invalid Kraus branch weight");
+            if (failure.kind == 2) throw IllegalStateException("invalid Kraus branch weights");
+            throw IllegalStateException("state collapsed to invalid norm after Kraus application");
+        }
+        check_status(rc);
+        if (pending) std::rethrow_exception(pending);
     } else {
         check_status(sc_apply_ops(st.s, ops.data(), ops.size(), mats.data(), mats.size() / 2));
     }

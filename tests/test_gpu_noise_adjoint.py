@@ -77,6 +77,105 @@ def test_engine_measurement_counts_bit_identical():
     assert res.measurement.counts == {int(v): int(c) for v, c in zip(vals, cnts)}
 
 
+def _noise_schedule(n, n_ops, rng, every):
+    """Channels after every `every`-th op (one or two each, on random qubits): sc_run_noisy events, the stacked
+    Kraus operators, and (op, target, operators) per event for the oracle."""
+    chans = [ErrorChannel.amplitude_damping(0.3, 0).kraus, ErrorChannel.phase_damping(0.4, 0).kraus,
+             ErrorChannel.depolarizing(0.2, 0).kraus, ErrorChannel.thermal_relaxation(50.0, 60.0, 10.0, 0).kraus,
+             ErrorChannel.phase_flip(0.25, 0).kraus]
+    events, kraus, per = [], [], []
+    for i in range(0, n_ops, every):
+        for _ in range(int(rng.integers(1, 3))):
+            k = chans[int(rng.integers(0, len(chans)))]
+            t = int(rng.integers(0, n))
+            events.append((i, t, len(kraus), len(k)))
+            per.append((i, t, k))
+            kraus.extend(k)
+    return events, kraus, per
+
+
+def _oracle_schedule(n, ops, mats, per, seed, init):
+    s = np.array(init, np.complex128, copy=True)
+    holder, j = [seed], 0
+    for i in range(len(ops)):
+        s = oracle.run_ops(n, ops[i:i + 1], mats, s)
+        while j < len(per) and per[j][0] == i:
+            oracle.apply_channel_1q(s, per[j][1], per[j][2], holder)
+            j += 1
+    return s
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("n,B,every", [(6, 1, 1), (13, 1, 1), (13, 1, 7), (16, 3, 5), (12, 4, 12)])
+def test_run_noisy_matches_oracle(n, B, every, jit):
+    """sc_run_noisy: state-dependent channels (amplitude and phase damping, depolarising, thermal relaxation,
+    phase flip) with the branch chosen on the device, after single ops (per-op kernels between channels) and
+    after longer segments (the context's execution mode), every batch element with its own uniforms, against
+    the oracle's applyErrorChannel with the same SplittableRandom streams (1e-12)."""
+    ctx = capi.Context(0)
+    ctx.set_option(capi.OPT_JIT, jit)
+    try:
+        rng = np.random.default_rng(9000 + 100 * n + 10 * B + every)
+        ops, mats = random_circuit(n, 36, rng, max_controls=2)
+        events, kraus, per = _noise_schedule(n, len(ops), rng, every)
+        seeds = [4000 + 17 * b for b in range(B)]
+        u = np.stack([SplittableRandom(sd).next_doubles(len(events)) for sd in seeds])
+        init = np.stack([random_state(n, rng) for _ in range(B)])
+        st = capi.State(ctx, n, B)
+        st.upload(init.reshape(-1))
+        st.run_noisy(ops, mats, events, kraus, u)
+        got = st.download().reshape(B, -1)
+        st.close()
+        for b in range(B):
+            want = _oracle_schedule(n, ops, mats, per, seeds[b], init[b])
+            assert np.max(np.abs(got[b] - want)) <= EPS, (b, np.max(np.abs(got[b] - want)))
+    finally:
+        ctx.close()
+
+
+def test_run_noisy_reports_the_first_failing_channel(hip_ctx):
+    """A channel whose weights are all zero (total <= 1e-15) or not a number fails the run with the event, its
+    weights and the kind (the reference's IllegalStateException); later calls on the context still work."""
+    n = 12
+    circuit = QuantumCircuit(n)
+    for q in range(n):
+        circuit = circuit.append(Hadamard(), q)
+    ops, mats = lower(circuit)
+    damp = ErrorChannel.amplitude_damping(0.3, 0).kraus
+    zero = [np.zeros((2, 2)), np.zeros((2, 2))]
+    nan = [np.eye(2), np.array([[np.nan, 0], [0, 0]])]
+    kraus = list(damp) + zero + nan
+    events = [(3, 1, 0, 2), (5, 2, 2, 2), (6, 0, 4, 2)]
+    st = capi.State(hip_ctx, n, 1)
+    st.set_basis(0)
+    with pytest.raises(capi.NoisyRunFailed) as e:
+        st.run_noisy(ops, mats, events, kraus, np.full((1, 3), 0.5))
+    assert (e.value.event, e.value.kind, e.value.weights[:2]) == (1, 2, [0.0, 0.0])
+    assert "total=0" in str(e.value)
+    st.set_basis(0)
+    with pytest.raises(capi.NoisyRunFailed) as e:
+        st.run_noisy(ops, mats, events[2:], kraus, np.full((1, 1), 0.5))
+    assert (e.value.event, e.value.kind, e.value.branch) == (0, 1, 1)
+    assert abs(e.value.weights[0] - 1.0) <= 1e-12 and np.isnan(e.value.weights[1])
+    # argument checks (before any kernel)
+    with pytest.raises(capi.IllegalArgumentException):
+        st.run_noisy(ops, mats, [(5, 0, 0, 2), (3, 0, 0, 2)], kraus, np.full((1, 2), 0.5))
+    with pytest.raises(capi.IllegalArgumentException):
+        st.run_noisy(ops, mats, [(3, n, 0, 2)], kraus, np.full((1, 1), 0.5))
+    with pytest.raises(capi.IllegalArgumentException):
+        st.run_noisy(ops, mats, [(3, 0, 5, 2)], kraus, np.full((1, 1), 0.5))
+    with pytest.raises(capi.IllegalArgumentException):
+        st.run_noisy(ops, mats, [(3, 0, 0, 2)], kraus, np.full((1, 1), 1.0))
+    # the context still runs: H on every qubit, damping on qubit 1 with weights (0.85, 0.15) and r = 0.75 picks
+    # K_0 = diag(1, sqrt(0.7)); the state is K_0 |+..+> / sqrt(0.85)
+    st.set_basis(0)
+    st.run_noisy(ops, mats, events[:1], kraus, np.full((1, 1), 0.75))
+    bit1 = (np.arange(1 << n) >> 1) & 1
+    want = np.where(bit1 == 1, np.sqrt(0.7), 1.0) * 2.0 ** (-n / 2) / np.sqrt(0.85)
+    assert np.max(np.abs(st.download() - want)) <= EPS
+    st.close()
+
+
 @pytest.mark.parametrize("n,B,p", [(12, 8, 0.1), (14, 5, 0.3)])
 def test_batched_trajectories(hip_ctx, n, B, p):
     circuit = workloads.ghz(n)
