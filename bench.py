@@ -61,6 +61,8 @@ def make_context(args) -> capi.Context:
         ctx.set_option(capi.OPT_CHUNK_BITS, args.chunk_bits)
     if args.mfma is not None:
         ctx.set_option(capi.OPT_MFMA, args.mfma)
+    if args.mfma_tile is not None:
+        ctx.set_option(capi.OPT_MFMA_TILE, args.mfma_tile)
     if args.max_fused is not None:
         ctx.set_option(capi.OPT_MAX_FUSED, args.max_fused)
     if args.pass_sched is not None:
@@ -77,7 +79,9 @@ def context_options(ctx) -> dict:
             "low_qubits_option": ctx.get_option(capi.OPT_LOW_QUBITS),
             "chunk_bits_option": ctx.get_option(capi.OPT_CHUNK_BITS),
             "pass_sched": ctx.get_option(capi.OPT_PASS_SCHED), "mfma": ctx.get_option(capi.OPT_MFMA),
-            "permlane": ctx.get_option(capi.OPT_PERMLANE), "fused_terms": ctx.get_option(capi.OPT_FUSED_TERMS)}
+            "permlane": ctx.get_option(capi.OPT_PERMLANE), "fused_terms": ctx.get_option(capi.OPT_FUSED_TERMS),
+            "mfma_tile": ctx.get_option(capi.OPT_MFMA_TILE),
+            "mfma_tile_autotuned": [ctx.get_option(capi.OPT_MFMA_PICK4), ctx.get_option(capi.OPT_MFMA_PICK5)]}
 
 
 def host_threads() -> int:
@@ -608,6 +612,9 @@ def main(argv=None) -> int:
     ap.add_argument("--mfma", type=int, default=None, choices=[0, 1, 2],
                     help="SC_OPT_MFMA, fused mode: 0 VALU blocks only, 1 5-qubit blocks on the FP64 matrix cores "
                          "(library default), 2 also 4-qubit blocks")
+    ap.add_argument("--mfma-tile", type=int, default=None, choices=[-1, 0, 1, 2, 3, 4, 5],
+                    help="SC_OPT_MFMA_TILE, fused mode: the matrix-core blocks' tile shape (-1: autotuned, the "
+                         "library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="c2: skip the adjoint-gradient block of the metric")
     ap.add_argument("--no-other-configs", action="store_true",

@@ -134,6 +134,12 @@ typedef struct sc_op {
 #define SC_OPT_FUSED_TERMS 15   /* 1: sc_run_jobs sums a job's Z-string terms in its last sweep pass as the states are
                                    stored (one read of the batch fewer); results agree with the separate calls to
                                    rounding instead of bit for bit; 0: a term kernel after the circuit; default 1 */
+#define SC_OPT_MFMA_TILE 16     /* tile shape of the matrix-core dense blocks: -1 (default) chosen by the library's
+                                   autotuner, which times every shape on a scratch state the first time a context
+                                   runs a block size; 0..5 forces one: 16-group tiles per wavefront step 1 (0-2) or 2
+                                   (3-5), resident workgroups per CU 16, 8, 32 (0, 1, 2) and (3, 4, 5) */
+#define SC_OPT_MFMA_PICK4 17    /* read-only: the tile shape the autotuner chose for 4-qubit blocks (-1: not tuned) */
+#define SC_OPT_MFMA_PICK5 18    /* read-only: the tile shape the autotuner chose for 5-qubit blocks (-1: not tuned) */
 #define SC_EXEC_DIRECT 0 /* one kernel launch per reference op (StatevectorEngine.java:90-151 order) */
 #define SC_EXEC_FUSED 1  /* gate fusion into dense 2^k blocks, one HBM sweep per block */
 #define SC_EXEC_SWEEP 2  /* fusion + LDS-blocked multi-gate sweeps (default) */

@@ -29,6 +29,9 @@ OPT_MFMA = 12
 OPT_PASS_SCHED = 13
 OPT_PERMLANE = 14
 OPT_FUSED_TERMS = 15
+OPT_MFMA_TILE = 16
+OPT_MFMA_PICK4 = 17
+OPT_MFMA_PICK5 = 18
 EXEC_DIRECT, EXEC_FUSED, EXEC_SWEEP = range(3)
 KFAM_GATE, KFAM_FUSED, KFAM_SWEEP, KFAM_MEASURE, KFAM_NOISE, KFAM_ADJOINT, KFAM_COPY = range(7)
 

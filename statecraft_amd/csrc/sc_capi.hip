@@ -580,6 +580,12 @@ int sc_ctx_set_option(sc_ctx *ctx, int option, int64_t value) {
     case SC_OPT_FUSED_TERMS:
         ctx->fused_terms = value != 0;
         return SC_OK;
+    case SC_OPT_MFMA_TILE:
+        if (value < -1 || value >= kMfmaTiles) return fail(SC_E_ARG, "SC_OPT_MFMA_TILE must be -1..5");
+        ctx->mfma_tile = (int)value;
+        return SC_OK;
+    case SC_OPT_MFMA_PICK4:
+    case SC_OPT_MFMA_PICK5: return fail(SC_E_ARG, "the autotuner's tile choice is read-only");
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }
@@ -603,6 +609,9 @@ int sc_ctx_get_option(sc_ctx *ctx, int option, int64_t *value) {
     case SC_OPT_PASS_SCHED: *value = ctx->pass_sched; return SC_OK;
     case SC_OPT_PERMLANE: *value = ctx->permlane; return SC_OK;
     case SC_OPT_FUSED_TERMS: *value = ctx->fused_terms; return SC_OK;
+    case SC_OPT_MFMA_TILE: *value = ctx->mfma_tile; return SC_OK;
+    case SC_OPT_MFMA_PICK4: *value = ctx->mfma_pick[0]; return SC_OK;
+    case SC_OPT_MFMA_PICK5: *value = ctx->mfma_pick[1]; return SC_OK;
     default: return fail(SC_E_ARG, "unknown option " + std::to_string(option));
     }
 }

@@ -187,12 +187,13 @@ __global__ void __launch_bounds__(kBlock) k_dense(double2 *__restrict__ amp, int
 // that own it, after all of the wavefront's loads, so the block is applied in place.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <int K>
+template <int K, int T>
 __global__ void __launch_bounds__(kBlock) k_dense_mfma(double2 *__restrict__ amp, int n, uint64_t fixed,
                                                        uint64_t qoff0, uint64_t qoff1, uint64_t qoff2, uint64_t qoff3,
                                                        uint64_t qoff4, int log_groups, uint64_t total,
                                                        const double2 *__restrict__ mat) {
     static_assert(K == 4 || K == 5, "MFMA dense blocks span 4 or 5 qubits");
+    static_assert(T == 1 || T == 2, "one or two 16-group tiles per wavefront step");
     constexpr int D = 1 << K;
     constexpr int KS = D / 4;   // k-steps of 4
     constexpr int MT = D / 16;  // row tiles of 16
@@ -222,33 +223,39 @@ __global__ void __launch_bounds__(kBlock) k_dense_mfma(double2 *__restrict__ amp
 #pragma unroll
         for (int v = 0; v < 4; v++) moff[mt][v] = block_off(16u * mt + 4u * v + kq);
     const uint64_t gmask = (1ULL << log_groups) - 1ULL;
-    const uint64_t tiles = total >> 4;
+    const uint64_t steps = (total >> 4) / T;  // the launcher passes a tile count divisible by T
     const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
-    for (uint64_t t = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < tiles; t += waves) {
-        const uint64_t g = (t << 4) | col;
-        const uint64_t b = expand_mask(g & gmask, fixed);
-        double2 *base = amp + ((g >> log_groups) << n) + b;
-        double xr[KS], xi[KS], nxi[KS];
+    for (uint64_t w = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < steps; w += waves) {
+        // every tile's loads first (T independent 16-group reads in flight), then its products and stores
+        double2 *base[T];
+        double xr[T][KS], xi[T][KS], nxi[T][KS];
 #pragma unroll
-        for (int s = 0; s < KS; s++) {
-            const double2 x = base[koff[s]];
-            xr[s] = x.x;
-            xi[s] = x.y;
-            nxi[s] = -x.y;
-        }
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++) {
-            v4d yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+        for (int tt = 0; tt < T; tt++) {
+            const uint64_t g = ((w * T + tt) << 4) | col;
+            base[tt] = amp + ((g >> log_groups) << n) + expand_mask(g & gmask, fixed);
 #pragma unroll
             for (int s = 0; s < KS; s++) {
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xr[s], yr, 0, 0, 0);
-                yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], nxi[s], yr, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xi[s], yi, 0, 0, 0);
-                yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], xr[s], yi, 0, 0, 0);
+                const double2 x = base[tt][koff[s]];
+                xr[tt][s] = x.x;
+                xi[tt][s] = x.y;
+                nxi[tt][s] = -x.y;
             }
-#pragma unroll
-            for (int v = 0; v < 4; v++) base[moff[mt][v]] = make_double2(yr[v], yi[v]);
         }
+#pragma unroll
+        for (int tt = 0; tt < T; tt++)
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                v4d yr = {0.0, 0.0, 0.0, 0.0}, yi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < KS; s++) {
+                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xr[tt][s], yr, 0, 0, 0);
+                    yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], nxi[tt][s], yr, 0, 0, 0);
+                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ur[mt][s], xi[tt][s], yi, 0, 0, 0);
+                    yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ui[mt][s], xr[tt][s], yi, 0, 0, 0);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) base[tt][moff[mt][v]] = make_double2(yr[v], yi[v]);
+            }
     }
 }
 
@@ -339,6 +346,95 @@ int launch_copy(Context *ctx, const double2 *src, double2 *dst, uint64_t count) 
     return SC_OK;
 }
 
+namespace {
+
+constexpr int kMfmaTileSteps[kMfmaTiles] = {1, 1, 1, 2, 2, 2};  // 16-group tiles per wavefront step
+constexpr int kMfmaPerCu[kMfmaTiles] = {16, 8, 32, 16, 8, 32};   // resident workgroups per CU (grid cap)
+
+template <int K>
+void launch_mfma_variant(Context *ctx, double2 *amp, int n, int steps_per_wave, uint64_t grid, uint64_t fixed,
+                         const uint64_t *qo, int log_groups, uint64_t total, const double2 *m) {
+    if (steps_per_wave == 2)
+        hipLaunchKernelGGL((k_dense_mfma<K, 2>), dim3((unsigned)grid), dim3(kBlock), 0, ctx->stream, amp, n, fixed,
+                           qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
+    else
+        hipLaunchKernelGGL((k_dense_mfma<K, 1>), dim3((unsigned)grid), dim3(kBlock), 0, ctx->stream, amp, n, fixed,
+                           qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
+}
+
+// One k-qubit block (k = 4, 5) on the matrix cores with tile shape `shape`.  A two-tile shape needs an even
+// tile count; otherwise its one-tile counterpart runs.
+int launch_dense_mfma(Context *ctx, double2 *amp, int n, int k, int shape, uint64_t fixed, const uint64_t *qo,
+                      int log_groups, uint64_t total, const double2 *m) {
+    const uint64_t tiles = total >> 4;
+    const int T = (kMfmaTileSteps[shape] == 2 && tiles % 2 == 0) ? 2 : 1;
+    const uint64_t steps = tiles / T;
+    uint64_t blocks = (steps + kBlock / 64 - 1) / (kBlock / 64);
+    const uint64_t cap = (uint64_t)ctx->cu_count * kMfmaPerCu[shape];
+    if (blocks > cap) blocks = cap;
+    if (k == 4)
+        launch_mfma_variant<4>(ctx, amp, n, T, blocks, fixed, qo, log_groups, total, m);
+    else
+        launch_mfma_variant<5>(ctx, amp, n, T, blocks, fixed, qo, log_groups, total, m);
+    SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
+}
+
+// The tile-shape autotuner: every shape applies a k-qubit block (qubits 1, 6, 11, 16[, 21] -- block qubits
+// below and above the 4 coalesced positions) to a scratch state of 2^26 amplitudes (2^24 when that does not
+// fit), one warm-up launch and three timed ones each; the fastest shape is kept for the context, where a shape
+// replaces the earlier choice only when at least 3% faster (shape 0 is the untuned kernel).  Runs once per
+// context and block size, before the first matrix-core block.
+int tune_dense_mfma(Context *ctx, int k) {
+    SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    DevBuf buf;
+    int n = 26;
+    const size_t mat_bytes = (size_t)16 << (2 * k);
+    if (buf.alloc(((size_t)16 << n) + mat_bytes)) {
+        n = 24;
+        if (buf.alloc(((size_t)16 << n) + mat_bytes)) {
+            ctx->mfma_pick[k - 4] = 0;  // no room for a scratch state: the untuned shape
+            return SC_OK;
+        }
+    }
+    double2 *amp = (double2 *)buf.p;
+    const double2 *m = (const double2 *)((char *)buf.p + ((size_t)16 << n));
+    SC_HIP_CHECK(hipMemsetAsync(buf.p, 0, ((size_t)16 << n) + mat_bytes, ctx->stream));
+    const int qubits[5] = {1, 6, 11, 16, 21};
+    uint64_t qo[5] = {0, 0, 0, 0, 0}, fixed = 0;
+    for (int j = 0; j < k; j++) {
+        qo[j] = 1ULL << qubits[j];
+        fixed |= qo[j];
+    }
+    const int log_groups = n - k;
+    const uint64_t total = 1ULL << log_groups;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SC_HIP_CHECK(hipEventCreate(&e0));
+    SC_HIP_CHECK(hipEventCreate(&e1));
+    int best = 0, rc = SC_OK;
+    float best_ms = 0.0f;
+    for (int shape = 0; shape < kMfmaTiles && rc == SC_OK; shape++) {
+        rc = launch_dense_mfma(ctx, amp, n, k, shape, fixed, qo, log_groups, total, m);
+        if (rc == SC_OK && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = fail(SC_E_HIP, "hipEventRecord failed");
+        for (int r = 0; r < 3 && rc == SC_OK; r++) rc = launch_dense_mfma(ctx, amp, n, k, shape, fixed, qo, log_groups, total, m);
+        if (rc == SC_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = fail(SC_E_HIP, "hipEventRecord failed");
+        float ms = 0.0f;
+        if (rc == SC_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(SC_E_HIP, "timing the dense-block tile shapes failed");
+        if (rc == SC_OK && (shape == 0 || ms < 0.97f * best_ms)) {
+            best = shape;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->mfma_pick[k - 4] = best;
+    return SC_OK;
+}
+
+}  // namespace
+
 int launch_dense(Context *ctx, State *st, int k, const int *qubits, const double *mat_dev) {
     uint64_t qo[5] = {0, 0, 0, 0, 0};
     uint64_t fixed = 0;
@@ -350,24 +446,20 @@ int launch_dense(Context *ctx, State *st, int k, const int *qubits, const double
     uint64_t total = (uint64_t)st->batch << log_groups;
     unsigned grid = grid_for(ctx, total);
     const double2 *m = reinterpret_cast<const double2 *>(mat_dev);
-    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_FUSED, 32.0 * (double)((uint64_t)st->batch << st->n));
     // matrix-core variant (blocks with whole 16-group tiles, n - k >= 4): 5-qubit blocks by default, where the
     // VALU kernel is arithmetic-bound (17.6 ms vs 7.0 ms per 30-qubit sweep on MI355X); 4-qubit blocks with
     // SC_OPT_MFMA 2 (the VALU kernel is at HBM speed there: 6.8 ms vs 7.2 ms)
-    if (((ctx->mfma >= 1 && k == 5) || (ctx->mfma >= 2 && k == 4)) && log_groups >= 4) {
-        const uint64_t tiles = total >> 4;
-        uint64_t blocks = (tiles + kBlock / 64 - 1) / (kBlock / 64);
-        const uint64_t cap = (uint64_t)ctx->cu_count * 16;
-        if (blocks > cap) blocks = cap;
-        if (k == 4)
-            hipLaunchKernelGGL(k_dense_mfma<4>, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
-                               fixed, qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
-        else
-            hipLaunchKernelGGL(k_dense_mfma<5>, dim3((unsigned)blocks), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
-                               fixed, qo[0], qo[1], qo[2], qo[3], qo[4], log_groups, total, m);
-        SC_HIP_CHECK(hipGetLastError());
-        return SC_OK;
+    const bool mfma = ((ctx->mfma >= 1 && k == 5) || (ctx->mfma >= 2 && k == 4)) && log_groups >= 4;
+    int shape = ctx->mfma_tile;
+    if (mfma && shape < 0) {
+        if (ctx->mfma_pick[k - 4] < 0) {
+            int rc = tune_dense_mfma(ctx, k);  // outside the profiled region
+            if (rc) return rc;
+        }
+        shape = ctx->mfma_pick[k - 4];
     }
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_FUSED, 32.0 * (double)((uint64_t)st->batch << st->n));
+    if (mfma) return launch_dense_mfma(ctx, st->amp, st->n, k, shape, fixed, qo, log_groups, total, m);
     switch (k) {
     case 1:
         hipLaunchKernelGGL(k_dense<1>, dim3(grid), dim3(kBlock), 0, ctx->stream, st->amp, st->n, fixed, qo[0], qo[1],

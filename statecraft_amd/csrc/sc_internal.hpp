@@ -69,6 +69,8 @@ struct Context {
     int chunk_bits = 0;   // SC_OPT_CHUNK_BITS; 0: chosen per circuit structure
     int direct_io = 1;    // SC_OPT_DIRECT_IO
     int mfma = 1;         // SC_OPT_MFMA
+    int mfma_tile = -1;   // SC_OPT_MFMA_TILE; -1: autotuned
+    int mfma_pick[2] = {-1, -1};  // the autotuner's tile shape for 4- and 5-qubit blocks (-1: not tuned yet)
     int permlane = 0;     // SC_OPT_PERMLANE
     int fused_terms = 1;  // SC_OPT_FUSED_TERMS
     int pass_sched = 4;   // SC_OPT_PASS_SCHED
@@ -118,6 +120,10 @@ int drain_profiler(Context *ctx);
 
 // ---------------------------------------------------------------- device-side op encodings
 // Dense per-op gates (DIRECT mode) take their matrices by value.
+// Tile shapes of the matrix-core dense-block kernel (SC_OPT_MFMA_TILE): 16-group tiles per wavefront step 1
+// (shapes 0-2) or 2 (3-5), resident workgroups per CU 16, 8, 32.
+constexpr int kMfmaTiles = 6;
+
 struct Mat2 {
     double v[8];  // row-major 2x2 complex (re, im)
 };

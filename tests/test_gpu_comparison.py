@@ -12,6 +12,7 @@ import oracle
 from statecraft_amd import comparison
 from statecraft_amd.circuit import lower
 from test_comparison_host import BY_ID, EXPECTED, circuit_text
+from test_gpu_noise_adjoint import oracle_noisy
 
 pytestmark = pytest.mark.gpu
 
@@ -118,6 +119,16 @@ def test_noisy_fixtures_match_density_matrix(engine, tmp_path, fid):
             rho = sum(_on(n, q, k) @ rho @ _on(n, q, k).conj().T for k in ks)
     exact = np.real(np.diag(rho))
     assert np.max(np.abs(p - exact)) <= 0.1  # 512 trajectories: a few standard errors
+    # the pin: trajectory t runs with seed S + t, so the oracle's applyErrorChannel with the same seeds gives
+    # the same 512 trajectories and the same average
+    o = comparison.Options.parse(["--circuit-json", str(tmp_path / f"{fid}.json"), "--fixture-id", fid,
+                                  "--category", f["category"]] + flags)
+    model = o.noise_model()
+    avg = np.zeros(1 << n)
+    for t in range(512):
+        s = oracle_noisy(circ, model, noise["seed"] + t)
+        avg += s.real * s.real + s.imag * s.imag
+    assert np.max(np.abs(p - avg / 512)) <= 1e-12
     # the same seeds give the same trajectories
     again = run_fixture(engine, tmp_path, fid, flags)
     assert again["probabilities"] == d["probabilities"]
@@ -132,3 +143,13 @@ def test_thermal_relaxation_fixture(engine, tmp_path):
     d = run_fixture(engine, tmp_path, "noise_thermal_relaxation_3q", flags)
     p = np.array(d["probabilities"])
     assert abs(p.sum() - 1.0) <= 1e-12 and p.min() >= 0.0
+    # the same 128 trajectories (seed S + t) through the oracle's applyErrorChannel
+    circ = comparison.parse_circuit_json(circuit_text(f))
+    o = comparison.Options.parse(["--circuit-json", str(tmp_path / "noise_thermal_relaxation_3q.json"),
+                                  "--fixture-id", "noise_thermal_relaxation_3q", "--category", f["category"]] + flags)
+    model = o.noise_model()
+    avg = np.zeros(1 << f["qubits"])
+    for t in range(128):
+        s = oracle_noisy(circ, model, noise["seed"] + t)
+        avg += s.real * s.real + s.imag * s.imag
+    assert np.max(np.abs(p - avg / 128)) <= 1e-12

@@ -65,3 +65,55 @@ def test_mfma_option_roundtrip(hip_ctx):
     with pytest.raises(capi.IllegalArgumentException):
         hip_ctx.set_option(capi.OPT_MFMA, 3)
     hip_ctx.set_option(capi.OPT_MFMA, 1)
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_mfma_tile_shapes_agree(hip_ctx, k):
+    """Every tile shape of SC_OPT_MFMA_TILE gives the same amplitudes bit for bit (one product per group in
+    the same order), within 1e-12 of the oracle; a batch of three 9-qubit states (an odd tile count with
+    5-qubit blocks) runs the two-tile shapes as one-tile steps."""
+    for n, batch in ((13, 1), (9, 3)):
+        rng = np.random.default_rng(4300 + 10 * n + k)
+        ops, mats = random_circuit(n, 120, rng, max_controls=1)
+        init = np.stack([random_state(n, rng) for _ in range(batch)])
+        outs = []
+        try:
+            for shape in range(6):
+                hip_ctx.set_option(capi.OPT_MFMA_TILE, shape)
+                outs.append(run_fused(hip_ctx, n, ops, mats, init.reshape(-1), k, mfma=True, batch=batch))
+        finally:
+            hip_ctx.set_option(capi.OPT_MFMA_TILE, -1)
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        got = outs[0].reshape(batch, -1)
+        for b in range(batch):
+            assert np.max(np.abs(got[b] - oracle.run_ops(n, ops, mats, init[b]))) <= EPS
+
+
+def test_mfma_tile_autotuner():
+    """With SC_OPT_MFMA_TILE -1 (the default) the first matrix-core block of a size times the six shapes and
+    keeps one for the context; the choice is readable and read-only, and the result matches the oracle."""
+    ctx = capi.Context(0)
+    try:
+        assert ctx.get_option(capi.OPT_MFMA_TILE) == -1
+        assert (ctx.get_option(capi.OPT_MFMA_PICK4), ctx.get_option(capi.OPT_MFMA_PICK5)) == (-1, -1)
+        n = 12
+        rng = np.random.default_rng(4400)
+        ops, mats = random_circuit(n, 80, rng, max_controls=1)
+        init = random_state(n, rng)
+        # library default SC_OPT_MFMA 1: only the 5-qubit blocks run on the matrix cores
+        ctx.set_option(capi.OPT_EXEC_MODE, capi.EXEC_FUSED)
+        ctx.set_option(capi.OPT_MAX_FUSED, 5)
+        st = capi.State(ctx, n)
+        st.upload(init)
+        st.apply(ops, mats)
+        got = st.download()
+        st.close()
+        assert np.max(np.abs(got - oracle.run_ops(n, ops, mats, init))) <= EPS
+        assert 0 <= ctx.get_option(capi.OPT_MFMA_PICK5) < 6 and ctx.get_option(capi.OPT_MFMA_PICK4) == -1
+        with pytest.raises(capi.IllegalArgumentException):
+            ctx.set_option(capi.OPT_MFMA_PICK5, 0)
+        with pytest.raises(capi.IllegalArgumentException):
+            ctx.set_option(capi.OPT_MFMA_TILE, 6)
+    finally:
+        ctx.close()
