@@ -7,7 +7,9 @@
 // choice runs on the device: per channel and target, one read of the state for every weight
 // (k_noise_weights), one workgroup per batch element to sum them, check them and pick the branch
 // (k_noise_select), and one sweep that applies the chosen operator already divided by sqrt(w_j)
-// (k_noise_apply; ||K_j psi||^2 is w_j, the norm the reference recomputes after the apply).  Nothing returns
+// (k_noise_apply; ||K_j psi||^2 is w_j, the norm the reference recomputes after the apply).  Channels that
+// follow the same op share sweeps: the apply of one and the weights of the next are one read and one write
+// (k_noise_apply_weights).  Nothing returns
 // to the host until the circuit ends; the ops between channels run as per-op kernels when there are few of
 // them and through the context's execution mode otherwise.
 #include <cmath>
@@ -195,41 +197,139 @@ __global__ void __launch_bounds__(kBlock) k_noise_apply(double2 *__restrict__ am
     }
 }
 
+// The previous event's chosen operator on `ta` and, from the updated amplitudes, the partial weights of the
+// next event's operators on `tb` (the two events follow the same op): one read and one write of the state
+// instead of a write pass and a read pass.  With ta == tb a thread takes the pair the two share; otherwise
+// the four amplitudes of a (ta, tb) quad, whose two ta-pairs are updated before its two tb-pairs are weighed.
+// Same arithmetic per pair as k_noise_apply and k_noise_weights; the partials are laid out as
+// k_noise_weights writes them (the sums run in another order).
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_noise_apply_weights(double2 *__restrict__ amp, int n, int ta,
+                                                                const double *__restrict__ sel, int tb, Mat2 k0,
+                                                                Mat2 k1, Mat2 k2, Mat2 k3,
+                                                                double *__restrict__ partial) {
+    const Mat2 *ks[4] = {&k0, &k1, &k2, &k3};
+    double2 *base = amp + ((uint64_t)blockIdx.y << n);
+    const double *m = sel + 8 * (size_t)blockIdx.y;
+    double s[8];
+#pragma unroll
+    for (int x = 0; x < 8; x++) s[x] = m[x];
+    double acc[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) acc[j] = 0.0;
+    auto apply = [&](double2 &a0, double2 &a1) {
+        double2 n0, n1;
+        n0.x = s[0] * a0.x - s[1] * a0.y;
+        n0.y = s[0] * a0.y + s[1] * a0.x;
+        n0.x = fma(s[2], a1.x, n0.x);
+        n0.x = fma(-s[3], a1.y, n0.x);
+        n0.y = fma(s[2], a1.y, n0.y);
+        n0.y = fma(s[3], a1.x, n0.y);
+        n1.x = s[4] * a0.x - s[5] * a0.y;
+        n1.y = s[4] * a0.y + s[5] * a0.x;
+        n1.x = fma(s[6], a1.x, n1.x);
+        n1.x = fma(-s[7], a1.y, n1.x);
+        n1.y = fma(s[6], a1.y, n1.y);
+        n1.y = fma(s[7], a1.x, n1.y);
+        a0 = n0;
+        a1 = n1;
+    };
+    auto weigh = [&](const double2 a0, const double2 a1) {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const double *g = ks[j]->v;
+            const double r0 = g[0] * a0.x - g[1] * a0.y + g[2] * a1.x - g[3] * a1.y;
+            const double i0 = g[0] * a0.y + g[1] * a0.x + g[2] * a1.y + g[3] * a1.x;
+            const double r1 = g[4] * a0.x - g[5] * a0.y + g[6] * a1.x - g[7] * a1.y;
+            const double i1 = g[4] * a0.y + g[5] * a0.x + g[6] * a1.y + g[7] * a1.x;
+            acc[j] += r0 * r0 + i0 * i0 + r1 * r1 + i1 * i1;
+        }
+    };
+    const uint64_t sa = 1ULL << ta, sb = 1ULL << tb;
+    const uint64_t step = (uint64_t)gridDim.x * kBlock;
+    if (ta == tb) {
+        const uint64_t pairs = 1ULL << (n - 1);
+        for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < pairs; i += step) {
+            const uint64_t lo = i & (sa - 1ULL);
+            const uint64_t i0 = ((i ^ lo) << 1) | lo;
+            double2 a0 = base[i0], a1 = base[i0 | sa];
+            apply(a0, a1);
+            base[i0] = a0;
+            base[i0 | sa] = a1;
+            weigh(a0, a1);
+        }
+    } else {
+        const int lo_bit = ta < tb ? ta : tb, hi_bit = ta < tb ? tb : ta;
+        const uint64_t quads = 1ULL << (n - 2);
+        for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < quads; i += step) {
+            uint64_t q = i;
+            uint64_t lo = q & ((1ULL << lo_bit) - 1ULL);
+            q = ((q ^ lo) << 1) | lo;
+            lo = q & ((1ULL << hi_bit) - 1ULL);
+            const uint64_t b = ((q ^ lo) << 1) | lo;
+            double2 x00 = base[b], x10 = base[b | sa], x01 = base[b | sb], x11 = base[b | sa | sb];
+            apply(x00, x10);  // pairs along ta
+            apply(x01, x11);
+            base[b] = x00;
+            base[b | sa] = x10;
+            base[b | sb] = x01;
+            base[b | sa | sb] = x11;
+            weigh(x00, x01);  // pairs along tb
+            weigh(x10, x11);
+        }
+    }
+    double r[K];
+    block_sum<K>(acc, r);
+    if (threadIdx.x == 0) {
+        double *out = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * K;
+#pragma unroll
+        for (int j = 0; j < K; j++) out[j] = r[j];
+    }
+}
+
 unsigned grid_for(Context *ctx, uint64_t total) {
     uint64_t blocks = (total + kBlock - 1) / kBlock;
     const uint64_t cap = (uint64_t)ctx->cu_count * 16;
     return (unsigned)std::max<uint64_t>(1, std::min(blocks, cap));
 }
 
-int launch_event(Context *ctx, State *st, const sc_noise_event &ev, const double *kraus, uint64_t e, uint64_t n_events,
-                 int bpe, double *d_uniforms, double *d_partial, double *d_sel, NoiseFailDev *d_fail) {
-    Mat2 m[kMaxKraus];
+void event_ops(const sc_noise_event &ev, const double *kraus, Mat2 (&m)[kMaxKraus]) {
     for (int j = 0; j < kMaxKraus; j++)
         for (int x = 0; x < 8; x++) m[j].v[x] = j < ev.n_kraus ? kraus[2 * ev.kraus + 8 * j + x] : 0.0;
-    const uint64_t pairs_per_elem = 1ULL << (st->n - 1);
-    {
-        ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 16.0 * 2.0 * (double)pairs_per_elem * st->batch);
-        const dim3 grid((unsigned)bpe, (unsigned)st->batch);
-        switch (ev.n_kraus) {
-        case 1:
-            hipLaunchKernelGGL(k_noise_weights<1>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, ev.target, m[0],
-                               m[1], m[2], m[3], d_partial);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_noise_weights<2>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, ev.target, m[0],
-                               m[1], m[2], m[3], d_partial);
-            break;
-        case 3:
-            hipLaunchKernelGGL(k_noise_weights<3>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, ev.target, m[0],
-                               m[1], m[2], m[3], d_partial);
-            break;
-        default:
-            hipLaunchKernelGGL(k_noise_weights<4>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, ev.target, m[0],
-                               m[1], m[2], m[3], d_partial);
-            break;
-        }
-        SC_HIP_CHECK(hipGetLastError());
+}
+
+// Partial weights of event `ev` into d_partial: alone (prev < 0), or fused with the apply of the chosen
+// operator of the previous event on qubit `prev`.
+int launch_weights(Context *ctx, State *st, const sc_noise_event &ev, const double *kraus, int prev, int bpe,
+                   double *d_partial, const double *d_sel) {
+    Mat2 m[kMaxKraus];
+    event_ops(ev, kraus, m);
+    const double amps = (double)((uint64_t)st->batch << st->n);
+    ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, (prev < 0 ? 16.0 : 32.0) * amps);
+    const dim3 grid((unsigned)bpe, (unsigned)st->batch);
+#define SC_NOISE_WEIGHTS(K)                                                                                       \
+    if (prev < 0)                                                                                                 \
+        hipLaunchKernelGGL(k_noise_weights<K>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, ev.target, m[0], \
+                           m[1], m[2], m[3], d_partial);                                                          \
+    else                                                                                                          \
+        hipLaunchKernelGGL(k_noise_apply_weights<K>, grid, dim3(kBlock), 0, ctx->stream, st->amp, st->n, prev,      \
+                           d_sel, ev.target, m[0], m[1], m[2], m[3], d_partial);
+    switch (ev.n_kraus) {
+    case 1: SC_NOISE_WEIGHTS(1) break;
+    case 2: SC_NOISE_WEIGHTS(2) break;
+    case 3: SC_NOISE_WEIGHTS(3) break;
+    default: SC_NOISE_WEIGHTS(4) break;
     }
+#undef SC_NOISE_WEIGHTS
+    SC_HIP_CHECK(hipGetLastError());
+    return SC_OK;
+}
+
+int launch_select(Context *ctx, State *st, const sc_noise_event &ev, const double *kraus, uint64_t e,
+                  uint64_t n_events, int bpe, const double *d_uniforms, const double *d_partial, double *d_sel,
+                  NoiseFailDev *d_fail) {
+    Mat2 m[kMaxKraus];
+    event_ops(ev, kraus, m);
     const dim3 sgrid((unsigned)st->batch);
     switch (ev.n_kraus) {
     case 1:
@@ -250,10 +350,14 @@ int launch_event(Context *ctx, State *st, const sc_noise_event &ev, const double
         break;
     }
     SC_HIP_CHECK(hipGetLastError());
-    const uint64_t total = pairs_per_elem * st->batch;
+    return SC_OK;
+}
+
+int launch_apply(Context *ctx, State *st, int target, const double *d_sel) {
+    const uint64_t total = ((uint64_t)st->batch << st->n) / 2;
     ProfScope ps(&ctx->prof, ctx->stream, SC_KFAM_NOISE, 32.0 * 2.0 * (double)total);
     hipLaunchKernelGGL(k_noise_apply, dim3(grid_for(ctx, total)), dim3(kBlock), 0, ctx->stream, st->amp, st->n,
-                       ev.target, total, d_sel);
+                       target, total, d_sel);
     SC_HIP_CHECK(hipGetLastError());
     return SC_OK;
 }
@@ -310,10 +414,19 @@ int run_noisy(State *st, const sc_op *ops, uint64_t n_ops, const double *mats, u
         if (seg.size() <= kDirectOps) return exec_direct(ctx, st, seg, mats);
         return apply_planned(st, seg, {mats}, false);
     };
+    // per event: weights, choice, apply.  An event that follows the same op as the one before it takes its
+    // weights in the same sweep as the previous event's apply.
+    bool pending = false;  // the previous event's operator is chosen (d_sel) but not applied yet
     for (uint64_t e = 0; e < n_events; e++) {
-        if ((rc = segment(events[e].after_op + 1))) return rc;
-        if ((rc = launch_event(ctx, st, events[e], kraus, e, n_events, bpe, d_uni, d_part, d_sel, d_fail))) return rc;
+        const sc_noise_event &ev = events[e];
+        const bool fuse = pending && ev.after_op == events[e - 1].after_op;
+        if (pending && !fuse && (rc = launch_apply(ctx, st, events[e - 1].target, d_sel))) return rc;
+        if (!fuse && (rc = segment(ev.after_op + 1))) return rc;
+        if ((rc = launch_weights(ctx, st, ev, kraus, fuse ? events[e - 1].target : -1, bpe, d_part, d_sel))) return rc;
+        if ((rc = launch_select(ctx, st, ev, kraus, e, n_events, bpe, d_uni, d_part, d_sel, d_fail))) return rc;
+        pending = true;
     }
+    if (pending && (rc = launch_apply(ctx, st, events[n_events - 1].target, d_sel))) return rc;
     if ((rc = segment(n_ops))) return rc;
     NoiseFailDev rec;
     SC_HIP_CHECK(hipMemcpyAsync(&rec, d_fail, sizeof rec, hipMemcpyDeviceToHost, ctx->stream));

@@ -133,6 +133,32 @@ def test_run_noisy_matches_oracle(n, B, every, jit):
         ctx.close()
 
 
+@pytest.mark.parametrize("n", [5, 13])
+def test_run_noisy_channels_after_one_op(hip_ctx, n):
+    """Several channels after one op share sweeps (the apply of one with the weights of the next): on the same
+    qubit, on another qubit below and above it, then a channel after a later op; against the oracle (1e-12)."""
+    rng = np.random.default_rng(9500 + n)
+    ops, mats = random_circuit(n, 12, rng, max_controls=1)
+    ad = list(ErrorChannel.amplitude_damping(0.35, 0).kraus)
+    pd = list(ErrorChannel.phase_damping(0.5, 0).kraus)
+    dep = list(ErrorChannel.depolarizing(0.4, 0).kraus)
+    kraus = ad + pd + dep
+    sched = [(2, 3, ad), (2, 3, pd), (2, n - 1, dep), (2, 0, ad), (2, 0, ad), (7, 1, pd), (7, n - 2, ad)]
+    first = {id(ad): 0, id(pd): 2, id(dep): 4}
+    events = [(i, t, first[id(k)], len(k)) for i, t, k in sched]
+    per = [(i, t, k) for i, t, k in sched]
+    for seed in (31, 32, 33):
+        u = SplittableRandom(seed).next_doubles(len(events)).reshape(1, -1)
+        init = random_state(n, rng)
+        st = capi.State(hip_ctx, n, 1)
+        st.upload(init)
+        st.run_noisy(ops, mats, events, kraus, u)
+        got = st.download()
+        st.close()
+        want = _oracle_schedule(n, ops, mats, per, seed, init)
+        assert np.max(np.abs(got - want)) <= EPS, seed
+
+
 def test_run_noisy_reports_the_first_failing_channel(hip_ctx):
     """A channel whose weights are all zero (total <= 1e-15) or not a number fails the run with the event, its
     weights and the kind (the reference's IllegalStateException); later calls on the context still work."""
