@@ -894,7 +894,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
             if (ops[i].mat >= 0 && ops[i].kind != SC_OP_CNOT && ops[i].kind != SC_OP_SWAP)
                 inverse_matrix(ops[i].kind, elems[e] + 2 * (int64_t)ops[i].mat,
                                inv.data() + 2 * (e * n_mat_complex + ops[i].mat));
-    DevBuf dinv, dlam, dckpt, dpart;
+    DevBuf dinv, dpart;
     if ((rc = dinv.alloc(inv.size() * sizeof(double)))) return rc;
     SC_HIP_CHECK(hipMemcpyAsync(dinv.p, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     const uint64_t inv_stride = mats_per_element ? 2 * n_mat_complex : 0;
@@ -902,7 +902,31 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     // ---- forward with checkpoints
     const uint64_t C = (uint64_t)checkpoint_every;
     uint64_t n_ckpt = (C > 0 && n_ops > 0) ? (n_ops - 1) / C : 0;
-    if (n_ckpt > 0 && (rc = dckpt.alloc(state_bytes * n_ckpt))) return rc;
+    // the context's workspace: n_ckpt checkpoints, lambda, and (when it fits) a copy of the forward output
+    bool saved = false;
+    {
+        const size_t need = state_bytes * (n_ckpt + 2), least = state_bytes * (n_ckpt + 1);
+        if (ctx->adj_ws_bytes < need) {
+            SC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            if (ctx->adj_ws) (void)hipFree(ctx->adj_ws);
+            ctx->adj_ws = nullptr;
+            ctx->adj_ws_bytes = 0;
+            for (size_t want : {need, least}) {
+                if (hipMalloc(&ctx->adj_ws, want) == hipSuccess) {
+                    ctx->adj_ws_bytes = want;
+                    break;
+                }
+                (void)hipGetLastError();
+                ctx->adj_ws = nullptr;
+            }
+            if (!ctx->adj_ws)
+                return fail(SC_E_OOM, "device allocation of " + std::to_string(least) +
+                                          " bytes for the adjoint checkpoints and lambda failed");
+        }
+        saved = ctx->adj_ws_bytes >= need;
+    }
+    char *ws = (char *)ctx->adj_ws;
+    char *ckpt = ws;  // checkpoint k at ckpt + k * state_bytes
     {
         // segment k ends in checkpoint k, segment k + 1 starts from it: with runtime-compiled passes the
         // segment's last pass writes the checkpoint and the next segment's first pass reads it, no copies
@@ -910,7 +934,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         const double2 *cur = st->amp;
         for (uint64_t k = 0; k <= n_ckpt; k++) {
             const uint64_t end = (k < n_ckpt) ? (k + 1) * C : n_ops;
-            double2 *dst = k < n_ckpt ? (double2 *)((char *)dckpt.p + state_bytes * k) : st->amp;
+            double2 *dst = k < n_ckpt ? (double2 *)(ckpt + state_bytes * k) : st->amp;
             if (end > begin) {
                 rc = apply_ops_between(st, ops + begin, end - begin, mats, n_mat_complex,
                                        mats_per_element ? n_mat_complex : 0, mats_per_element != 0, cur, dst);
@@ -923,8 +947,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         }
     }
     // ---- lambda = H psi
-    if ((rc = dlam.alloc(state_bytes))) return rc;
-    double2 *lam = (double2 *)dlam.p;
+    double2 *lam = (double2 *)(ws + state_bytes * n_ckpt);
     const uint64_t total = B << n;
     {
         // terms grouped by their X mask, coefficients with i^ny folded in
@@ -989,11 +1012,13 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         if ((rc = finish_parts(ctx, epart, B, e.data()))) return rc;
         for (uint64_t b = 0; b < B; b++) energies[b] = e[2 * b];
     }
-    // ---- keep the forward output: it is what psi must hold when this call returns, and a copy is far cheaper
-    // than running the circuit again (which remains the fallback when the copy does not fit)
-    DevBuf dsave;
-    const bool saved = dsave.alloc(state_bytes) == SC_OK;
-    if (saved) SC_HIP_CHECK(hipMemcpyAsync(dsave.p, st->amp, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    // ---- the forward output is what psi must hold when this call returns: the reverse sweep uncomputes a copy
+    // of it (far cheaper than running the circuit again, which remains the fallback when the copy does not fit:
+    // then the sweep runs on psi itself)
+    double2 *dsave = (double2 *)(ws + state_bytes * (n_ckpt + 1));
+    if (saved) SC_HIP_CHECK(hipMemcpyAsync(dsave, st->amp, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    State work = *st;  // the state the reverse sweep uncomputes
+    if (saved) work.amp = dsave;
     // ---- reverse sweep
     ctx->adj_reanchors = 0;
     ctx->adj_max_drift = 0.0;
@@ -1001,15 +1026,15 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     const unsigned dgrid = grid_for(ctx, total);
     // psi should equal the state the forward run checkpointed after its segment k: re-anchor when it drifted
     auto check_drift = [&](uint64_t k) -> int {
-        double2 *ck = (double2 *)((char *)dckpt.p + state_bytes * k);
+        double2 *ck = (double2 *)(ckpt + state_bytes * k);
         if (ctx->test_drift > 0) {
             hipLaunchKernelGGL(k_add_to_first, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, ctx->stream,
-                               st->amp, n, (int)B, (double)ctx->test_drift * 1e-12);
+                               work.amp, n, (int)B, (double)ctx->test_drift * 1e-12);
             SC_HIP_CHECK(hipGetLastError());
         }
         int rc2 = ensure_scratch(ctx, (size_t)dgrid * sizeof(double));
         if (rc2) return rc2;
-        hipLaunchKernelGGL(k_maxdiff, dim3(dgrid), dim3(kBlock), 0, ctx->stream, st->amp, ck, total,
+        hipLaunchKernelGGL(k_maxdiff, dim3(dgrid), dim3(kBlock), 0, ctx->stream, work.amp, ck, total,
                            (double *)ctx->scratch);
         SC_HIP_CHECK(hipGetLastError());
         mx.resize(dgrid);
@@ -1019,7 +1044,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         double drift = *std::max_element(mx.begin(), mx.end());
         ctx->adj_max_drift = std::max(ctx->adj_max_drift, drift);
         if (!(drift <= 1e-10)) {
-            SC_HIP_CHECK(hipMemcpyAsync(st->amp, ck, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            SC_HIP_CHECK(hipMemcpyAsync(work.amp, ck, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
             ctx->adj_reanchors++;
         }
         return SC_OK;
@@ -1031,7 +1056,7 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
     if (fused) {
         std::vector<const double *> inv_elems;
         for (uint64_t e = 0; e < E; e++) inv_elems.push_back(inv.data() + 2 * e * n_mat_complex);
-        rc = fused_backward(ctx, st, lam, ops, n_ops, slot, n_slots, inv_elems, n_mat_complex, C, n_ckpt,
+        rc = fused_backward(ctx, &work, lam, ops, n_ops, slot, n_slots, inv_elems, n_mat_complex, C, n_ckpt,
                             check_drift, res.data());
         if (rc) return rc;
     } else {
@@ -1050,21 +1075,21 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
                 int target = op.kind == SC_OP_CNOT ? op.q1 : op.q0;
                 uint64_t controls = op.controls;
                 int lg = n - 1 - __builtin_popcountll(controls);
-                hipLaunchKernelGGL(k_adj_pair, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                hipLaunchKernelGGL(k_adj_pair, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, work.amp, lam,
                                    n, target, controls | (1ULL << target), controls, lg, dinvp, inv_stride,
                                    op.is_x ? 1 : 0, slot[g] >= 0 ? o.gen : 0, pslot);
                 break;
             }
             case SC_OP_TWO:
-                hipLaunchKernelGGL(k_adj_two, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                hipLaunchKernelGGL(k_adj_two, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, work.amp, lam,
                                    n, op.q0, op.q1, dinvp, inv_stride);
                 break;
             case SC_OP_DIAG2:
-                hipLaunchKernelGGL(k_adj_diag2, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                hipLaunchKernelGGL(k_adj_diag2, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, work.amp, lam,
                                    n, op.q0, op.q1, dinvp, inv_stride, slot[g] >= 0 ? o.gen : 0, pslot);
                 break;
             case SC_OP_SWAP:
-                hipLaunchKernelGGL(k_adj_swap, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, st->amp, lam,
+                hipLaunchKernelGGL(k_adj_swap, dim3(kParts, (unsigned)B), dim3(kBlock), 0, ctx->stream, work.amp, lam,
                                    n, op.q0, op.q1);
                 break;
             }
@@ -1079,10 +1104,8 @@ int sc_adjoint_gradient(sc_state *st, const sc_op *ops, uint64_t n_ops, const do
         for (uint64_t b = 0; b < B; b++)
             grads[b * n_params + ops[i].param] += 2.0 * ops[i].coeff * res[2 * ((uint64_t)slot[i] * B + b) + 1];
     }
-    // restore psi to the forward output (the reverse sweep uncomputed it)
-    if (saved) {
-        SC_HIP_CHECK(hipMemcpyAsync(st->amp, dsave.p, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    } else {
+    // psi holds the forward output unless the reverse sweep had to uncompute it in place
+    if (!saved) {
         set_error("");
         rc = apply_ops_impl(st, ops, n_ops, mats, n_mat_complex, mats_per_element ? n_mat_complex : 0,
                             mats_per_element != 0);

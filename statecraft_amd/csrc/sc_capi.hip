@@ -511,6 +511,7 @@ int sc_ctx_destroy(sc_ctx *ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->scratch_host) (void)hipHostFree(ctx->scratch_host);
     if (ctx->plan_dev) (void)hipFree(ctx->plan_dev);
+    if (ctx->adj_ws) (void)hipFree(ctx->adj_ws);
     destroy_job_graphs(ctx);
     delete ctx->plan_cache;
     (void)hipStreamDestroy(ctx->stream);
@@ -652,7 +653,17 @@ int sc_state_create(sc_ctx *ctx, int num_qubits, int batch, sc_state **out) {
     st->n = num_qubits;
     st->batch = batch;
     const size_t bytes = ((size_t)batch << num_qubits) * sizeof(double2);
-    if (hipMalloc((void **)&st->amp, bytes) != hipSuccess) {
+    hipError_t e = hipMalloc((void **)&st->amp, bytes);
+    if (e != hipSuccess && ctx->adj_ws) {
+        // the adjoint workspace kept from an earlier call gives way to a state
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->adj_ws);
+        ctx->adj_ws = nullptr;
+        ctx->adj_ws_bytes = 0;
+        e = hipMalloc((void **)&st->amp, bytes);
+    }
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(SC_E_OOM, "cannot allocate " + std::to_string(bytes) + " bytes of HBM for " +
                                   std::to_string(num_qubits) + " qubits x " + std::to_string(batch));

@@ -79,6 +79,10 @@ struct Context {
     // scratch (grown on demand, freed with the context)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // sc_adjoint_gradient's checkpoints, lambda and copy of psi, kept for the next call (allocating and freeing
+    // ~100 GiB per call has cost seconds); released by sc_ctx_destroy or when a state allocation needs the HBM
+    void *adj_ws = nullptr;
+    size_t adj_ws_bytes = 0;
     void *scratch_host = nullptr;  // pinned
     size_t scratch_host_bytes = 0;
     // device copy of a sweep plan's records and matrix pool (used by sweep execution only)
