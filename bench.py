@@ -565,6 +565,70 @@ def bench_c5(args):
                  adjoint_stats=ctx.adjoint_stats(), energy_0=float(energies[0]))
 
 
+def bench_noisy(args):
+    """State-dependent noise in one call (sc_run_noisy): GHZ-n with amplitude damping (gamma 0.02) and phase
+    damping (lambda 0.03) on both qubits after every CNOT, one trajectory per run, for n = 12, 20, 26; beside it
+    the per-channel loop the engine ran before (sc_kraus_weights, the branch chosen on the host,
+    sc_apply_kraus per channel and target) on the same uniforms, and the largest difference of the two."""
+    from statecraft_amd.noise import ErrorChannel
+    ad = list(ErrorChannel.amplitude_damping(0.02, 0).kraus)
+    pd = list(ErrorChannel.phase_damping(0.03, 0).kraus)
+    kraus = ad + pd
+    ctx = make_context(args)
+    sizes = {}
+    for n in (12, 20, 26):
+        ops, mats = lower(workloads.ghz(n))
+        events = [(i, q, f, 2) for i in range(len(ops)) if ops[i]["kind"] == capi.OP_CNOT
+                  for q in (int(ops[i]["q0"]), int(ops[i]["q1"])) for f in (0, 2)]
+        u = SplittableRandom(5).next_doubles(len(events)).reshape(1, -1)
+        st = capi.State(ctx, n, 1)
+
+        def device():
+            st.set_basis(0)
+            st.run_noisy(ops, mats, events, kraus, u)
+
+        def per_channel():
+            st.set_basis(0)
+            done = 0
+            for j, (i, t, f, c) in enumerate(events):
+                if i + 1 > done:
+                    st.apply(ops[done:i + 1], mats)
+                    done = i + 1
+                w = st.kraus_weights(t, kraus[f:f + c])
+                r, cum, idx = u[0, j] * float(sum(w)), 0.0, c - 1
+                for q, x in enumerate(w):
+                    cum += float(x)
+                    if r < cum:
+                        idx = q
+                        break
+                st.apply_kraus(t, kraus[f + idx])
+            if done < len(ops):
+                st.apply(ops[done:], mats)
+
+        times = {}
+        for name, fn in (("run_noisy", device), ("per_channel", per_channel)):
+            for _ in range(max(1, args.warmup)):
+                fn()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            ctx.synchronize()
+            times[name] = (time.perf_counter() - t0) / args.steps
+            if name == "run_noisy":
+                a = st.download()
+        diff = float(np.max(np.abs(st.download() - a)))
+        st.close()
+        sizes[str(n)] = {"channels": len(events), "run_noisy_ms": 1e3 * times["run_noisy"],
+                         "per_channel_ms": 1e3 * times["per_channel"],
+                         "speedup": times["per_channel"] / times["run_noisy"], "max_abs_diff": diff}
+        log(f"noisy {n} q: {sizes[str(n)]}")
+    ctx.close()
+    t = sizes["20"]["run_noisy_ms"] * 1e-3
+    return _line("noisy GHZ-20 runs/s (amplitude + phase damping after every CNOT, sc_run_noisy)", 1.0 / t, "runs/s",
+                 args, 1e3 * t, {"workload": "ghz-noisy-damping", "gamma": 0.02, "lambda": 0.03}, sizes=sizes)
+
+
 def other_configs(args):
     """Short runs of the other benchmark configurations at their full sizes, reported beside the headline:
     20-qubit Ry+QFT (3 steps), 33-qubit Grover (1 step), batched QAOA + GHZ-20 trajectories (2 steps); one
@@ -589,7 +653,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "noisy"])
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--trajectories", type=int, default=1024)
     ap.add_argument("--param-sets", type=int, default=64)
@@ -644,7 +708,8 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is designed, not built",
               file=sys.stderr)
         return 2
-    res = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](args)
+    res = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+           "noisy": bench_noisy}[args.workload](args)
     if args.workload == "c2" and not args.no_adjoint:
         # the metric's second half: adjoint gradients of the 24-qubit, 384-parameter HEA against parameter shift
         log("adjoint block (24-qubit HEA, 384 parameters, batch 64) ...")
