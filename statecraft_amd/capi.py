@@ -607,10 +607,11 @@ class State:
         for i, (after, target, first, count) in enumerate(events):
             ev[i] = NoiseEvent(int(after), int(target), int(count), 4 * int(first))
         u = np.ascontiguousarray(uniforms, np.float64).reshape(self.batch, len(events))
+        if not u.size:
+            u = np.zeros(1)  # not read without events; the library wants a pointer
         fail = NoiseFailure()
         rc = lib().sc_run_noisy(self.h, ops.ctypes.data_as(P), len(ops), dptr(m), m.size // 2,
-                                ctypes.cast(ev, P), len(events), dptr(k), k.size // 2,
-                                dptr(u) if u.size else dptr(np.zeros(1)), ctypes.byref(fail))
+                                ctypes.cast(ev, P), len(events), dptr(k), k.size // 2, dptr(u), ctypes.byref(fail))
         if rc == SC_E_STATE and fail.event >= 0:
             raise NoisyRunFailed(lib().sc_last_error().decode("utf-8", "replace"), fail)
         check(rc)

@@ -13,6 +13,7 @@
 // to the host until the circuit ends; the ops between channels run as per-op kernels when there are few of
 // them and through the context's execution mode otherwise.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>

@@ -23,8 +23,6 @@ from .circuit import QuantumCircuit, lower
 from .noise import CompositeChannel, NoiseModel
 from .rng import SplittableRandom
 
-MIN_KRAUS_WEIGHT = 1e-15  # StatevectorEngine.java:36
-
 
 # ---------------------------------------------------------------- request / result types
 @dataclass(frozen=True)
